@@ -1,0 +1,163 @@
+/*
+ * rnpx.h -- C ABI of the MI355X (gfx950) blocked Householder QR that drops
+ * in for RNP2's LA::QR path.
+ *
+ * Conventions are RNP's (RNP/LA/QR.hpp): column-major, 0-based, size_t
+ * sizes, complex numbers interleaved like std::complex<double>, side/trans
+ * read from the first character ("L"/"R", "N"/"C").  Every function returns
+ * an int status: 0 on success, -i when argument i (1-based) is invalid
+ * (LAPACK style), or one of the RNPX_ERR_* codes below.
+ *
+ * Two kinds of entry point:
+ *  - drop-in ones (rnpx_dgeqrf, ...) take RNP's exact argument list.  The
+ *    matrix pointers may be host memory (staged to the GPU and back inside
+ *    the call) or device memory (used in place; the call still returns only
+ *    when the result is in place).  The workspace query (work == NULL or
+ *    *lwork == 0) returns the same size the reference returns.  work itself
+ *    is not touched: the GPU keeps its own workspace.
+ *  - handle ones (rnpx_*_dev, rnpx_ls_*, batched, serving) take device
+ *    pointers, run on the handle's HIP stream and return without waiting.
+ */
+#ifndef RNPX_H
+#define RNPX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rnpx_zcomplex {
+	double re, im;
+} rnpx_zcomplex;
+
+typedef struct rnpx_handle_s *rnpx_handle;
+
+enum {
+	RNPX_OK = 0,
+	RNPX_ERR_HIP = -1000,       /* a HIP runtime call failed */
+	RNPX_ERR_NOGPU = -1001,     /* no usable gfx950 device */
+	RNPX_ERR_NOMEM = -1002,     /* device allocation failed / over budget */
+	RNPX_ERR_SYNC = -1003,      /* a grid barrier timed out (results invalid) */
+	RNPX_ERR_UNSUPPORTED = -1004, /* shape/option outside what is built */
+	RNPX_ERR_NOTCONV = -1005    /* iterative refinement did not reach its target */
+};
+
+/* Tuning hook: the counterpart of specialising QR::Tuning<T>
+ * (RNP/LA/QR.hpp:41-51).  Zero fields keep the built-in choice. */
+typedef struct rnpx_tuning {
+	int nb;           /* panel width: 32, 64 (128 real) */
+	int panel_rpt;    /* panel rows per thread: 1, 2, 4 */
+	int ksplit;       /* split-K slices for W = V^H C (0 = automatic) */
+	int lookahead;    /* 1 = overlap the next panel with the trailing update */
+} rnpx_tuning;
+
+/* ---------------------------------------------------------------------
+ * Drop-in entry points (host or device pointers).
+ * ------------------------------------------------------------------- */
+
+/* QR::Factor<double> -- RNP/LA/QR.hpp:129-197 (dgeqrf). */
+int rnpx_dgeqrf(size_t m, size_t n, double *a, size_t lda, double *tau,
+                size_t *lwork, double *work);
+/* QR::Factor<std::complex<double>> -- RNP/LA/QR.hpp:129-197 (zgeqrf). */
+int rnpx_zgeqrf(size_t m, size_t n, rnpx_zcomplex *a, size_t lda, rnpx_zcomplex *tau,
+                size_t *lwork, rnpx_zcomplex *work);
+
+/* QR::MultQ<double> -- RNP/LA/QR.hpp:337-429 (dormqr). */
+int rnpx_dormqr(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                const double *a, size_t lda, const double *tau, double *c, size_t ldc,
+                size_t *lwork, double *work);
+/* QR::MultQ<std::complex<double>> -- RNP/LA/QR.hpp:337-429 (zunmqr). */
+int rnpx_zunmqr(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                const rnpx_zcomplex *a, size_t lda, const rnpx_zcomplex *tau,
+                rnpx_zcomplex *c, size_t ldc, size_t *lwork, rnpx_zcomplex *work);
+
+/* QR::GenerateQ<double> -- RNP/LA/QR.hpp:509-570 (dorgqr). */
+int rnpx_dorgqr(size_t m, size_t n, size_t k, double *a, size_t lda, const double *tau,
+                size_t *lwork, double *work);
+/* QR::GenerateQ<std::complex<double>> -- RNP/LA/QR.hpp:509-570 (zungqr). */
+int rnpx_zungqr(size_t m, size_t n, size_t k, rnpx_zcomplex *a, size_t lda,
+                const rnpx_zcomplex *tau, size_t *lwork, rnpx_zcomplex *work);
+
+/* ---------------------------------------------------------------------
+ * Handles (device-resident work on one HIP stream).
+ * ------------------------------------------------------------------- */
+
+/* Create a handle on the current device with its own stream.
+ * hbm_budget_bytes bounds the handle's own workspace (0 = unbounded). */
+int rnpx_create(rnpx_handle *h, size_t hbm_budget_bytes);
+int rnpx_destroy(rnpx_handle h);
+/* Use an existing hipStream_t (NULL = the handle's own stream). */
+int rnpx_set_stream(rnpx_handle h, void *hip_stream);
+void *rnpx_get_stream(rnpx_handle h);
+int rnpx_set_tuning(rnpx_handle h, const rnpx_tuning *t);
+int rnpx_get_tuning(rnpx_handle h, rnpx_tuning *t);
+/* Wait for the handle's stream; reports RNPX_ERR_SYNC if a panel barrier
+ * timed out in any call since the last check. */
+int rnpx_synchronize(rnpx_handle h);
+/* Bytes of device workspace the handle currently holds. */
+size_t rnpx_workspace_bytes(rnpx_handle h);
+
+/* Live kernel timing on the handle's stream (HIP events around each launch
+ * of a kernel family).  kernel: 0 trailing update C -= V Z, 1 W = V^H C,
+ * 2 panel, 3 T factor.  flops/bytes are the algorithmic counts of the
+ * recorded launches. */
+int rnpx_profile_enable(rnpx_handle h, int on);
+int rnpx_profile_read(rnpx_handle h, int kernel, long *launches, double *ms, double *flops,
+                      double *bytes);
+
+/* Device-pointer forms of Factor / MultQ / GenerateQ (same semantics). */
+int rnpx_dgeqrf_dev(rnpx_handle h, size_t m, size_t n, double *a, size_t lda, double *tau);
+int rnpx_zgeqrf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex *a, size_t lda,
+                    rnpx_zcomplex *tau);
+int rnpx_dormqr_dev(rnpx_handle h, const char *side, const char *trans, size_t m, size_t n,
+                    size_t k, const double *a, size_t lda, const double *tau, double *c,
+                    size_t ldc);
+int rnpx_zunmqr_dev(rnpx_handle h, const char *side, const char *trans, size_t m, size_t n,
+                    size_t k, const rnpx_zcomplex *a, size_t lda, const rnpx_zcomplex *tau,
+                    rnpx_zcomplex *c, size_t ldc);
+int rnpx_dorgqr_dev(rnpx_handle h, size_t m, size_t n, size_t k, double *a, size_t lda,
+                    const double *tau);
+int rnpx_zungqr_dev(rnpx_handle h, size_t m, size_t n, size_t k, rnpx_zcomplex *a, size_t lda,
+                    const rnpx_zcomplex *tau);
+
+/* Building blocks, device pointers (the reference's Reflector functions):
+ * T factor of k <= nb forward columnwise reflectors stored in a (n-by-k,
+ * unit diagonal implicit) -- Reflector::GenerateBlockTr("F","C"),
+ * RNP/LA/Reflector.hpp:573-626.  t is k-by-k with leading dimension ldt. */
+int rnpx_dlarft_dev(rnpx_handle h, size_t n, size_t k, const double *v, size_t ldv,
+                    const double *tau, double *t, size_t ldt);
+int rnpx_zlarft_dev(rnpx_handle h, size_t n, size_t k, const rnpx_zcomplex *v, size_t ldv,
+                    const rnpx_zcomplex *tau, rnpx_zcomplex *t, size_t ldt);
+/* Reflector::ApplyBlock(side, trans, "F", "C", ...) -- Reflector.hpp:725-815. */
+int rnpx_dlarfb_dev(rnpx_handle h, const char *side, const char *trans, size_t m, size_t n,
+                    size_t k, const double *v, size_t ldv, const double *t, size_t ldt,
+                    double *c, size_t ldc);
+int rnpx_zlarfb_dev(rnpx_handle h, const char *side, const char *trans, size_t m, size_t n,
+                    size_t k, const rnpx_zcomplex *v, size_t ldv, const rnpx_zcomplex *t,
+                    size_t ldt, rnpx_zcomplex *c, size_t ldc);
+
+/* C = alpha op(A) op(B) + beta C on FP64 MFMA, op in "N","T","C" (the
+ * MultMM the reference forwards to xgemm, src/BLAS_mix.cpp:498-517). */
+int rnpx_dgemm_dev(rnpx_handle h, const char *transa, const char *transb, size_t m, size_t n,
+                   size_t k, double alpha, const double *a, size_t lda, const double *b,
+                   size_t ldb, double beta, double *c, size_t ldc);
+int rnpx_zgemm_dev(rnpx_handle h, const char *transa, const char *transb, size_t m, size_t n,
+                   size_t k, rnpx_zcomplex alpha, const rnpx_zcomplex *a, size_t lda,
+                   const rnpx_zcomplex *b, size_t ldb, rnpx_zcomplex beta, rnpx_zcomplex *c,
+                   size_t ldc);
+
+/* RNP's random generator on the device (RNP/Random.hpp:45-320): fills
+ * count values of the stream that starts at iseed (column-major fill of a
+ * matrix = one stream).  dist: 0 Uniform01, 1 Uniform_11, 2 Normal01. */
+int rnpx_dlarnv_dev(rnpx_handle h, int dist, const int iseed[4], size_t count, double *x);
+int rnpx_zlarnv_dev(rnpx_handle h, int dist, const int iseed[4], size_t count, rnpx_zcomplex *x);
+
+/* Library version string and the device it runs on. */
+const char *rnpx_version(void);
+int rnpx_device_name(char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,159 @@
+/*
+ * rnp_oracle.h -- CPU restatement of RNP2's LA::QR path (test infrastructure).
+ *
+ * THIS IS THE ORACLE, NOT THE PRODUCT.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it, and only as the checker (or as
+ * the timed CPU baseline).  The product path (rnp2_amd/, include/) never links
+ * or calls anything here.
+ *
+ * What it restates (reference file:line):
+ *   QR::Tuning            RNP/LA/QR.hpp:41-51       (orc_tuning, defaults 32/2/128)
+ *   QR::Factor_unblocked  RNP/LA/QR.hpp:78-102
+ *   QR::Factor            RNP/LA/QR.hpp:129-197
+ *   QR::MultQ_unblocked   RNP/LA/QR.hpp:234-298
+ *   QR::MultQ             RNP/LA/QR.hpp:337-429
+ *   QR::GenerateQ_unblocked RNP/LA/QR.hpp:451-485
+ *   QR::GenerateQ         RNP/LA/QR.hpp:509-570
+ *   Reflector::LastNonzeroColumnLimit/RowLimit  RNP/LA/Reflector.hpp:36-75
+ *   Reflector::Generate   RNP/LA/Reflector.hpp:117-158
+ *   Reflector::Apply      RNP/LA/Reflector.hpp:191-360 (vone in {0,1}, vconj=false:
+ *                         the branches QR reaches; vconj/bottom-one serve LQ/QL/RQ)
+ *   Reflector::GenerateBlockTr RNP/LA/Reflector.hpp:573-626 (dir "F", storev "C")
+ *   Reflector::ApplyBlock RNP/LA/Reflector.hpp:725-815 (dir "F", storev "C", side L/R)
+ *   BLAS::Norm2           RNP/BLAS/BLAS_mix.hpp:725-758 (one-pass scaled sum of squares)
+ *   Traits<real>::hypot3  RNP/Types.hpp:73-92
+ *   Level-2/3 BLAS (gemv, gerc, trmv, gemm, trmm, trsm): the external Fortran
+ *     BLAS that src/BLAS_mix.cpp:104-693 forwards to; restated from the BLAS
+ *     reference semantics (beta == 0 clears y/C).
+ *   Random::UniformRealVector / GenerateVector  RNP/Random.hpp:45-320
+ *   MatrixGen::GenerateDiagonal / RandomMatrix (Nonsymmetric)  RNP/MatrixGen.hpp:31-198
+ *
+ * Complex numbers are C99 double complex, laid out like std::complex<double>
+ * (interleaved re, im).  All sizes are size_t, matrices column-major, 0-based.
+ * Functions return void like the reference; bad arguments print an
+ * "Assertion failed" line to stderr and continue (RNP/Debug.hpp:9-17).
+ */
+#ifndef RNP_ORACLE_H
+#define RNP_ORACLE_H
+
+#include <stddef.h>
+#include <complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double complex orc_z;
+
+/* Block-size tuning, QR::Tuning<T> (QR.hpp:41-51).  A NULL pointer means the
+ * reference defaults: nb = 32, nbmin = 2, crossover nx = 128. */
+typedef struct orc_tuning {
+	size_t nb;
+	size_t nbmin;
+	size_t nx;
+} orc_tuning;
+
+/* ---- level-1 / reflector pieces ---- */
+double orc_d_norm2(size_t n, const double *x, size_t incx);
+double orc_z_norm2(size_t n, const orc_z *x, size_t incx);
+double orc_hypot3(double x, double y, double z);
+
+void orc_d_generate(size_t n, double *alpha, double *x, size_t incx, double *tau);
+void orc_z_generate(size_t n, orc_z *alpha, orc_z *x, size_t incx, orc_z *tau);
+
+size_t orc_d_last_nonzero_column_limit(size_t m, size_t n, const double *a, size_t lda);
+size_t orc_d_last_nonzero_row_limit(size_t m, size_t n, const double *a, size_t lda);
+size_t orc_z_last_nonzero_column_limit(size_t m, size_t n, const orc_z *a, size_t lda);
+size_t orc_z_last_nonzero_row_limit(size_t m, size_t n, const orc_z *a, size_t lda);
+
+void orc_d_apply(const char *side, int vone, size_t m, size_t n, const double *v, size_t incv,
+                 double tau, double *c, size_t ldc, double *work);
+void orc_z_apply(const char *side, int vone, size_t m, size_t n, const orc_z *v, size_t incv,
+                 orc_z tau, orc_z *c, size_t ldc, orc_z *work);
+
+void orc_d_generate_block_tr(size_t n, size_t k, const double *v, size_t ldv, const double *tau,
+                             double *t, size_t ldt);
+void orc_z_generate_block_tr(size_t n, size_t k, const orc_z *v, size_t ldv, const orc_z *tau,
+                             orc_z *t, size_t ldt);
+
+void orc_d_apply_block(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                       const double *v, size_t ldv, const double *t, size_t ldt,
+                       double *c, size_t ldc, double *work, size_t ldwork);
+void orc_z_apply_block(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                       const orc_z *v, size_t ldv, const orc_z *t, size_t ldt,
+                       orc_z *c, size_t ldc, orc_z *work, size_t ldwork);
+
+/* ---- QR ---- */
+void orc_d_factor_unblocked(size_t m, size_t n, double *a, size_t lda, double *tau, double *work);
+void orc_z_factor_unblocked(size_t m, size_t n, orc_z *a, size_t lda, orc_z *tau, orc_z *work);
+void orc_d_factor(size_t m, size_t n, double *a, size_t lda, double *tau, size_t *lwork,
+                  double *work, const orc_tuning *tune);
+void orc_z_factor(size_t m, size_t n, orc_z *a, size_t lda, orc_z *tau, size_t *lwork,
+                  orc_z *work, const orc_tuning *tune);
+/* Factor stopped after the first `npanels` blocked panels (the CPU-baseline
+ * sample: the same loop as orc_d_factor, QR.hpp:173-190, cut short). */
+void orc_d_factor_panels(size_t m, size_t n, double *a, size_t lda, double *tau,
+                         double *work, size_t nb, size_t npanels);
+
+void orc_d_multq_unblocked(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                           const double *a, size_t lda, const double *tau, double *c, size_t ldc,
+                           double *work);
+void orc_z_multq_unblocked(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                           const orc_z *a, size_t lda, const orc_z *tau, orc_z *c, size_t ldc,
+                           orc_z *work);
+void orc_d_multq(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                 const double *a, size_t lda, const double *tau, double *c, size_t ldc,
+                 size_t *lwork, double *work, const orc_tuning *tune);
+void orc_z_multq(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                 const orc_z *a, size_t lda, const orc_z *tau, orc_z *c, size_t ldc,
+                 size_t *lwork, orc_z *work, const orc_tuning *tune);
+
+void orc_d_generateq_unblocked(size_t m, size_t n, size_t k, double *a, size_t lda,
+                               const double *tau, double *work);
+void orc_z_generateq_unblocked(size_t m, size_t n, size_t k, orc_z *a, size_t lda,
+                               const orc_z *tau, orc_z *work);
+void orc_d_generateq(size_t m, size_t n, size_t k, double *a, size_t lda, const double *tau,
+                     size_t *lwork, double *work, const orc_tuning *tune);
+void orc_z_generateq(size_t m, size_t n, size_t k, orc_z *a, size_t lda, const orc_z *tau,
+                     size_t *lwork, orc_z *work, const orc_tuning *tune);
+
+/* Least squares min ||A x - B|| for m >= n by Factor, MultQ("L","C") and an
+ * upper-triangular back substitution with R (BLAS dtrsm "L","U","N","N";
+ * the composition of QR.hpp:129, QR.hpp:337 and BLAS_mix.hpp:2096).
+ * A is overwritten by its factorization, B (m-by-nrhs) by Q^H B with the
+ * solution in its first n rows. */
+void orc_d_ls_solve(size_t m, size_t n, size_t nrhs, double *a, size_t lda, double *b, size_t ldb);
+void orc_z_ls_solve(size_t m, size_t n, size_t nrhs, orc_z *a, size_t lda, orc_z *b, size_t ldb);
+
+/* ---- BLAS restatement (column-major, Fortran BLAS semantics) ---- */
+void orc_d_gemm(const char *transa, const char *transb, size_t m, size_t n, size_t k,
+                double alpha, const double *a, size_t lda, const double *b, size_t ldb,
+                double beta, double *c, size_t ldc);
+void orc_z_gemm(const char *transa, const char *transb, size_t m, size_t n, size_t k,
+                orc_z alpha, const orc_z *a, size_t lda, const orc_z *b, size_t ldb,
+                orc_z beta, orc_z *c, size_t ldc);
+void orc_set_threads(int nthreads);
+int orc_get_threads(void);
+
+/* ---- harness: RNP's generators ---- */
+enum { ORC_UNIFORM01 = 0, ORC_UNIFORM_11 = 1, ORC_NORMAL01 = 2, ORC_UNITDISC = 3, ORC_UNITCIRCLE = 4 };
+/* iseed == NULL uses the internal static seed {2398,691,782,721} (Random.hpp:126). */
+void orc_uniform_real_vector(size_t n, double *x, int *iseed);
+void orc_reset_internal_seed(void);
+void orc_d_generate_vector(int dist, size_t n, double *x, int *iseed);
+void orc_z_generate_vector(int dist, size_t n, orc_z *x, int *iseed);
+
+enum { ORC_DIAG_GIVEN = 0, ORC_DIAG_FIRST1_REMAINING_COND = 1, ORC_DIAG_LASTCOND_REMAINING1 = 2,
+       ORC_DIAG_GRADED_EXPONENTIALLY = 3, ORC_DIAG_GRADED_LINEARLY = 4,
+       ORC_DIAG_RANDOM_EXPONENTIAL = 5, ORC_DIAG_RANDOM_DIST = 6 };
+void orc_generate_diagonal(int dist, int mode, int reverse, int randsign, double cond,
+                           double dscale, size_t n, double *d, int *iseed);
+void orc_d_random_matrix(int dist, int mode, int reverse, double cond, double dmax,
+                         size_t m, size_t n, double *a, size_t lda, int *iseed);
+void orc_z_random_matrix(int dist, int mode, int reverse, double cond, double dmax,
+                         size_t m, size_t n, orc_z *a, size_t lda, int *iseed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,655 @@
+/*
+ * rnp_oracle_gen.h -- type-generic body of the oracle (test infrastructure).
+ * Included twice by rnp_oracle.c: once with T = double, once with
+ * T = double complex.  Every function cites the reference file:line it
+ * restates.  Macros supplied by the includer:
+ *   T, FN(name), IS_CPLX, RE(x), IM(x), CONJ(x), MKT(re,im), TABS(x)
+ */
+
+/* BLAS::Norm2, RNP/BLAS/BLAS_mix.hpp:725-758: one-pass scaled sum of squares,
+ * real and imaginary parts fed separately, in index order. */
+double FN(norm2)(size_t n, const T *x, size_t incx){
+	double ssq = 1.0, scale = 0.0;
+	while(n-- > 0){
+		double parts[2];
+		int np = 1;
+		parts[0] = RE(*x);
+#if IS_CPLX
+		parts[1] = IM(*x);
+		np = 2;
+#endif
+		for(int p = 0; p < np; ++p){
+			if(0.0 != parts[p]){
+				double temp = fabs(parts[p]);
+				if(scale < temp){
+					double r = scale/temp;
+					ssq = ssq*r*r + 1.0;
+					scale = temp;
+				}else{
+					double r = temp/scale;
+					ssq += r*r;
+				}
+			}
+		}
+		x += incx;
+	}
+	return scale*sqrt(ssq);
+}
+
+/* BLAS::Scale, RNP/BLAS/BLAS_mix.hpp:580-586 (scalar of type T). */
+static void FN(scal)(size_t n, T alpha, T *x, size_t incx){
+	while(n-- > 0){ *x *= alpha; x += incx; }
+}
+/* BLAS::Scale with a real scale factor (TS = real_type). */
+static void FN(scal_real)(size_t n, double alpha, T *x, size_t incx){
+	while(n-- > 0){ *x *= alpha; x += incx; }
+}
+
+/* Reflector::LastNonzeroColumnLimit, Reflector.hpp:36-48.  For m == 0 the
+ * reference reads a[m-1+...] with m-1 wrapped around; here that case
+ * returns 0 (an empty row range has no nonzero column). */
+size_t FN(last_nonzero_column_limit)(size_t m, size_t n, const T *a, size_t lda){
+	if(0 == n){ return 0; }
+	if(0 == m){ return 0; }
+	if(0.0 != a[0+(n-1)*lda] || 0.0 != a[m-1+(n-1)*lda]){ return n; }
+	size_t j = n;
+	while(j-- > 0){
+		for(size_t i = 0; i < m; ++i){
+			if(0.0 != a[i+j*lda]){ return j+1; }
+		}
+	}
+	return 0;
+}
+
+/* Reflector::LastNonzeroRowLimit, Reflector.hpp:63-75 (n == 0 returns 0). */
+size_t FN(last_nonzero_row_limit)(size_t m, size_t n, const T *a, size_t lda){
+	if(0 == m){ return 0; }
+	if(0 == n){ return 0; }
+	if(0.0 != a[m-1+0*lda] || 0.0 != a[m-1+(n-1)*lda]){ return m; }
+	size_t i = m;
+	while(i-- > 0){
+		for(size_t j = 0; j < n; ++j){
+			if(0.0 != a[i+j*lda]){ return i+1; }
+		}
+	}
+	return 0;
+}
+
+/* ---------------------------------------------------------------------
+ * Level-2 BLAS restatement (what src/BLAS_mix.cpp forwards to Fortran).
+ * ------------------------------------------------------------------- */
+
+/* MultMV -> xgemv (src/BLAS_mix.cpp:104-123): y = alpha*op(A)*x + beta*y. */
+static void FN(gemv)(const char *trans, size_t m, size_t n, T alpha, const T *a, size_t lda,
+                     const T *x, size_t incx, T beta, T *y, size_t incy){
+	const int notr = ('N' == trans[0]);
+	const int conjt = ('C' == trans[0]);
+	const size_t leny = notr ? m : n;
+	const size_t lenx = notr ? n : m;
+	if(0 == m || 0 == n){ /* y = beta*y still applies when the other dim is 0 */ }
+	if(1.0 != beta){
+		for(size_t i = 0; i < leny; ++i){
+			y[i*incy] = (0.0 == beta) ? (T)0 : beta*y[i*incy];
+		}
+	}
+	if(0.0 == alpha || 0 == lenx){ return; }
+	if(notr){
+		for(size_t j = 0; j < n; ++j){
+			T temp = alpha*x[j*incx];
+			for(size_t i = 0; i < m; ++i){ y[i*incy] += temp*a[i+j*lda]; }
+		}
+	}else{
+		for(size_t j = 0; j < n; ++j){
+			T temp = 0;
+			if(conjt){
+				for(size_t i = 0; i < m; ++i){ temp += CONJ(a[i+j*lda])*x[i*incx]; }
+			}else{
+				for(size_t i = 0; i < m; ++i){ temp += a[i+j*lda]*x[i*incx]; }
+			}
+			y[j*incy] += alpha*temp;
+		}
+	}
+}
+
+/* ConjugateRank1Update -> xger / xgerc (src/BLAS_mix.cpp:366-385):
+ * A += alpha * x * y^H. */
+static void FN(gerc)(size_t m, size_t n, T alpha, const T *x, size_t incx,
+                     const T *y, size_t incy, T *a, size_t lda){
+	if(0 == m || 0 == n || 0.0 == alpha){ return; }
+	for(size_t j = 0; j < n; ++j){
+		T yj = y[j*incy];
+		if(0.0 != yj){
+			T temp = alpha*CONJ(yj);
+			for(size_t i = 0; i < m; ++i){ a[i+j*lda] += x[i*incx]*temp; }
+		}
+	}
+}
+
+/* MultTrV -> xtrmv (src/BLAS_mix.cpp:242-257), uplo "U", trans "N",
+ * diag "N" -- the only form GenerateBlockTr uses (Reflector.hpp:618). */
+static void FN(trmv_unn)(size_t n, const T *a, size_t lda, T *x){
+	for(size_t j = 0; j < n; ++j){
+		if(0.0 != x[j]){
+			T temp = x[j];
+			for(size_t i = 0; i < j; ++i){ x[i] += temp*a[i+j*lda]; }
+			x[j] *= a[j+j*lda];
+		}
+	}
+}
+
+/* MultTrM -> xtrmm (src/BLAS_mix.cpp:653-672), side "R" only:
+ * B = alpha * B * op(A), A k-by-k triangular, B m-by-k. */
+static void FN(trmm_r)(const char *uplo, const char *transa, const char *diag,
+                       size_t m, size_t k, T alpha, const T *a, size_t lda, T *b, size_t ldb){
+	if(0 == m || 0 == k){ return; }
+	const int upper = ('U' == uplo[0]);
+	const int nounit = ('N' == diag[0]);
+	const char tr = transa[0];
+	/* Work column by column into a scratch copy of one row block. */
+	T *tmp = (T*)malloc(sizeof(T)*k);
+	for(size_t i = 0; i < m; ++i){
+		for(size_t j = 0; j < k; ++j){ tmp[j] = b[i+j*ldb]; }
+		for(size_t j = 0; j < k; ++j){
+			/* (B*op(A))(i,j) = sum_l B(i,l) op(A)(l,j) */
+			T s = 0;
+			for(size_t l = 0; l < k; ++l){
+				T opa;
+				size_t r, c;
+				if('N' == tr){ r = l; c = j; }else{ r = j; c = l; }
+				int inside = upper ? (r <= c) : (r >= c);
+				if(!inside){ continue; }
+				if(r == c && !nounit){ opa = 1; }
+				else{ opa = a[r+c*lda]; if('C' == tr){ opa = CONJ(opa); } }
+				s += tmp[l]*opa;
+			}
+			b[i+j*ldb] = alpha*s;
+		}
+	}
+	free(tmp);
+}
+
+/* SolveTrM -> xtrsm (src/BLAS_mix.cpp:674-693), "L","U","N","N":
+ * B = inv(A) * B with A upper triangular, non-unit. */
+static void FN(trsm_lunn)(size_t m, size_t n, const T *a, size_t lda, T *b, size_t ldb){
+	for(size_t j = 0; j < n; ++j){
+		size_t k = m;
+		while(k-- > 0){
+			if(0.0 != b[k+j*ldb]){
+				b[k+j*ldb] /= a[k+k*lda];
+				T bk = b[k+j*ldb];
+				for(size_t i = 0; i < k; ++i){ b[i+j*ldb] -= bk*a[i+k*lda]; }
+			}
+		}
+	}
+}
+
+/* ---------------------------------------------------------------------
+ * Reflectors
+ * ------------------------------------------------------------------- */
+
+/* Reflector::Generate, Reflector.hpp:117-158.
+ * Sign rule (line 131-132): beta = +hypot3(...) unless real(alpha) > 0, so
+ * beta > 0 when real(alpha) == 0 (LAPACK's xlarfg makes it negative there).
+ * safmin = DBL_MIN/(2*DBL_EPSILON) = 2^-971 (line 133).
+ * Rescale loop (lines 139-144): its condition calls the C library's
+ * int abs(int) on the double beta; the reading that terminates (and that an
+ * optimizing build of the reference executes) is "truncate beta to an int
+ * and test it against 0", i.e. the loop repeats while |beta| < 1.  For any
+ * finite nonzero beta below 2^-971 that is exactly two scalings by 2^971. */
+void FN(generate)(size_t n, T *alpha, T *x, size_t incx, T *tau){
+	if(0 == n){ *tau = 0; return; }
+	double xnorm = FN(norm2)(n-1, x, incx);
+	if(0.0 == xnorm && 0.0 == IM(*alpha)){
+		*tau = 0;
+		return;
+	}
+	double beta = orc_hypot3(RE(*alpha), IM(*alpha), xnorm);
+	if(RE(*alpha) > 0){ beta = -beta; }
+	const double safmin = DBL_MIN / (2*DBL_EPSILON);
+	const double rsafmn = 1.0 / safmin;
+	size_t knt = 0;
+	if(fabs(beta) < safmin){
+		do{
+			++knt;
+			FN(scal_real)(n-1, rsafmn, x, incx);
+			beta *= rsafmn;
+			*alpha *= rsafmn;
+		}while(fabs(beta) < 1.0);
+		xnorm = FN(norm2)(n-1, x, incx);
+		beta = orc_hypot3(RE(*alpha), IM(*alpha), xnorm);
+		if(RE(*alpha) > 0){ beta = -beta; }
+	}
+	/* tau = (beta - alpha)/beta : complex over real, componentwise */
+	*tau = MKT((beta - RE(*alpha))/beta, (-IM(*alpha))/beta);
+	/* alpha = 1/(alpha - beta) : real(1) over T (std::complex division) */
+	{
+		T den = MKT(RE(*alpha) - beta, IM(*alpha));
+		T one = MKT(1.0, 0.0);
+		*alpha = one / den;
+	}
+	FN(scal)(n-1, *alpha, x, incx);
+	while(knt-- > 0){ beta *= safmin; }
+	*alpha = MKT(beta, 0.0);
+}
+
+/* Reflector::Apply, Reflector.hpp:191-360, for vconj == false and
+ * vone in {0, 1} -- the branches QR::Factor_unblocked (QR.hpp:95-98) and
+ * QR::MultQ_unblocked (QR.hpp:275, 295) reach.  H = I - tau v v^H. */
+void FN(apply)(const char *side, int vone, size_t m, size_t n, const T *v, size_t incv,
+               T tau, T *c, size_t ldc, T *work){
+	size_t lenv = 0, lenc = 0;
+	const int left = ('L' == side[0]);
+	if(0.0 != tau){
+		lenv = left ? m : n;
+		if(0 == lenv){ return; }
+		size_t i = (lenv-1)*incv;
+		while(lenv > 0 && 0.0 == v[i]){
+			--lenv;
+			i -= incv;
+		}
+		if(left){
+			lenc = FN(last_nonzero_column_limit)(lenv, n, c, ldc);
+		}else{
+			lenc = FN(last_nonzero_row_limit)(m, lenv, c, ldc);
+		}
+		if(0 == lenc){ return; }
+	}
+	const T one = 1;
+	if(0 == vone){
+		if(left){
+			FN(gemv)("C", lenv, lenc, one, c, ldc, v, incv, 0, work, 1);
+			FN(gerc)(lenv, lenc, -tau, v, incv, work, 1, c, ldc);
+		}else{
+			FN(gemv)("N", lenc, lenv, one, c, ldc, v, incv, 0, work, 1);
+			FN(gerc)(lenc, lenv, -tau, work, 1, v, incv, c, ldc);
+		}
+	}else{
+		if(left){
+			if(lenv > 1){
+				for(size_t j = 0; j < lenc; ++j){ work[j] = CONJ(c[0+j*ldc]); }
+				FN(gemv)("C", lenv-1, lenc, one, &c[1], ldc, &v[incv], incv, one, work, 1);
+			}else{
+				for(size_t j = 0; j < lenc; ++j){ work[j] = CONJ(c[0+j*ldc]); }
+			}
+			FN(gerc)(1, lenc, -tau, &one, incv, work, 1, c, ldc);
+			if(lenv > 1){
+				FN(gerc)(lenv-1, lenc, -tau, &v[incv], incv, work, 1, &c[1], ldc);
+			}
+		}else{
+			for(size_t i = 0; i < lenc; ++i){ work[i] = c[i]; }
+			if(lenv > 1){
+				FN(gemv)("N", lenc, lenv-1, one, &c[0+1*ldc], ldc, &v[incv], incv, one, work, 1);
+			}
+			/* Axpy(lenc, -tau, work, 1, c, 1) */
+			for(size_t i = 0; i < lenc; ++i){ c[i] += -tau*work[i]; }
+			if(lenv > 1){
+				FN(gerc)(lenc, lenv-1, -tau, work, 1, &v[incv], incv, &c[0+1*ldc], ldc);
+			}
+		}
+	}
+}
+
+/* Reflector::GenerateBlockTr, Reflector.hpp:573-626, dir "F", storev "C":
+ * T upper k-by-k with H[0] H[1] ... H[k-1] = I - V T V^H. */
+void FN(generate_block_tr)(size_t n, size_t k, const T *v, size_t ldv, const T *tau,
+                           T *t, size_t ldt){
+	if(0 == n){ return; }
+	size_t prevlastv = n-1;
+	for(size_t i = 0; i < k; ++i){
+		if(i > prevlastv){ prevlastv = i; }
+		if(0.0 == tau[i]){
+			for(size_t j = 0; j <= i; ++j){ t[j+i*ldt] = 0; }
+		}else{
+			size_t lastv;
+			for(lastv = n-1; lastv > i; --lastv){
+				if(0.0 != v[lastv+i*ldv]){ break; }
+			}
+			for(size_t j = 0; j < i; ++j){
+				t[j+i*ldt] = -tau[i] * CONJ(v[i+j*ldv]);
+			}
+			size_t jlim = (lastv < prevlastv ? lastv : prevlastv);
+			FN(gemv)("C", jlim-i, i, -tau[i], &v[i+1+0*ldv], ldv,
+			         &v[i+1+i*ldv], 1, 1, &t[0+i*ldt], 1);
+			FN(trmv_unn)(i, t, ldt, &t[0+i*ldt]);
+			t[i+i*ldt] = tau[i];
+			if(i > 0){
+				if(lastv > prevlastv){ prevlastv = lastv; }
+			}else{
+				prevlastv = lastv;
+			}
+		}
+	}
+}
+
+/* Reflector::ApplyBlock, Reflector.hpp:725-815, dir "F", storev "C":
+ * side "L": C = H C or H^H C;  side "R": C = C H or C H^H,
+ * H = I - V T V^H, work is ldwork-by-k. */
+void FN(apply_block)(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                     const T *v, size_t ldv, const T *t, size_t ldt,
+                     T *c, size_t ldc, T *work, size_t ldwork){
+	if(0 == m || 0 == n){ return; }
+	const char *transt = ('N' == trans[0] ? "C" : "N");
+	const T one = 1, mone = -1;
+	if('L' == side[0]){
+		size_t lastv = FN(last_nonzero_row_limit)(m, k, v, ldv);
+		if(k > lastv){ lastv = k; }
+		const size_t lastc = FN(last_nonzero_column_limit)(lastv, n, c, ldc);
+		for(size_t j = 0; j < k; ++j){
+			for(size_t i = 0; i < lastc; ++i){ work[i+j*ldwork] = CONJ(c[j+i*ldc]); }
+		}
+		FN(trmm_r)("L", "N", "U", lastc, k, one, v, ldv, work, ldwork);
+		if(lastv > k){
+			FN(gemm)("C", "N", lastc, k, lastv-k, one, &c[k], ldc, &v[k], ldv, one, work, ldwork);
+		}
+		FN(trmm_r)("U", transt, "N", lastc, k, one, t, ldt, work, ldwork);
+		if(m > k){
+			FN(gemm)("N", "C", lastv-k, lastc, k, mone, &v[k], ldv, work, ldwork, one, &c[k], ldc);
+		}
+		FN(trmm_r)("L", "C", "U", lastc, k, one, v, ldv, work, ldwork);
+		for(size_t j = 0; j < k; ++j){
+			for(size_t i = 0; i < lastc; ++i){ c[j+i*ldc] -= CONJ(work[i+j*ldwork]); }
+		}
+	}else{
+		size_t lastv = FN(last_nonzero_row_limit)(n, k, v, ldv);
+		if(k > lastv){ lastv = k; }
+		const size_t lastc = FN(last_nonzero_row_limit)(m, lastv, c, ldc);
+		for(size_t j = 0; j < k; ++j){
+			for(size_t i = 0; i < lastc; ++i){ work[i+j*ldwork] = c[i+j*ldc]; }
+		}
+		FN(trmm_r)("L", "N", "U", lastc, k, one, v, ldv, work, ldwork);
+		if(lastv > k){
+			FN(gemm)("N", "N", lastc, k, lastv-k, one, &c[k*ldc], ldc, &v[k], ldv, one, work, ldwork);
+		}
+		FN(trmm_r)("U", trans, "N", lastc, k, one, t, ldt, work, ldwork);
+		if(lastv > k){
+			FN(gemm)("N", "C", lastc, lastv-k, k, mone, work, ldwork, &v[k], ldv, one, &c[k*ldc], ldc);
+		}
+		FN(trmm_r)("L", "C", "U", lastc, k, one, v, ldv, work, ldwork);
+		for(size_t j = 0; j < k; ++j){
+			for(size_t i = 0; i < lastc; ++i){ c[i+j*ldc] -= work[i+j*ldwork]; }
+		}
+	}
+}
+
+/* ---------------------------------------------------------------------
+ * QR
+ * ------------------------------------------------------------------- */
+
+/* QR::Factor_unblocked, QR.hpp:78-102. */
+void FN(factor_unblocked)(size_t m, size_t n, T *a, size_t lda, T *tau, T *work){
+	size_t k = m < n ? m : n;
+	for(size_t i = 0; i < k; ++i){
+		size_t row = m-1; if(i+1 < row){ row = i+1; }
+		FN(generate)(m-i, &a[i+i*lda], &a[row+i*lda], 1, &tau[i]);
+		if(i < n-1){
+			T alpha = a[i+i*lda];
+			a[i+i*lda] = 1;
+			FN(apply)("L", 0, m-i, n-i-1, &a[i+i*lda], 1, CONJ(tau[i]), &a[i+(i+1)*lda], lda, work);
+			a[i+i*lda] = alpha;
+		}
+	}
+}
+
+/* QR::Factor, QR.hpp:129-197 (workspace query and short-lwork degrade
+ * included; the Tuning hook is the optional `tune` argument). */
+void FN(factor)(size_t m, size_t n, T *a, size_t lda, T *tau, size_t *lwork, T *work,
+                const orc_tuning *tune){
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(NULL == lwork){ orc_assert_fail("NULL != lwork", __LINE__); return; }
+	if(!(0 == *lwork || *lwork >= m)){ orc_assert_fail("0 == *lwork || *lwork >= m", __LINE__); }
+	const size_t k = (m < n ? m : n);
+	if(0 == k){ return; }
+	size_t nb = tune ? tune->nb : 32;
+	if(NULL == work || 0 == *lwork){
+		*lwork = nb*n;
+		return;
+	}
+	size_t nbmin = 2, nx = 0, iws = n, ldwork = n;
+	if(nb > 1 && nb < k){
+		nx = tune ? tune->nx : 128;
+		if(nx < k){
+			iws = ldwork*nb;
+			if(*lwork < iws){
+				nb = *lwork / ldwork;
+				nbmin = tune ? tune->nbmin : 2;
+				if(2 > nbmin){ nbmin = 2; }
+			}
+		}
+	}
+	size_t i;
+	if(nb >= nbmin && nb < k && nx < k){
+		for(i = 0; i+nx < k; i += nb){
+			const size_t ib = (k-i < nb ? k-i : nb);
+			FN(factor_unblocked)(m-i, ib, &a[i+i*lda], lda, &tau[i], work);
+			if(i+ib < n){
+				FN(generate_block_tr)(m-i, ib, &a[i+i*lda], lda, &tau[i], work, ldwork);
+				FN(apply_block)("L", "C", m-i, n-i-ib, ib, &a[i+i*lda], lda,
+				                work, ldwork, &a[i+(i+ib)*lda], lda, &work[ib], ldwork);
+			}
+		}
+	}else{
+		i = 0;
+	}
+	if(i < k){
+		FN(factor_unblocked)(m-i, n-i, &a[i+i*lda], lda, &tau[i], work);
+	}
+}
+
+/* QR::MultQ_unblocked, QR.hpp:234-298. */
+void FN(multq_unblocked)(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                         const T *a, size_t lda, const T *tau, T *c, size_t ldc, T *work){
+	if(!(ldc >= m)){ orc_assert_fail("ldc >= m", __LINE__); }
+	const int left = ('L' == side[0]);
+	const int notran = ('N' == trans[0]);
+	if(!((left && k <= m) || (!left && k <= n))){ orc_assert_fail("k range", __LINE__); }
+	if(!(lda >= k)){ orc_assert_fail("lda >= k", __LINE__); }
+	if(m < 1 || n < 1 || k < 1){ return; }
+	size_t mi = m, ni = n, ic = 0, jc = 0;
+	const int forward = ((left && !notran) || (!left && notran));
+	for(size_t s = 0; s < k; ++s){
+		size_t i = forward ? s : k-1-s;
+		if(left){ mi = m-i; ic = i; }else{ ni = n-i; jc = i; }
+		T taui = notran ? tau[i] : CONJ(tau[i]);
+		FN(apply)(side, 1, mi, ni, &a[i+i*lda], 1, taui, &c[ic+jc*ldc], ldc, work);
+	}
+}
+
+/* QR::MultQ, QR.hpp:337-429 (T recomputed for every block, as there). */
+void FN(multq)(const char *side, const char *trans, size_t m, size_t n, size_t k,
+               const T *a, size_t lda, const T *tau, T *c, size_t ldc,
+               size_t *lwork, T *work, const orc_tuning *tune){
+	if(!(ldc >= m)){ orc_assert_fail("ldc >= m", __LINE__); }
+	if(0 == m || 0 == n || 0 == k){ return; }
+	const int left = ('L' == side[0]);
+	const int notran = ('N' == trans[0]);
+	if(!((left && k <= m) || (!left && k <= n))){ orc_assert_fail("k range", __LINE__); }
+	if(!(lda >= k)){ orc_assert_fail("lda >= k", __LINE__); }
+	const size_t nq = left ? m : n;
+	const size_t nw = left ? n : m;
+	size_t nb = tune ? tune->nb : 32;
+	if(0 == *lwork || NULL == work){
+		*lwork = nb*nw + nb*nb;
+		return;
+	}
+	size_t nbmin = 2, ldwork = nw;
+	T *t = NULL;
+	size_t ldt = nb;
+	if(nb > 1 && nb < k){
+		size_t iws = nw*nb + nb*nb;
+		t = work + nb*ldwork;
+		if(*lwork < iws){
+			nb = *lwork / (2*ldwork);
+			t = work + nb*ldwork;
+			ldt = nb;
+			nbmin = tune ? tune->nbmin : 2;
+		}
+	}
+	if(nb < nbmin || nb >= k){
+		FN(multq_unblocked)(side, trans, m, n, k, a, lda, tau, c, ldc, work);
+		return;
+	}
+	size_t mi = m, ni = n, ic = 0, jc = 0;
+	if((left && !notran) || (!left && notran)){
+		for(size_t i = 0; i < k; i += nb){
+			size_t ib = (k < nb+i ? k-i : nb);
+			FN(generate_block_tr)(nq-i, ib, &a[i+i*lda], lda, &tau[i], t, ldt);
+			if(left){ mi = m-i; ic = i; }else{ ni = n-i; jc = i; }
+			FN(apply_block)(side, trans, mi, ni, ib, &a[i+i*lda], lda, t, ldt,
+			                &c[ic+jc*ldc], ldc, work, ldwork);
+		}
+	}else{
+		size_t i = ((k-1)/nb)*nb + nb;
+		do{
+			i -= nb;
+			size_t ib = (k < nb+i ? k-i : nb);
+			FN(generate_block_tr)(nq-i, ib, &a[i+i*lda], lda, &tau[i], t, ldt);
+			if(left){ mi = m-i; ic = i; }else{ ni = n-i; jc = i; }
+			FN(apply_block)(side, trans, mi, ni, ib, &a[i+i*lda], lda, t, ldt,
+			                &c[ic+jc*ldc], ldc, work, ldwork);
+		}while(i > 0);
+	}
+}
+
+/* QR::GenerateQ_unblocked, QR.hpp:451-485. */
+void FN(generateq_unblocked)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *tau,
+                             T *work){
+	if(!(m >= n)){ orc_assert_fail("m >= n", __LINE__); }
+	if(!(k <= n)){ orc_assert_fail("k <= n", __LINE__); }
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(n < 1){ return; }
+	for(size_t j = k; j < n; ++j){
+		for(size_t i = 0; i < m; ++i){ a[i+j*lda] = 0; }
+		a[j+j*lda] = 1;
+	}
+	size_t i = k;
+	while(i-- > 0){
+		if(i+1 < n){
+			a[i+i*lda] = 1;
+			FN(apply)("L", 0, m-i, n-i-1, &a[i+i*lda], 1, tau[i], &a[i+(i+1)*lda], lda, work);
+		}
+		if(i+1 < m){
+			FN(scal)(m-i-1, -tau[i], &a[i+1+i*lda], 1);
+		}
+		a[i+i*lda] = 1.0 - tau[i];
+		for(size_t l = 0; l < i; ++l){ a[l+i*lda] = 0; }
+	}
+}
+
+/* QR::GenerateQ, QR.hpp:509-570. */
+void FN(generateq)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *tau,
+                   size_t *lwork, T *work, const orc_tuning *tune){
+	if(!(k <= n)){ orc_assert_fail("k <= n", __LINE__); }
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(0 == n){ return; }
+	size_t nb = tune ? tune->nb : 32;
+	if(0 == *lwork || NULL == work){
+		*lwork = n*nb;
+		return;
+	}
+	size_t nbmin = 2, nx = 0, ldwork = n;
+	if(nb > 1 && nb < k){
+		nx = tune ? tune->nx : 128;
+		if(nx < k){
+			size_t iws = ldwork*nb;
+			if(*lwork < iws){
+				nb = *lwork / ldwork;
+				nbmin = tune ? tune->nbmin : 2;
+			}
+		}
+	}
+	size_t ki = 0, kk = 0;
+	if(nb >= nbmin && nb < k && nx < k){
+		ki = ((k-nx-1)/nb) * nb;
+		kk = (k < ki+nb ? k : ki+nb);
+		for(size_t j = kk; j < n; ++j){
+			for(size_t i = 0; i < kk; ++i){ a[i+j*lda] = 0; }
+		}
+	}
+	if(kk < n){
+		FN(generateq_unblocked)(m-kk, n-kk, k-kk, &a[kk+kk*lda], lda, &tau[kk], work);
+	}
+	if(kk > 0){
+		size_t i = ki + nb;
+		do{
+			i -= nb;
+			const size_t ib = (nb+i < k ? nb : k-i);
+			if(i+ib < n){
+				FN(generate_block_tr)(m-i, ib, &a[i+i*lda], lda, &tau[i], work, ldwork);
+				FN(apply_block)("L", "N", m-i, n-i-ib, ib, &a[i+i*lda], lda, work, ldwork,
+				                &a[i+(i+ib)*lda], lda, &work[ib], ldwork);
+			}
+			FN(generateq_unblocked)(m-i, ib, ib, &a[i+i*lda], lda, &tau[i], work);
+			for(size_t j = i; j < i+ib; ++j){
+				for(size_t l = 0; l < i; ++l){ a[l+j*lda] = 0; }
+			}
+		}while(i > 0);
+	}
+}
+
+/* Least squares by QR: Factor (QR.hpp:129), MultQ("L","C") on B
+ * (QR.hpp:337), then R x = (Q^H B)(0:n,:) by back substitution
+ * (SolveTrM "L","U","N","N", BLAS_mix.hpp:2096). */
+void FN(ls_solve)(size_t m, size_t n, size_t nrhs, T *a, size_t lda, T *b, size_t ldb){
+	size_t lw = 0;
+	FN(factor)(m, n, a, lda, NULL, &lw, NULL, NULL);
+	size_t k = m < n ? m : n;
+	T *tau = (T*)calloc(k ? k : 1, sizeof(T));
+	T *work = (T*)calloc(lw ? lw : 1, sizeof(T));
+	FN(factor)(m, n, a, lda, tau, &lw, work, NULL);
+	free(work);
+	lw = 0;
+	FN(multq)("L", "C", m, nrhs, n, a, lda, tau, b, ldb, &lw, NULL, NULL);
+	work = (T*)calloc(lw ? lw : 1, sizeof(T));
+	FN(multq)("L", "C", m, nrhs, n, a, lda, tau, b, ldb, &lw, work, NULL);
+	free(work);
+	FN(trsm_lunn)(n, nrhs, a, lda, b, ldb);
+	free(tau);
+}
+
+#if !IS_CPLX
+/* RNP::MatrixGen::RandomMatrix, MatrixGen.hpp:120-198, Symmetry::Nonsymmetric:
+ * diag(d) pre- and post-multiplied by random Householder reflectors drawn
+ * from Normal01 (mindim of them on each side, last index first). */
+void FN(random_matrix)(int dist, int mode, int reverse, double cond, double dmax,
+                       size_t m, size_t n, T *a, size_t lda, int *iseed){
+	const size_t mindim = (m < n ? m : n);
+	if(0 == mindim){ return; }
+	double *d = (double*)malloc(sizeof(double)*mindim);
+	T *work = (T*)malloc(sizeof(T)*(m+n));
+	orc_generate_diagonal(dist, mode, reverse, 0, cond, dmax, mindim, d, iseed);
+	for(size_t j = 0; j < n; ++j){
+		for(size_t i = 0; i < m; ++i){ a[i+j*lda] = (i == j) ? (T)d[i] : (T)0; }
+	}
+	free(d);
+	size_t i = mindim;
+	while(i-- > 0){
+		for(int pass = 0; pass < 2; ++pass){
+			const int leftside = (0 == pass);
+			const size_t len = leftside ? m-i : n-i;
+			if(!(i+1 < (leftside ? m : n))){ continue; }
+			FN(generate_vector)(ORC_NORMAL01, len, work, iseed);
+			double wn = FN(norm2)(len, work, 1);
+			T wa = (wn / TABS(work[0])) * work[0];
+			T tau = 0;
+			if(0.0 != wn){
+				T wb = work[0] + wa;
+				{
+					T inv = MKT(1.0, 0.0) / wb;
+					for(size_t q = 1; q < len; ++q){ work[q] *= inv; }
+				}
+				work[0] = 1;
+				tau = RE(wb / wa);
+			}
+			if(leftside){
+				FN(gemv)("C", m-i, n-i, 1, &a[i+i*lda], lda, work, 1, 0, &work[m], 1);
+				FN(gerc)(m-i, n-i, -tau, work, 1, &work[m], 1, &a[i+i*lda], lda);
+			}else{
+				FN(gemv)("N", m-i, n-i, 1, &a[i+i*lda], lda, work, 1, 0, &work[n], 1);
+				FN(gerc)(m-i, n-i, -tau, &work[n], 1, work, 1, &a[i+i*lda], lda);
+			}
+		}
+	}
+	free(work);
+}
+#endif /* !IS_CPLX: the reference's RandomMatrix takes Traits<real>::abs of a T
+        (MatrixGen.hpp:172), which only compiles for real T */

@@ -1,0 +1,345 @@
+"""rnp2_amd -- MI355X (gfx950) blocked Householder QR, drop-in for RNP2's LA::QR.
+
+Python view of the C ABI in ``include/rnpx.h`` (library ``rnp2_amd/librnpx.so``).
+The functions mirror RNP's QR interface (RNP/LA/QR.hpp): ``Factor``,
+``MultQ`` and ``GenerateQ`` with the same argument meaning, column-major
+arrays, 0-based, and the reference's workspace-query convention; the
+lower-case helpers wrap them for numpy arrays and torch device tensors.
+
+There is no CPU fallback: every call goes to the HIP library, and a missing
+library or GPU raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librnpx.so")
+_lib = None
+
+RNPX_OK = 0
+RNPX_ERR_HIP = -1000
+RNPX_ERR_NOGPU = -1001
+RNPX_ERR_NOMEM = -1002
+RNPX_ERR_SYNC = -1003
+RNPX_ERR_UNSUPPORTED = -1004
+RNPX_ERR_NOTCONV = -1005
+
+_sz = ctypes.c_size_t
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_s = ctypes.c_char_p
+
+
+class RnpxError(RuntimeError):
+    def __init__(self, status, what):
+        super().__init__(f"{what} failed with status {status}")
+        self.status = status
+
+
+class ZComplex(ctypes.Structure):
+    """rnpx_zcomplex (layout of std::complex<double>)."""
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+class Tuning(ctypes.Structure):
+    """rnpx_tuning: the QR::Tuning<T> hook (QR.hpp:41-51)."""
+    _fields_ = [("nb", _i), ("panel_rpt", _i), ("ksplit", _i), ("lookahead", _i)]
+
+
+# name -> (restype, argtypes); kept in the order of include/rnpx.h
+SIGNATURES = {
+    "rnpx_dgeqrf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_zgeqrf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_dormqr": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "rnpx_zunmqr": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "rnpx_dorgqr": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_zungqr": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_create": (_i, [ctypes.POINTER(_p), _sz]),
+    "rnpx_destroy": (_i, [_p]),
+    "rnpx_set_stream": (_i, [_p, _p]),
+    "rnpx_get_stream": (_p, [_p]),
+    "rnpx_set_tuning": (_i, [_p, ctypes.POINTER(Tuning)]),
+    "rnpx_get_tuning": (_i, [_p, ctypes.POINTER(Tuning)]),
+    "rnpx_synchronize": (_i, [_p]),
+    "rnpx_workspace_bytes": (_sz, [_p]),
+    "rnpx_profile_enable": (_i, [_p, _i]),
+    "rnpx_profile_read": (_i, [_p, _i, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "rnpx_dgeqrf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_zgeqrf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_dormqr_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
+    "rnpx_zunmqr_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
+    "rnpx_dorgqr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p]),
+    "rnpx_zungqr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p]),
+    "rnpx_dlarft_dev": (_i, [_p, _sz, _sz, _p, _sz, _p, _p, _sz]),
+    "rnpx_zlarft_dev": (_i, [_p, _sz, _sz, _p, _sz, _p, _p, _sz]),
+    "rnpx_dlarfb_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, _p, _sz, _p, _sz, _p, _sz]),
+    "rnpx_zlarfb_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, _p, _sz, _p, _sz, _p, _sz]),
+    "rnpx_dgemm_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, ctypes.c_double, _p, _sz, _p, _sz, ctypes.c_double, _p,
+                            _sz]),
+    "rnpx_zgemm_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, None, _p, _sz, _p, _sz, None, _p, _sz]),
+    "rnpx_dlarnv_dev": (_i, [_p, _i, ctypes.POINTER(_i), _sz, _p]),
+    "rnpx_zlarnv_dev": (_i, [_p, _i, ctypes.POINTER(_i), _sz, _p]),
+    "rnpx_version": (_s, []),
+    "rnpx_device_name": (_i, [ctypes.c_char_p, _sz]),
+}
+
+
+def lib():
+    """Load librnpx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rnp2_amd.build` "
+                               "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = [ZComplex if a is None else a for a in args]
+        _lib = L
+    return _lib
+
+
+def _check(status, what):
+    if status != RNPX_OK:
+        raise RnpxError(status, what)
+
+
+def _pre(dtype):
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return "d"
+    if dt == np.complex128:
+        return "z"
+    raise TypeError(f"rnp2_amd supports float64 and complex128, not {dt}")
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------------------
+# Mirror of RNP::LA::QR (host numpy arrays in, results in place)
+# ---------------------------------------------------------------------------
+
+def Factor(m, n, a, lda, tau, lwork, work):
+    """QR::Factor(m, n, a, lda, tau, &lwork, work) -- QR.hpp:129-131.
+
+    ``a``/``tau``/``work`` are numpy arrays (or None for ``work`` in a query);
+    ``lwork`` is a one-element numpy uint64 array, updated by a query.
+    Returns the status."""
+    pre = _pre(a.dtype)
+    lw = _sz(int(lwork[0]))
+    st = getattr(lib(), f"rnpx_{pre}geqrf")(m, n, _ptr(a), lda, _ptr(tau), ctypes.byref(lw), _ptr(work))
+    lwork[0] = lw.value
+    return st
+
+
+def MultQ(side, trans, m, n, k, a, lda, tau, c, ldc, lwork, work):
+    """QR::MultQ(side, trans, m, n, k, a, lda, tau, c, ldc, &lwork, work) -- QR.hpp:337-341."""
+    pre = _pre(c.dtype)
+    fn = "rnpx_dormqr" if pre == "d" else "rnpx_zunmqr"
+    lw = _sz(int(lwork[0]))
+    st = getattr(lib(), fn)(side.encode(), trans.encode(), m, n, k, _ptr(a), lda, _ptr(tau), _ptr(c), ldc,
+                            ctypes.byref(lw), _ptr(work))
+    lwork[0] = lw.value
+    return st
+
+
+def GenerateQ(m, n, k, a, lda, tau, lwork, work):
+    """QR::GenerateQ(m, n, k, a, lda, tau, &lwork, work) -- QR.hpp:510-512."""
+    pre = _pre(a.dtype)
+    fn = "rnpx_dorgqr" if pre == "d" else "rnpx_zungqr"
+    lw = _sz(int(lwork[0]))
+    st = getattr(lib(), fn)(m, n, k, _ptr(a), lda, _ptr(tau), ctypes.byref(lw), _ptr(work))
+    lwork[0] = lw.value
+    return st
+
+
+# ---------------------------------------------------------------------------
+# Convenience: numpy arrays (column-major copies are made as needed)
+# ---------------------------------------------------------------------------
+
+def factor(a):
+    """QR-factor a numpy matrix; returns (packed R/V factor, tau)."""
+    a = np.array(a, dtype=np.result_type(a, np.float64), order="F", copy=True)
+    m, n = a.shape
+    k = min(m, n)
+    tau = np.zeros(max(k, 1), dtype=a.dtype)
+    lw = np.zeros(1, dtype=np.uint64)
+    _check(Factor(m, n, a, max(m, 1), tau, lw, None), "Factor query")
+    work = np.zeros(max(int(lw[0]), 1), dtype=a.dtype)
+    _check(Factor(m, n, a, max(m, 1), tau, lw, work), "Factor")
+    return a, tau[:k]
+
+
+def multq(side, trans, afac, tau, c, k=None):
+    """op(Q) C or C op(Q) for Q held in (afac, tau)."""
+    c = np.array(c, dtype=np.result_type(c, afac, np.float64), order="F", copy=True)
+    afac = np.asfortranarray(afac, dtype=c.dtype)
+    tau = np.ascontiguousarray(tau, dtype=c.dtype)
+    m, n = c.shape
+    k = tau.size if k is None else k
+    lw = np.zeros(1, dtype=np.uint64)
+    _check(MultQ(side, trans, m, n, k, afac, afac.shape[0], tau, c, max(m, 1), lw, None), "MultQ query")
+    work = np.zeros(max(int(lw[0]), 1), dtype=c.dtype)
+    _check(MultQ(side, trans, m, n, k, afac, afac.shape[0], tau, c, max(m, 1), lw, work), "MultQ")
+    return c
+
+
+def generateq(afac, tau, n=None, k=None):
+    """The first n columns of Q from (afac, tau)."""
+    a = np.array(afac, dtype=np.result_type(afac, np.float64), order="F", copy=True)
+    m = a.shape[0]
+    n = a.shape[1] if n is None else n
+    a = np.asfortranarray(a[:, :n])
+    tau = np.ascontiguousarray(tau, dtype=a.dtype)
+    k = min(tau.size, n) if k is None else k
+    lw = np.zeros(1, dtype=np.uint64)
+    _check(GenerateQ(m, n, k, a, max(m, 1), tau, lw, None), "GenerateQ query")
+    work = np.zeros(max(int(lw[0]), 1), dtype=a.dtype)
+    _check(GenerateQ(m, n, k, a, max(m, 1), tau, lw, work), "GenerateQ")
+    return a
+
+
+def version():
+    return lib().rnpx_version().decode()
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    _check(lib().rnpx_device_name(buf, 256), "device_name")
+    return buf.value.decode()
+
+
+# ---------------------------------------------------------------------------
+# Device-resident work through a handle (torch tensors as device memory)
+# ---------------------------------------------------------------------------
+
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _tpre(t):
+    import torch
+    if t.dtype == torch.float64:
+        return "d"
+    if t.dtype == torch.complex128:
+        return "z"
+    raise TypeError(f"unsupported dtype {t.dtype}")
+
+
+def _ld(t):
+    """Leading dimension of a column-major (Fortran-strided) 2-D tensor."""
+    if t.dim() == 1:
+        return max(t.shape[0], 1)
+    if t.stride(0) != 1:
+        raise ValueError("device matrices must be column-major (stride(0) == 1)")
+    return max(t.stride(1), t.shape[0], 1)
+
+
+def colmajor_empty(m, n, dtype, device="cuda"):
+    """A column-major m x n device tensor (the transpose of a row-major n x m)."""
+    import torch
+    return torch.empty((n, m), dtype=dtype, device=device).t()
+
+
+class Handle:
+    """An rnpx handle: a HIP stream plus device workspace."""
+
+    def __init__(self, hbm_budget_bytes=0, stream=None):
+        h = ctypes.c_void_p()
+        _check(lib().rnpx_create(ctypes.byref(h), hbm_budget_bytes), "rnpx_create")
+        self.h = h
+        if stream is not None:
+            _check(lib().rnpx_set_stream(self.h, ctypes.c_void_p(stream)), "rnpx_set_stream")
+
+    def close(self):
+        if self.h:
+            lib().rnpx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return lib().rnpx_get_stream(self.h)
+
+    def set_tuning(self, nb=0, panel_rpt=0, ksplit=0, lookahead=0):
+        t = Tuning(nb, panel_rpt, ksplit, lookahead)
+        _check(lib().rnpx_set_tuning(self.h, ctypes.byref(t)), "rnpx_set_tuning")
+
+    KERNELS = {"update": 0, "wgemm": 1, "panel": 2, "tfactor": 3}
+
+    def profile(self, on=True):
+        _check(lib().rnpx_profile_enable(self.h, 1 if on else 0), "rnpx_profile_enable")
+
+    def profile_read(self, kernel):
+        n, ms, fl, by = ctypes.c_long(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().rnpx_profile_read(self.h, self.KERNELS[kernel], ctypes.byref(n), ctypes.byref(ms),
+                                       ctypes.byref(fl), ctypes.byref(by)), "rnpx_profile_read")
+        return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+
+    def synchronize(self):
+        _check(lib().rnpx_synchronize(self.h), "rnpx_synchronize")
+
+    def workspace_bytes(self):
+        return lib().rnpx_workspace_bytes(self.h)
+
+    def geqrf(self, a, tau):
+        m, n = a.shape
+        pre = _tpre(a)
+        _check(getattr(lib(), f"rnpx_{pre}geqrf_dev")(self.h, m, n, _dptr(a), _ld(a), _dptr(tau)),
+               "geqrf_dev")
+
+    def ormqr(self, side, trans, a, tau, c, k=None):
+        m, n = c.shape
+        k = tau.shape[0] if k is None else k
+        fn = "rnpx_dormqr_dev" if _tpre(c) == "d" else "rnpx_zunmqr_dev"
+        _check(getattr(lib(), fn)(self.h, side.encode(), trans.encode(), m, n, k, _dptr(a), _ld(a),
+                                  _dptr(tau), _dptr(c), _ld(c)), "ormqr_dev")
+
+    def orgqr(self, a, tau, n=None, k=None):
+        m = a.shape[0]
+        n = a.shape[1] if n is None else n
+        k = min(tau.shape[0], n) if k is None else k
+        fn = "rnpx_dorgqr_dev" if _tpre(a) == "d" else "rnpx_zungqr_dev"
+        _check(getattr(lib(), fn)(self.h, m, n, k, _dptr(a), _ld(a), _dptr(tau)), "orgqr_dev")
+
+    def larft(self, v, tau, t):
+        n, k = v.shape
+        fn = "rnpx_dlarft_dev" if _tpre(v) == "d" else "rnpx_zlarft_dev"
+        _check(getattr(lib(), fn)(self.h, n, k, _dptr(v), _ld(v), _dptr(tau), _dptr(t), _ld(t)), "larft_dev")
+
+    def larfb(self, side, trans, v, t, c):
+        m, n = c.shape
+        k = v.shape[1]
+        fn = "rnpx_dlarfb_dev" if _tpre(c) == "d" else "rnpx_zlarfb_dev"
+        _check(getattr(lib(), fn)(self.h, side.encode(), trans.encode(), m, n, k, _dptr(v), _ld(v), _dptr(t),
+                                  _ld(t), _dptr(c), _ld(c)), "larfb_dev")
+
+    def gemm(self, transa, transb, alpha, a, b, beta, c):
+        m, n = c.shape
+        k = a.shape[1] if transa == "N" else a.shape[0]
+        if _tpre(c) == "d":
+            _check(lib().rnpx_dgemm_dev(self.h, transa.encode(), transb.encode(), m, n, k, float(alpha), _dptr(a),
+                                        _ld(a), _dptr(b), _ld(b), float(beta), _dptr(c), _ld(c)), "dgemm_dev")
+        else:
+            al, be = complex(alpha), complex(beta)
+            _check(lib().rnpx_zgemm_dev(self.h, transa.encode(), transb.encode(), m, n, k, ZComplex(al.real, al.imag),
+                                        _dptr(a), _ld(a), _dptr(b), _ld(b), ZComplex(be.real, be.imag), _dptr(c),
+                                        _ld(c)), "zgemm_dev")
+
+    def larnv(self, dist, iseed, x):
+        seed = (_i * 4)(*[int(s) for s in iseed])
+        fn = "rnpx_dlarnv_dev" if _tpre(x) == "d" else "rnpx_zlarnv_dev"
+        _check(getattr(lib(), fn)(self.h, dist, seed, x.numel(), _dptr(x)), "larnv_dev")

@@ -1,0 +1,367 @@
+// rnpx_api.cpp -- the extern "C" surface of include/rnpx.h.
+//
+// Drop-in entry points keep RNP's argument lists and conventions
+// (RNP/LA/QR.hpp:129-131, 337-341, 509-512): the workspace query returns
+// exactly what the reference returns (its Tuning default nb = 32), zero
+// sizes return before the query as in the reference, and each matrix
+// argument may be host memory (staged through the handle's device buffers)
+// or device memory (used in place).  Invalid arguments return -i instead of
+// the reference's print-and-continue assert; the C++ surface in
+// include/rnpx_qr.hpp prints the reference's message for them.
+#include "rnpx_internal.hpp"
+
+#include <climits>
+#include <cstdio>
+#include <cstring>
+
+using namespace rnpx;
+
+namespace {
+
+constexpr size_t kRefNb = 32;  // QR::Tuning<T> defaults, RNP/LA/QR.hpp:43-50
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+bool fits_int(size_t v) { return v <= (size_t)INT_MAX; }
+
+// One matrix argument of a drop-in call: device pointer used in place, or a
+// host array staged into buf (rows x cols, leading dimension ld on the host,
+// rows on the device).
+template <typename T>
+struct Arg {
+  T* host = nullptr;
+  T* dev = nullptr;
+  long ld = 0;
+  size_t rows = 0, cols = 0;
+  bool staged = false;
+  int in(Handle* h, Buffer& buf, T* p, size_t r, size_t c, size_t ldp, bool copy_in) {
+    rows = r;
+    cols = c;
+    if (is_device_ptr(p)) {
+      dev = p;
+      ld = (long)ldp;
+      return RNPX_OK;
+    }
+    host = p;
+    staged = true;
+    ld = (long)(r ? r : 1);
+    hipError_t e = buf.ensure(sizeof(T) * (size_t)ld * (c ? c : 1), h);
+    if (e != hipSuccess) return hip_status(e);
+    dev = (T*)buf.p;
+    if (copy_in && r && c) {
+      e = hipMemcpy2DAsync(dev, sizeof(T) * ld, host, sizeof(T) * ldp, sizeof(T) * r, c, hipMemcpyHostToDevice,
+                           h->stream);
+      if (e != hipSuccess) return hip_status(e);
+    }
+    return RNPX_OK;
+  }
+  int out(Handle* h, size_t ldp) {
+    if (!staged || !rows || !cols) return RNPX_OK;
+    hipError_t e = hipMemcpy2DAsync(host, sizeof(T) * ldp, dev, sizeof(T) * ld, sizeof(T) * rows, cols,
+                                    hipMemcpyDeviceToHost, h->stream);
+    return hip_status(e);
+  }
+};
+
+template <typename T>
+int dropin_geqrf(size_t m, size_t n, T* a, size_t lda, T* tau, size_t* lwork) {
+  if (lda < m || lda == 0) return -4;
+  if (!lwork) return -6;
+  if (!(*lwork == 0 || *lwork >= m)) return -6;
+  const size_t k = m < n ? m : n;
+  if (k == 0) return RNPX_OK;
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU;
+  if ((st = A.in(h, h->stage1, a, m, n, lda, true))) return st;
+  if ((st = TAU.in(h, h->stage2, tau, k, 1, k, false))) return st;
+  st = geqrf_dev<T>(h, (int)m, (int)n, A.dev, A.ld, TAU.dev);
+  if (st) return st;
+  if ((st = A.out(h, lda))) return st;
+  if ((st = TAU.out(h, k))) return st;
+  return sync_handle(h);
+}
+
+template <typename T>
+int dropin_ormqr(const char* side, const char* trans, size_t m, size_t n, size_t k, const T* a, size_t lda,
+                 const T* tau, T* c, size_t ldc, size_t* lwork, bool query) {
+  if (!side || !trans) return !side ? -1 : -2;
+  if (ldc < m || ldc == 0) return -10;
+  if (m == 0 || n == 0 || k == 0) return RNPX_OK;
+  const bool left = ('L' == side[0]);
+  if (side[0] != 'L' && side[0] != 'R') return -1;
+  if (trans[0] != 'N' && trans[0] != 'C' && trans[0] != 'T') return -2;
+  if ((left && k > m) || (!left && k > n)) return -5;
+  const size_t nq = left ? m : n, nw = left ? n : m;
+  if (lda < k || lda < nq) return -7;
+  if (!lwork) return -11;
+  if (query) {
+    *lwork = kRefNb * nw + kRefNb * kRefNb;
+    return RNPX_OK;
+  }
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU, C;
+  if ((st = A.in(h, h->stage1, const_cast<T*>(a), nq, k, lda, true))) return st;
+  if ((st = TAU.in(h, h->stage2, const_cast<T*>(tau), k, 1, k, true))) return st;
+  if ((st = C.in(h, h->stage3, c, m, n, ldc, true))) return st;
+  st = ormqr_dev<T>(h, side[0], trans[0], (int)m, (int)n, (int)k, A.dev, A.ld, TAU.dev, C.dev, C.ld);
+  if (st) return st;
+  if ((st = C.out(h, ldc))) return st;
+  return sync_handle(h);
+}
+
+template <typename T>
+int dropin_orgqr(size_t m, size_t n, size_t k, T* a, size_t lda, const T* tau, size_t* lwork, bool query) {
+  if (k > n) return -3;
+  if (lda < m || lda == 0) return -5;
+  if (n == 0) return RNPX_OK;
+  if (!lwork) return -7;
+  if (query) {
+    *lwork = n * kRefNb;
+    return RNPX_OK;
+  }
+  if (n > m) return -2;
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU;
+  if ((st = A.in(h, h->stage1, a, m, n, lda, true))) return st;
+  if ((st = TAU.in(h, h->stage2, const_cast<T*>(tau), k ? k : 1, 1, k ? k : 1, k > 0))) return st;
+  st = orgqr_dev<T>(h, (int)m, (int)n, (int)k, A.dev, A.ld, TAU.dev);
+  if (st) return st;
+  if ((st = A.out(h, lda))) return st;
+  return sync_handle(h);
+}
+
+Handle* H(rnpx_handle h) { return reinterpret_cast<Handle*>(h); }
+
+}  // namespace
+
+struct rnpx_handle_s {};
+
+extern "C" {
+
+const char* rnpx_version(void) { return "rnpx 0.1 (gfx950, FP64 MFMA)"; }
+
+int rnpx_device_name(char* buf, size_t len) {
+  int st = Handle::check_gpu();
+  if (st) return st;
+  int dev = 0;
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, dev);
+  if (e != hipSuccess) return hip_status(e);
+  if (buf && len) {
+    snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  }
+  return RNPX_OK;
+}
+
+int rnpx_dgeqrf(size_t m, size_t n, double* a, size_t lda, double* tau, size_t* lwork, double* work) {
+  if (lwork && (work == nullptr || *lwork == 0)) {
+    if (lda < m || lda == 0) return -4;
+    if ((m < n ? m : n) == 0) return RNPX_OK;
+    *lwork = kRefNb * n;
+    return RNPX_OK;
+  }
+  return dropin_geqrf<double>(m, n, a, lda, tau, lwork);
+}
+int rnpx_zgeqrf(size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau, size_t* lwork,
+                rnpx_zcomplex* work) {
+  if (lwork && (work == nullptr || *lwork == 0)) {
+    if (lda < m || lda == 0) return -4;
+    if ((m < n ? m : n) == 0) return RNPX_OK;
+    *lwork = kRefNb * n;
+    return RNPX_OK;
+  }
+  return dropin_geqrf<zd>(m, n, reinterpret_cast<zd*>(a), lda, reinterpret_cast<zd*>(tau), lwork);
+}
+
+int rnpx_dormqr(const char* side, const char* trans, size_t m, size_t n, size_t k, const double* a, size_t lda,
+                const double* tau, double* c, size_t ldc, size_t* lwork, double* work) {
+  return dropin_ormqr<double>(side, trans, m, n, k, a, lda, tau, c, ldc, lwork,
+                              lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zunmqr(const char* side, const char* trans, size_t m, size_t n, size_t k, const rnpx_zcomplex* a, size_t lda,
+                const rnpx_zcomplex* tau, rnpx_zcomplex* c, size_t ldc, size_t* lwork, rnpx_zcomplex* work) {
+  return dropin_ormqr<zd>(side, trans, m, n, k, reinterpret_cast<const zd*>(a), lda,
+                          reinterpret_cast<const zd*>(tau), reinterpret_cast<zd*>(c), ldc, lwork,
+                          lwork && (work == nullptr || *lwork == 0));
+}
+
+int rnpx_dorgqr(size_t m, size_t n, size_t k, double* a, size_t lda, const double* tau, size_t* lwork,
+                double* work) {
+  return dropin_orgqr<double>(m, n, k, a, lda, tau, lwork, lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zungqr(size_t m, size_t n, size_t k, rnpx_zcomplex* a, size_t lda, const rnpx_zcomplex* tau, size_t* lwork,
+                rnpx_zcomplex* work) {
+  return dropin_orgqr<zd>(m, n, k, reinterpret_cast<zd*>(a), lda, reinterpret_cast<const zd*>(tau), lwork,
+                          lwork && (work == nullptr || *lwork == 0));
+}
+
+int rnpx_create(rnpx_handle* h, size_t hbm_budget_bytes) {
+  if (!h) return -1;
+  Handle* p = nullptr;
+  int st = create_handle(&p, hbm_budget_bytes);
+  if (st) return st;
+  *h = reinterpret_cast<rnpx_handle>(p);
+  return RNPX_OK;
+}
+int rnpx_destroy(rnpx_handle h) {
+  delete H(h);
+  return RNPX_OK;
+}
+int rnpx_set_stream(rnpx_handle h, void* s) {
+  if (!h) return -1;
+  H(h)->stream = s ? (hipStream_t)s : H(h)->own;
+  return RNPX_OK;
+}
+void* rnpx_get_stream(rnpx_handle h) { return h ? (void*)H(h)->stream : nullptr; }
+int rnpx_set_tuning(rnpx_handle h, const rnpx_tuning* t) {
+  if (!h) return -1;
+  if (!t) return -2;
+  H(h)->tune = *t;
+  return RNPX_OK;
+}
+int rnpx_get_tuning(rnpx_handle h, rnpx_tuning* t) {
+  if (!h) return -1;
+  if (!t) return -2;
+  *t = H(h)->tune;
+  return RNPX_OK;
+}
+int rnpx_synchronize(rnpx_handle h) {
+  if (!h) return -1;
+  return sync_handle(H(h));
+}
+size_t rnpx_workspace_bytes(rnpx_handle h) { return h ? H(h)->held_bytes() : 0; }
+int rnpx_profile_enable(rnpx_handle h, int on) {
+  if (!h) return -1;
+  prof_reset(H(h));
+  H(h)->prof = (on != 0);
+  return RNPX_OK;
+}
+int rnpx_profile_read(rnpx_handle h, int kernel, long* launches, double* ms, double* flops, double* bytes) {
+  if (!h) return -1;
+  if (kernel < 0 || kernel >= K_COUNT) return -2;
+  if (!launches || !ms || !flops || !bytes) return -3;
+  return prof_collect(H(h), kernel, launches, ms, flops, bytes);
+}
+
+#define RNPX_DEV_CHECK(h, m, n)                            \
+  if (!h) return -1;                                        \
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+
+int rnpx_dgeqrf_dev(rnpx_handle h, size_t m, size_t n, double* a, size_t lda, double* tau) {
+  RNPX_DEV_CHECK(h, m, n);
+  return geqrf_dev<double>(H(h), (int)m, (int)n, a, (long)lda, tau);
+}
+int rnpx_zgeqrf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau) {
+  RNPX_DEV_CHECK(h, m, n);
+  return geqrf_dev<zd>(H(h), (int)m, (int)n, reinterpret_cast<zd*>(a), (long)lda, reinterpret_cast<zd*>(tau));
+}
+int rnpx_dormqr_dev(rnpx_handle h, const char* side, const char* trans, size_t m, size_t n, size_t k, const double* a,
+                    size_t lda, const double* tau, double* c, size_t ldc) {
+  RNPX_DEV_CHECK(h, m, n);
+  if (!side) return -2;
+  if (!trans) return -3;
+  int st = ormqr_dev<double>(H(h), side[0], trans[0], (int)m, (int)n, (int)k, a, (long)lda, tau, c, (long)ldc);
+  return st < 0 && st > -100 ? st - 1 : st;
+}
+int rnpx_zunmqr_dev(rnpx_handle h, const char* side, const char* trans, size_t m, size_t n, size_t k,
+                    const rnpx_zcomplex* a, size_t lda, const rnpx_zcomplex* tau, rnpx_zcomplex* c, size_t ldc) {
+  RNPX_DEV_CHECK(h, m, n);
+  if (!side) return -2;
+  if (!trans) return -3;
+  int st = ormqr_dev<zd>(H(h), side[0], trans[0], (int)m, (int)n, (int)k, reinterpret_cast<const zd*>(a), (long)lda,
+                         reinterpret_cast<const zd*>(tau), reinterpret_cast<zd*>(c), (long)ldc);
+  return st < 0 && st > -100 ? st - 1 : st;
+}
+int rnpx_dorgqr_dev(rnpx_handle h, size_t m, size_t n, size_t k, double* a, size_t lda, const double* tau) {
+  RNPX_DEV_CHECK(h, m, n);
+  int st = orgqr_dev<double>(H(h), (int)m, (int)n, (int)k, a, (long)lda, tau);
+  return st < 0 && st > -100 ? st - 1 : st;
+}
+int rnpx_zungqr_dev(rnpx_handle h, size_t m, size_t n, size_t k, rnpx_zcomplex* a, size_t lda,
+                    const rnpx_zcomplex* tau) {
+  RNPX_DEV_CHECK(h, m, n);
+  int st = orgqr_dev<zd>(H(h), (int)m, (int)n, (int)k, reinterpret_cast<zd*>(a), (long)lda,
+                         reinterpret_cast<const zd*>(tau));
+  return st < 0 && st > -100 ? st - 1 : st;
+}
+int rnpx_dlarft_dev(rnpx_handle h, size_t n, size_t k, const double* v, size_t ldv, const double* tau, double* t,
+                    size_t ldt) {
+  RNPX_DEV_CHECK(h, n, k);
+  return larft_dev<double>(H(h), (int)n, (int)k, v, (long)ldv, tau, t, (long)ldt);
+}
+int rnpx_zlarft_dev(rnpx_handle h, size_t n, size_t k, const rnpx_zcomplex* v, size_t ldv, const rnpx_zcomplex* tau,
+                    rnpx_zcomplex* t, size_t ldt) {
+  RNPX_DEV_CHECK(h, n, k);
+  return larft_dev<zd>(H(h), (int)n, (int)k, reinterpret_cast<const zd*>(v), (long)ldv,
+                       reinterpret_cast<const zd*>(tau), reinterpret_cast<zd*>(t), (long)ldt);
+}
+int rnpx_dlarfb_dev(rnpx_handle h, const char* side, const char* trans, size_t m, size_t n, size_t k,
+                    const double* v, size_t ldv, const double* t, size_t ldt, double* c, size_t ldc) {
+  RNPX_DEV_CHECK(h, m, n);
+  if (!side) return -2;
+  if (!trans) return -3;
+  return larfb_dev<double>(H(h), side[0], trans[0], (int)m, (int)n, (int)k, v, (long)ldv, t, (long)ldt, c,
+                           (long)ldc);
+}
+int rnpx_zlarfb_dev(rnpx_handle h, const char* side, const char* trans, size_t m, size_t n, size_t k,
+                    const rnpx_zcomplex* v, size_t ldv, const rnpx_zcomplex* t, size_t ldt, rnpx_zcomplex* c,
+                    size_t ldc) {
+  RNPX_DEV_CHECK(h, m, n);
+  if (!side) return -2;
+  if (!trans) return -3;
+  return larfb_dev<zd>(H(h), side[0], trans[0], (int)m, (int)n, (int)k, reinterpret_cast<const zd*>(v), (long)ldv,
+                       reinterpret_cast<const zd*>(t), (long)ldt, reinterpret_cast<zd*>(c), (long)ldc);
+}
+int rnpx_dgemm_dev(rnpx_handle h, const char* transa, const char* transb, size_t m, size_t n, size_t k, double alpha,
+                   const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc) {
+  RNPX_DEV_CHECK(h, m, n);
+  if (!transa) return -2;
+  if (!transb) return -3;
+  if (!fits_int(k)) return RNPX_ERR_UNSUPPORTED;
+  return hip_status(gemm<double>(transa[0], transb[0], (int)m, (int)n, (int)k, alpha, a, (long)lda, b, (long)ldb,
+                                 beta, c, (long)ldc, 1, nullptr, H(h)->stream));
+}
+int rnpx_zgemm_dev(rnpx_handle h, const char* transa, const char* transb, size_t m, size_t n, size_t k,
+                   rnpx_zcomplex alpha, const rnpx_zcomplex* a, size_t lda, const rnpx_zcomplex* b, size_t ldb,
+                   rnpx_zcomplex beta, rnpx_zcomplex* c, size_t ldc) {
+  RNPX_DEV_CHECK(h, m, n);
+  if (!transa) return -2;
+  if (!transb) return -3;
+  if (!fits_int(k)) return RNPX_ERR_UNSUPPORTED;
+  return hip_status(gemm<zd>(transa[0], transb[0], (int)m, (int)n, (int)k, mkz(alpha.re, alpha.im),
+                             reinterpret_cast<const zd*>(a), (long)lda, reinterpret_cast<const zd*>(b), (long)ldb,
+                             mkz(beta.re, beta.im), reinterpret_cast<zd*>(c), (long)ldc, 1, nullptr, H(h)->stream));
+}
+int rnpx_dlarnv_dev(rnpx_handle h, int dist, const int iseed[4], size_t count, double* x) {
+  if (!h) return -1;
+  if (dist < 0 || dist > 2) return -2;
+  if (!iseed) return -3;
+  return hip_status(launch_larnv<double>(dist, iseed, (long long)count, x, H(h)->stream));
+}
+int rnpx_zlarnv_dev(rnpx_handle h, int dist, const int iseed[4], size_t count, rnpx_zcomplex* x) {
+  if (!h) return -1;
+  if (dist < 0 || dist > 2) return -2;
+  if (!iseed) return -3;
+  return hip_status(launch_larnv<zd>(dist, iseed, (long long)count, reinterpret_cast<zd*>(x), H(h)->stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+// rnpx_aux.hip -- small elementwise kernels: matrix set/copy, clean-V
+// extraction from a packed factor, zeroing, and RNP's random generator.
+#include "rnpx_kernels.hpp"
+
+namespace rnpx {
+
+template <typename T>
+__global__ void set_kernel(int m, int n, T off, T diag, T* A, long lda) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)m * n) return;
+  const int i = (int)(e % m), j = (int)(e / m);
+  A[(size_t)i + (size_t)j * lda] = (i == j) ? diag : off;
+}
+
+template <typename T>
+__global__ void copy_kernel(int m, int n, const T* A, long lda, T* B, long ldb) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)m * n) return;
+  const int i = (int)(e % m), j = (int)(e / m);
+  B[(size_t)i + (size_t)j * ldb] = A[(size_t)i + (size_t)j * lda];
+}
+
+template <typename T>
+__global__ void extract_v_kernel(int n, int k, const T* A, long lda, T* V, long ldv) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * k) return;
+  const int i = (int)(e % n), j = (int)(e / n);
+  V[(size_t)i + (size_t)j * ldv] = (i > j) ? A[(size_t)i + (size_t)j * lda] : (i == j ? S<T>::one() : S<T>::zero());
+}
+
+__global__ void zero_u32_kernel(unsigned* p, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) p[e] = 0u;
+}
+
+static inline unsigned blocks_for(long total) { return (unsigned)((total + 255) / 256); }
+
+template <typename T>
+hipError_t launch_set(int m, int n, T offdiag, T diag, T* A, long lda, hipStream_t st) {
+  if (m < 0 || n < 0 || lda < m) return hipErrorInvalidValue;
+  if ((long)m * n == 0) return hipSuccess;
+  hipLaunchKernelGGL((set_kernel<T>), dim3(blocks_for((long)m * n)), dim3(256), 0, st, m, n, offdiag, diag, A, lda);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_copy(int m, int n, const T* A, long lda, T* B, long ldb, hipStream_t st) {
+  if (m < 0 || n < 0 || lda < m || ldb < m) return hipErrorInvalidValue;
+  if ((long)m * n == 0) return hipSuccess;
+  hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks_for((long)m * n)), dim3(256), 0, st, m, n, A, lda, B, ldb);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_extract_v(int n, int k, const T* A, long lda, T* V, long ldv, hipStream_t st) {
+  if (n < 0 || k < 0 || lda < n || ldv < n) return hipErrorInvalidValue;
+  if ((long)n * k == 0) return hipSuccess;
+  hipLaunchKernelGGL((extract_v_kernel<T>), dim3(blocks_for((long)n * k)), dim3(256), 0, st, n, k, A, lda, V, ldv);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_zero_rows(int rows, int n, T* A, long lda, hipStream_t st) {
+  return launch_set<T>(rows, n, S<T>::zero(), S<T>::zero(), A, lda, st);
+}
+
+hipError_t launch_zero_u32(unsigned* p, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+// ---- RNP's 48-bit MCG (RNP/Random.hpp:45-183) ----
+// Stream value k (0-based) = 1 - (seed * a^(k+1) mod 2^48) / 2^48; the low 48
+// bits of a 64-bit product are the product mod 2^48.
+__device__ inline uint64_t mcg_pow(uint64_t e) {
+  const uint64_t mask = (1ull << 48) - 1;
+  uint64_t r = 1, b = 33952834046453ull;
+  while (e) {
+    if (e & 1) r = (r * b) & mask;
+    b = (b * b) & mask;
+    e >>= 1;
+  }
+  return r;
+}
+
+template <typename T>
+__global__ void larnv_kernel(int dist, uint64_t seed, long long count, T* x, int per_thread) {
+  const uint64_t mask = (1ull << 48) - 1;
+  const long long e0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * per_thread;
+  if (e0 >= count) return;
+  constexpr int U = S<T>::cplx ? 2 : 1;       // uniforms per element (Uniform01/_11)
+  const int upe = (dist == 2) ? 2 : U;           // Normal01 always draws pairs
+  uint64_t st = (seed * mcg_pow((uint64_t)(e0 * upe) + 1)) & mask;
+  const uint64_t a = 33952834046453ull;
+  const double twopi = 6.28318530717958647692528676656;
+  for (int t = 0; t < per_thread && e0 + t < count; ++t) {
+    double u[2];
+    for (int q = 0; q < upe; ++q) {
+      u[q] = 1.0 - (double)st * 0x1p-48;
+      st = (st * a) & mask;
+    }
+    T v;
+    if (dist == 2) {
+      const double r = sqrt(-2.0 * log1p(-u[0]));
+      const double ang = twopi * u[1];
+      if constexpr (S<T>::cplx) v = S<T>::make(r * cos(ang), r * sin(ang));
+      else v = S<T>::make(r * cos(ang), 0.0);
+    } else if (dist == 1) {
+      v = S<T>::make(2.0 * u[0] - 1.0, U == 2 ? 2.0 * u[1] - 1.0 : 0.0);
+    } else {
+      v = S<T>::make(u[0], U == 2 ? u[1] : 0.0);
+    }
+    x[e0 + t] = v;
+  }
+}
+
+template <typename T>
+hipError_t launch_larnv(int dist, const int iseed[4], long long count, T* x, hipStream_t st) {
+  if (dist < 0 || dist > 2 || count < 0) return hipErrorInvalidValue;
+  if (count == 0) return hipSuccess;
+  const uint64_t seed = ((uint64_t)(iseed[0] & 4095) << 36) | ((uint64_t)(iseed[1] & 4095) << 24) |
+                        ((uint64_t)(iseed[2] & 4095) << 12) | (uint64_t)((iseed[3] | 1) & 4095);
+  const int per = 64;
+  const long long threads = (count + per - 1) / per;
+  hipLaunchKernelGGL((larnv_kernel<T>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, dist, seed, count,
+                     x, per);
+  return hipGetLastError();
+}
+
+#define RNPX_INST(T)                                                                  \
+  template hipError_t launch_set<T>(int, int, T, T, T*, long, hipStream_t);           \
+  template hipError_t launch_copy<T>(int, int, const T*, long, T*, long, hipStream_t); \
+  template hipError_t launch_extract_v<T>(int, int, const T*, long, T*, long, hipStream_t); \
+  template hipError_t launch_zero_rows<T>(int, int, T*, long, hipStream_t);           \
+  template hipError_t launch_larnv<T>(int, const int*, long long, T*, hipStream_t);
+RNPX_INST(double)
+RNPX_INST(zd)
+#undef RNPX_INST
+
+}  // namespace rnpx

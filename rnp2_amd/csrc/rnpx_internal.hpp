@@ -1,0 +1,72 @@
+// rnpx_internal.hpp -- host-side types shared by the orchestration files:
+// the handle (stream, tuning, device workspace) and the templated device
+// entry points behind the C ABI.
+#pragma once
+#include "rnpx_kernels.hpp"
+#include "../../include/rnpx.h"
+
+#include <memory>
+#include <vector>
+
+namespace rnpx {
+
+struct Handle;
+
+// A growable device allocation.
+struct Buffer {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t b, Handle* h);
+  void release();
+};
+
+// Live per-kernel timing (HIP events recorded on the handle's stream around
+// the launches of one kernel family) with the launches' algorithmic work.
+enum KernelId { K_UPDATE = 0, K_WGEMM = 1, K_PANEL = 2, K_TFACTOR = 3, K_COUNT = 4 };
+struct ProfRec {
+  int kid;
+  hipEvent_t a, b;
+  double flops, bytes;
+};
+
+struct Handle {
+  int device = 0;
+  int num_cus = 1;
+  hipStream_t stream = nullptr;  // stream all work is queued on
+  hipStream_t own = nullptr;     // the handle's own stream
+  bool own_stream = false;
+  size_t budget = 0;             // 0 = unbounded
+  rnpx_tuning tune = {0, 0, 0, 0};
+  Buffer vbuf, vfull, gram, tbuf, wbuf, zbuf, part, ppart, rowbuf, sync, flag, misc;
+  Buffer stage1, stage2, stage3;  // host-argument staging for the drop-in calls
+  bool prof = false;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t take_event();
+  // Bracket one launch: call prof_begin before and prof_end after.
+  int prof_begin(hipEvent_t* a);
+  void prof_end(int kid, hipEvent_t a, double flops, double bytes);
+  size_t held_bytes() const;
+  static int check_gpu();
+  ~Handle();
+};
+
+int hip_status(hipError_t e);
+int create_handle(Handle** out, size_t budget);
+Handle* default_handle(int* status);
+int sync_handle(Handle* h);
+int prof_collect(Handle* h, int kid, long* launches, double* ms, double* flops, double* bytes);
+void prof_reset(Handle* h);
+
+template <typename T> int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau);
+template <typename T>
+int ormqr_dev(Handle* h, char side, char trans, int m, int n, int k, const T* A, long lda, const T* tau, T* C,
+              long ldc);
+template <typename T> int orgqr_dev(Handle* h, int m, int n, int k, T* A, long lda, const T* tau);
+template <typename T>
+int larft_dev(Handle* h, int n, int k, const T* V, long ldv, const T* tau, T* Tout, long ldt);
+template <typename T>
+int larfb_dev(Handle* h, char side, char trans, int m, int n, int k, const T* V, long ldv, const T* Tm, long ldt,
+              T* C, long ldc);
+
+}  // namespace rnpx

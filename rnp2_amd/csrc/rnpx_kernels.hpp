@@ -1,0 +1,77 @@
+// rnpx_kernels.hpp -- launch interfaces of the gfx950 kernels used by the
+// host orchestration (rnpx_qr.cpp).  Every launcher checks the shapes it is
+// given against what its kernel and grid assume and returns
+// hipErrorInvalidValue instead of launching when they do not match.
+#pragma once
+#include "rnpx_device.hpp"
+
+namespace rnpx {
+
+// Panel kernel: at most this many workgroups (each holds 256*RPT rows).
+constexpr int kMaxPanelGroups = 128;
+// Grid-barrier timeout: 2 s of the 100 MHz wall clock.
+constexpr uint64_t kBarrierTimeoutTicks = 200000000ull;
+
+template <typename T>
+struct PanelArgs {
+  int mp, nbp;         // panel rows, panel columns (nbp <= NB)
+  T* A; long lda;      // panel top-left (in: panel, out: R and V packed)
+  T* tau;              // out: tau[0 .. min(mp,nbp))
+  T* V; long ldv;      // out: clean V (unit diagonal, zeros above), or null
+  T* gram;             // out: NB x NB, gram(j,c) for j < c, tau_c on the diagonal
+  T* part;             // scratch: 2 * G * NB elements
+  T* rowbuf;           // scratch: 2 * NB elements
+  unsigned* counter;   // scratch: arrival counter, zero at launch
+  unsigned* abortf;    // sticky error flag: set when a barrier times out
+};
+
+template <typename T>
+hipError_t launch_panel(const PanelArgs<T>& a, int nb, int rpt, hipStream_t st);
+
+// T factor of a forward, columnwise block reflector from its Gram matrix
+// (the larft analogue, RNP/LA/Reflector.hpp:573-626): k <= NB reflectors,
+// gram(j,c) = V(:,j)^H V(:,c) for j < c and tau_c on the diagonal.
+// Writes the NB x NB block of T (zeros below the diagonal and beyond k).
+template <typename T>
+hipError_t launch_tfactor(const T* gram, int ldg, int k, int nb, T* Tout, int ldt, hipStream_t st);
+
+// Gram of a clean V and tau into the layout launch_tfactor reads:
+// gram(j,c) = sum_r conj(V(r,j)) V(r,c) for j < c, tau_c on the diagonal.
+// Deterministic two-stage reduction over row chunks (scratch holds
+// nchunks * nb * nb elements).
+template <typename T>
+hipError_t launch_gram(int n, int k, const T* V, long ldv, const T* tau, T* gram, int nb,
+                       T* scratch, int nchunks, hipStream_t st);
+
+// General matrix multiply on FP64 MFMA:
+//   C = alpha * op(A) * op(B) + beta * C,  op in {'N','T','C'}, column-major.
+// With ksplit > 1 the K range is cut into ksplit slices whose raw products go
+// to part[s] (M x N, ld M) and a second kernel sums them in slice order and
+// applies alpha/beta (deterministic).  part must hold ksplit*M*N elements.
+template <typename T>
+hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long lda,
+                const T* B, long ldb, T beta, T* C, long ldc, int ksplit, T* part,
+                hipStream_t st);
+
+// Small helpers.
+template <typename T>
+hipError_t launch_set(int m, int n, T offdiag, T diag, T* A, long lda, hipStream_t st);
+template <typename T>
+hipError_t launch_copy(int m, int n, const T* A, long lda, T* B, long ldb, hipStream_t st);
+// Clean V (unit lower trapezoidal, n x k) out of a packed factor.
+template <typename T>
+hipError_t launch_extract_v(int n, int k, const T* A, long lda, T* V, long ldv, hipStream_t st);
+// Zero the strictly upper triangle of columns [0,k) above row j (rows < j)
+// of an m x n block: A(l, j) = 0 for l < row_limit, j in [0, n).
+template <typename T>
+hipError_t launch_zero_rows(int rows, int n, T* A, long lda, hipStream_t st);
+hipError_t launch_zero_u32(unsigned* p, int n, hipStream_t st);
+
+// Random fill with RNP's generator (RNP/Random.hpp:45-320): the 48-bit MCG
+// stream of one seed, value k = 1 - (seed * a^(k+1) mod 2^48) / 2^48 with the
+// stream laid out column by column; dist 0 Uniform01, 1 Uniform_11,
+// 2 Normal01 (pairs of uniforms through sqrt(-2 log1p(-a)) cos/sin 2 pi b).
+template <typename T>
+hipError_t launch_larnv(int dist, const int iseed[4], long long count, T* x, hipStream_t st);
+
+}  // namespace rnpx
