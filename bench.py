@@ -70,6 +70,7 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--nb", type=int, default=0)
     ap.add_argument("--rpt", type=int, default=0)
+    ap.add_argument("--no-lookahead", action="store_true", help="serialize panels and updates")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -85,8 +86,8 @@ def main():
     m, n = args.m, args.n
     seed = [1, 2, 3, 5]
     h = R.Handle()
-    if args.nb or args.rpt:
-        h.set_tuning(nb=args.nb, panel_rpt=args.rpt)
+    if args.nb or args.rpt or args.no_lookahead:
+        h.set_tuning(nb=args.nb, panel_rpt=args.rpt, lookahead=-1 if args.no_lookahead else 0)
     a0 = R.colmajor_empty(m, n, torch.float64)
     h.larnv(2, seed, a0)          # RNP Normal01 stream, column-major fill
     a = R.colmajor_empty(m, n, torch.float64)

@@ -89,28 +89,72 @@ __device__ inline zd shfl_t(zd v, int src) { return mkz(__shfl(v.x, src, 64), __
 __device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
 __device__ __forceinline__ zd sel(bool c, zd a, zd b) { return mkz(c ? a.x : b.x, c ? a.y : b.y); }
 
+// ---- VALU cross-lane exchange (no LDS round trip) -------------------------
+__device__ __forceinline__ unsigned lo32(double d) { return (unsigned)(unsigned long long)__double_as_longlong(d); }
+__device__ __forceinline__ unsigned hi32(double d) {
+  return (unsigned)((unsigned long long)__double_as_longlong(d) >> 32);
+}
+__device__ __forceinline__ double mkd(unsigned lo, unsigned hi) {
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// One butterfly step of a reduce-scatter: x holds the lane's partials for the
+// lower half of the current index block, y for the upper half.  Lanes with
+// bit OFF clear keep the lower half, the others the upper half; the result
+// is own + partner for the kept half.  OFF 32/16 use gfx950's
+// v_permlane32_swap / v_permlane16_swap, OFF 8/4/2/1 DPP within a row
+// (row_mirror / row_half_mirror pair the two halves of a 16 / 8 lane group,
+// quad_perm the xor-2 / xor-1 partners).  Calling it with x == y adds the
+// partner's value to every lane (a full step).
+template <int OFF>
+__device__ __forceinline__ double bfly(double x, double y, int lane) {
+  if constexpr (OFF == 32 || OFF == 16) {
+    unsigned xl = lo32(x), xh = hi32(x), yl = lo32(y), yh = hi32(y);
+    if constexpr (OFF == 32) {
+      auto l = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+      auto h = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+      return mkd(l[0], h[0]) + mkd(l[1], h[1]);
+    } else {
+      auto l = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+      auto h = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+      return mkd(l[0], h[0]) + mkd(l[1], h[1]);
+    }
+  } else {
+    constexpr int ctrl = (OFF == 8) ? 0x140 : (OFF == 4) ? 0x141 : (OFF == 2) ? 0x4E : 0xB1;
+    const bool upper = (lane & OFF) != 0;
+    const double send = upper ? x : y;
+    const double keep = upper ? y : x;
+    const unsigned rl = __builtin_amdgcn_update_dpp(0u, lo32(send), ctrl, 0xF, 0xF, false);
+    const unsigned rh = __builtin_amdgcn_update_dpp(0u, hi32(send), ctrl, 0xF, 0xF, false);
+    return keep + mkd(rl, rh);
+  }
+}
+template <int OFF>
+__device__ __forceinline__ zd bfly(zd x, zd y, int lane) {
+  return mkz(bfly<OFF>(x.x, y.x, lane), bfly<OFF>(x.y, y.y, lane));
+}
+
+// Broadcast lane `src` (a compile-time or wave-uniform index) to SGPRs.
+__device__ __forceinline__ double readlane_t(double v, int src) {
+  return mkd(__builtin_amdgcn_readlane(lo32(v), src), __builtin_amdgcn_readlane(hi32(v), src));
+}
+__device__ __forceinline__ zd readlane_t(zd v, int src) { return mkz(readlane_t(v.x, src), readlane_t(v.y, src)); }
+
 // Reduce-scatter of N per-lane values across the wavefront: afterwards lane
 // l holds, in v[0 .. max(N/64,1)), the wave-wide sums of indices
 // base(l) .. base(l)+N/64-1 with base(l) = (l * N/64) mod N (for N < 64 the
-// upper lanes duplicate the lower ones).  Cost: about N shuffles per lane.
-// Written as template recursion so every index is a compile-time constant.
+// upper lanes duplicate the lower ones).  About N butterfly steps per lane,
+// all VALU.  Template recursion keeps every index a compile-time constant.
 template <typename T, int N, int OFF>
-__device__ __forceinline__ void rs_full(T (&v)[N]) {
-  if constexpr (OFF >= 1) {
+__device__ __forceinline__ void rs_full(T (&v)[N], int lane) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = v[i] + shfl_xor_t(v[i], OFF);
-  }
+  for (int i = 0; i < N; ++i) v[i] = bfly<OFF>(v[i], v[i], lane);
 }
 template <typename T, int N, int H, int OFF>
 __device__ __forceinline__ void rs_halve(T (&v)[N], int lane) {
   if constexpr (H >= 1 && OFF >= 1) {
-    const bool upper = (lane & OFF) != 0;
 #pragma unroll
-    for (int i = 0; i < H; ++i) {
-      T send = sel(upper, v[i], v[i + H]);
-      T keep = sel(upper, v[i + H], v[i]);
-      v[i] = keep + shfl_xor_t(send, OFF);
-    }
+    for (int i = 0; i < H; ++i) v[i] = bfly<OFF>(v[i], v[i + H], lane);
     rs_halve<T, N, H / 2, OFF / 2>(v, lane);
   }
 }
@@ -120,16 +164,16 @@ __device__ __forceinline__ void wave_reduce_scatter(T (&v)[N], int lane) {
   if constexpr (N >= 64) {
     rs_halve<T, N, N / 2, 32>(v, lane);
   } else if constexpr (N == 32) {
-    rs_full<T, N, 32>(v);
+    rs_full<T, N, 32>(v, lane);
     rs_halve<T, N, 16, 16>(v, lane);
   } else if constexpr (N == 16) {
-    rs_full<T, N, 32>(v);
-    rs_full<T, N, 16>(v);
+    rs_full<T, N, 32>(v, lane);
+    rs_full<T, N, 16>(v, lane);
     rs_halve<T, N, 8, 8>(v, lane);
   } else {
-    rs_full<T, N, 32>(v);
-    rs_full<T, N, 16>(v);
-    rs_full<T, N, 8>(v);
+    rs_full<T, N, 32>(v, lane);
+    rs_full<T, N, 16>(v, lane);
+    rs_full<T, N, 8>(v, lane);
     rs_halve<T, N, 4, 4>(v, lane);
   }
 }

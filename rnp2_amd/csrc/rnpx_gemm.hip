@@ -5,7 +5,8 @@
 //
 //   C = alpha * op(A) * op(B) + beta * C,   op in {N, T, C}, column-major.
 //
-// Tiles: a 256-thread workgroup computes a BM x BN tile of C as 2 x 2 waves,
+// Tiles: a 256-thread workgroup computes a BM x BN tile of C as 2 x 2 waves
+// (BM, BN in {64, 128}),
 // each wave (BM/2) x (BN/2) in 16 x 16 MFMA blocks (v_mfma_f64_16x16x4_f64).
 // op(A) and op(B) K-slices of depth BK are staged through LDS with the
 // global loads coalesced along whichever index is contiguous in memory, and
@@ -19,11 +20,17 @@
 // gemm_reduce sums the slices in order (results do not depend on timing).
 #include "rnpx_kernels.hpp"
 
+#include <cstdlib>
+
 namespace rnpx {
 
+// Tile shapes: (BM, BN) per workgroup, BK the K depth staged per LDS slice.
+// Real: 128 x 128 (or 64 x 128 when M <= 64), BK 32 -- one slice is 8K
+// cycles of MFMA per wave, longer than a global-load round trip, so the
+// register prefetch of the next slice hides its latency.  Complex: 64 x 64.
 template <typename T> struct Cfg;
-template <> struct Cfg<double> { static constexpr int BM = 128, BN = 128, BK = 16; };
-template <> struct Cfg<zd> { static constexpr int BM = 64, BN = 64, BK = 16; };
+template <> struct Cfg<double> { static constexpr int BK = 32; };
+template <> struct Cfg<zd> { static constexpr int BK = 16; };
 
 template <typename T, char OP>
 __device__ inline T ld_op(const T* p) {
@@ -114,6 +121,7 @@ template <> struct Acc<double> {
   __device__ inline void zero() { v = f64x4{0.0, 0.0, 0.0, 0.0}; }
   __device__ inline void mma(double b, double a) { v = mfma16(b, a, v); }
   __device__ inline double get(int r) const { return v[r]; }
+  __device__ inline void set(int r, double x) { v[r] = x; }
 };
 template <> struct Acc<zd> {
   f64x4 re, im;
@@ -125,13 +133,20 @@ template <> struct Acc<zd> {
     im = mfma16(b.y, a.x, im);
   }
   __device__ inline zd get(int r) const { return mkz(re[r], im[r]); }
+  __device__ inline void set(int r, zd x) {
+    re[r] = x.x;
+    im[r] = x.y;
+  }
 };
 
-template <typename T, char TA, char TB>
+// CINIT: C is loaded into the accumulators before the K loop (beta = 1,
+// alpha = +-1, the sign folded into the op(A) fragments), so the C read
+// overlaps the first slice instead of stalling the epilogue.
+template <typename T, char TA, char TB, int BM, int BN, bool CINIT>
 __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, long lda,
                                                    const T* __restrict__ B, long ldb, T beta, T* __restrict__ C,
                                                    long ldc, int kslice, T* __restrict__ part) {
-  constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN, BK = Cfg<T>::BK;
+  constexpr int BK = Cfg<T>::BK;
   using Tl = Tile<T, BM, BN, BK, TA, TB>;
   constexpr int FM = BM / 32, FN = BN / 32;
   __shared__ T As[BK][BM + Tl::PADA];
@@ -142,17 +157,28 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
   const int kbeg = blockIdx.z * kslice;
   const int kend = min(K, kbeg + kslice);
 
-  Acc<T> acc[FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j].zero();
-
   T ra[Tl::NA], rb[Tl::NBv];
   if (kbeg < kend) {
     Tl::fetchA(ra, A, lda, M, kend, m0, kbeg, tid);
     Tl::fetchB(rb, B, ldb, N, kend, n0, kbeg, tid);
   }
+  Acc<T> acc[FM][FN];
+  // accumulator item r of lane l is C(row0 + l%16, col0 + l/16 + 4r)
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      acc[i][j].zero();
+      if constexpr (CINIT) {
+        const int gi = m0 + wm * (BM / 2) + i * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gj = n0 + wn * (BN / 2) + j * 16 + (lane >> 4) + 4 * r;
+          if (gi < M && gj < N) acc[i][j].set(r, C[(size_t)gi + (size_t)gj * ldc]);
+        }
+      }
+    }
+  const bool neg = CINIT && (S<T>::re(alpha) < 0.0);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();
     Tl::storeA(ra, As, tid);
@@ -166,7 +192,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
     for (int kk = 0; kk < BK; kk += 4) {
       T af[FM], bf[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) af[i] = As[kk + (lane >> 4)][wm * (BM / 2) + i * 16 + (lane & 15)];
+      for (int i = 0; i < FM; ++i) {
+        af[i] = As[kk + (lane >> 4)][wm * (BM / 2) + i * 16 + (lane & 15)];
+        if (neg) af[i] = S<T>::zero() - af[i];
+      }
 #pragma unroll
       for (int j = 0; j < FN; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * (BN / 2) + j * 16 + (lane & 15)];
 #pragma unroll
@@ -188,7 +217,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
         const int gj = n0 + wn * (BN / 2) + j * 16 + (lane >> 4) + 4 * r;
         if (gi < M && gj < N) {
           const T v = acc[i][j].get(r);
-          if (part) {
+          if (CINIT) {
+            C[(size_t)gi + (size_t)gj * ldc] = v;
+          } else if (part) {
             part[(size_t)blockIdx.z * M * N + (size_t)gi + (size_t)gj * M] = v;
           } else {
             T* cp = C + (size_t)gi + (size_t)gj * ldc;
@@ -212,12 +243,53 @@ __global__ void gemm_reduce_kernel(int M, int N, int ns, const T* __restrict__ p
   *cp = S<T>::is_zero(beta) ? alpha * s : alpha * s + beta * (*cp);
 }
 
+template <typename T, char TA, char TB, int BM, int BN, bool CINIT>
+static void launch_gemm_tile(int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta, T* C,
+                             long ldc, int kslice, int ns, T* part, hipStream_t st) {
+  dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, ns);
+  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, BM, BN, CINIT>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb,
+                     beta, C, ldc, kslice, part);
+}
+
+// Real tile choice: RNPX_GEMM_TILE (read once) overrides the default
+// (0 auto, 1 128x128, 2 128x64, 3 64x128, 4 64x64) -- a tuning knob.
+static int gemm_tile_override() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("RNPX_GEMM_TILE");
+    v = e ? atoi(e) : 0;
+  }
+  return v;
+}
+
+template <typename T, char TA, char TB, bool CI>
+static void launch_real_tiles(int cfg, int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb,
+                              T beta, T* C, long ldc, int kslice, int ns, T* part, hipStream_t st) {
+  switch (cfg) {
+    case 1: launch_gemm_tile<T, TA, TB, 128, 128, CI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, part, st); break;
+    case 2: launch_gemm_tile<T, TA, TB, 128, 64, CI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, part, st); break;
+    case 3: launch_gemm_tile<T, TA, TB, 64, 128, CI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, part, st); break;
+    default: launch_gemm_tile<T, TA, TB, 64, 64, CI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, part, st); break;
+  }
+}
+
 template <typename T, char TA, char TB>
 static void launch_gemm_t(int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta, T* C,
                           long ldc, int kslice, int ns, T* part, hipStream_t st) {
-  dim3 grid((M + Cfg<T>::BM - 1) / Cfg<T>::BM, (N + Cfg<T>::BN - 1) / Cfg<T>::BN, ns);
-  hipLaunchKernelGGL((gemm_kernel<T, TA, TB>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                     kslice, part);
+  const bool cinit = (ns == 1) && !part && S<T>::re(beta) == 1.0 && S<T>::im(beta) == 0.0 &&
+                     (S<T>::re(alpha) == 1.0 || S<T>::re(alpha) == -1.0) && S<T>::im(alpha) == 0.0;
+  if constexpr (S<T>::cplx) {
+    if (cinit) launch_gemm_tile<T, TA, TB, 64, 64, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, part, st);
+    else launch_gemm_tile<T, TA, TB, 64, 64, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, part, st);
+  } else {
+    // Measured on MI355X: 64 x 64 tiles (2-3 workgroups per CU) are fastest
+    // for the trailing-update shapes (K = 64..128 or M = 64..128); 64 x 128
+    // for large products in all three dimensions.
+    int cfg = gemm_tile_override();
+    if (cfg == 0) cfg = (M >= 2048 && N >= 2048 && K >= 1024) ? 3 : 4;
+    if (cinit) launch_real_tiles<T, TA, TB, true>(cfg, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, part, st);
+    else launch_real_tiles<T, TA, TB, false>(cfg, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, part, st);
+  }
 }
 
 template <typename T>

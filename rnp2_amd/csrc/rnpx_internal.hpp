@@ -38,14 +38,18 @@ struct Handle {
   size_t budget = 0;             // 0 = unbounded
   rnpx_tuning tune = {0, 0, 0, 0};
   Buffer vbuf, vfull, gram, tbuf, wbuf, zbuf, part, ppart, rowbuf, sync, flag, misc;
+  Buffer vbuf2, gram2, tbuf2;    // second set for the look-ahead pipeline
+  hipStream_t pstream = nullptr;  // panel stream of the look-ahead pipeline
+  hipEvent_t ev_pan[2] = {nullptr, nullptr}, ev_next = nullptr, ev_start = nullptr;
   Buffer stage1, stage2, stage3;  // host-argument staging for the drop-in calls
   bool prof = false;
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> event_pool;
   hipEvent_t take_event();
   // Bracket one launch: call prof_begin before and prof_end after.
-  int prof_begin(hipEvent_t* a);
-  void prof_end(int kid, hipEvent_t a, double flops, double bytes);
+  int prof_begin(hipEvent_t* a, hipStream_t s = nullptr);
+  void prof_end(int kid, hipEvent_t a, double flops, double bytes, hipStream_t s = nullptr);
+  int ensure_pipeline();
   size_t held_bytes() const;
   static int check_gpu();
   ~Handle();

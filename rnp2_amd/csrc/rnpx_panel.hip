@@ -112,7 +112,6 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
   const int kp = (mp < nbp) ? mp : nbp;
 
   __shared__ T red[4][NB];
-  __shared__ T wsh[NB];
   __shared__ double dsh[4];
 
   T a[RPT][NB];
@@ -128,12 +127,43 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
   }
 
   unsigned epoch = 0;
-  // After the barrier: sum the G partials of index j in a fixed order.
-  auto gather = [&](int j) -> T {
-    const T* base = p.part + (size_t)(epoch & 1u) * (size_t)G * NB + j;
-    T s = S<T>::zero();
-    for (int gg = 0; gg < G; ++gg) s = s + base[(size_t)gg * NB];
-    return s;
+  // After the barrier: all 256 threads fetch the G x NB partials at once
+  // (one memory latency instead of G) -- thread t sums groups t/NB,
+  // t/NB + SUB, ... of index t%NB -- then every lane j < NB adds the SUB
+  // sub-sums in order.  The order is fixed for a given G.
+  constexpr int SUB = 256 / NB;
+  __shared__ T gred[SUB * NB];
+  auto gather_all = [&](T (&Sj)[(NB + 63) / 64]) {
+    {
+      // MAXQ loads are issued back to back before any is consumed.
+      constexpr int MAXQ = 8;
+      const int j = tid % NB, sub = tid / NB;
+      const T* base = p.part + (size_t)(epoch & 1u) * (size_t)G * NB + j;
+      T s = S<T>::zero();
+      for (int g0 = sub; g0 < G; g0 += SUB * MAXQ) {
+        T buf[MAXQ];
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) {
+          const int gg = g0 + q * SUB;
+          buf[q] = S<T>::zero();
+          if (gg < G) buf[q] = base[(size_t)gg * NB];
+        }
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) s = s + buf[q];
+      }
+      gred[sub * NB + j] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < (NB + 63) / 64; ++u) {
+      const int j = lane + 64 * u;
+      T s = S<T>::zero();
+      if (j < NB) {
+#pragma unroll
+        for (int q = 0; q < SUB; ++q) s = s + gred[q * NB + j];
+      }
+      Sj[u] = s;
+    }
   };
   auto barrier = [&]() {
     ++epoch;
@@ -163,18 +193,15 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
                                       tid);
     }
     barrier();
-    // Every wave gathers what it needs: lane j < NB handles index j (and j+64).
+    // Every wave ends up with lane j < NB holding index j (and j+64).
     T Sj[(NB + 63) / 64], rcj[(NB + 63) / 64];
 #pragma unroll
     for (int u = 0; u < (NB + 63) / 64; ++u) {
       const int j = lane + 64 * u;
-      Sj[u] = S<T>::zero();
       rcj[u] = S<T>::zero();
-      if (j < NB) {
-        Sj[u] = gather(j);
-        rcj[u] = p.rowbuf[(c & 1) * NB + j];
-      }
+      if (j < NB) rcj[u] = p.rowbuf[(c & 1) * NB + j];
     }
+    gather_all(Sj);
     ++epoch;
     const int cu = c >> 6, cl = c & 63;
     T Sc = Sj[0], alpha = rcj[0];
@@ -183,8 +210,8 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
       Sc = sel(cu == u, Sj[u], Sc);
       alpha = sel(cu == u, rcj[u], alpha);
     }
-    Sc = shfl_t(Sc, cl);
-    alpha = shfl_t(alpha, cl);
+    Sc = readlane_t(Sc, cl);  // c is wave-uniform
+    alpha = readlane_t(alpha, cl);
     const double ssq = S<T>::re(Sc);
     const double ar = S<T>::re(alpha), ai = S<T>::im(alpha);
 
@@ -273,32 +300,25 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
                                         p.part + (size_t)(epoch & 1u) * (size_t)G * NB + (size_t)g * NB, lane, wave,
                                         tid);
         barrier();
-#pragma unroll
-        for (int u = 0; u < (NB + 63) / 64; ++u) {
-          const int j = lane + 64 * u;
-          if (j < NB) Sj[u] = gather(j);
-        }
+        __syncthreads();
+        gather_all(Sj);
         ++epoch;
       }
     }
 
-    // ---- w_j (times conj(tau)) and Gram entries ----
+    // ---- w_j (times conj(tau)) and Gram entries, lane j of every wave ----
     const T ctau = S<T>::conj(tau);
+    T ctw[(NB + 63) / 64];
 #pragma unroll
     for (int u = 0; u < (NB + 63) / 64; ++u) {
       const int j = lane + 64 * u;
-      if (j < NB) {
-        // fast: S_j = sum conj(x_r) a_rj ; slow: S_j = sum conj(v_r) a_rj
-        const T dotv = sel(fast, S<T>::conj(inv) * Sj[u], Sj[u]);
-        if (wave == 0) {
-          const T w = rcj[u] + dotv;
-          wsh[j] = sel(j > c && j < nbp && !identity, ctau * w, S<T>::zero());
-          if (g == 0 && j < c) {
-            // gram(j,c) = conj(V(c,j)) + sum_{r>c} conj(V(r,j)) v_r
-            p.gram[(size_t)j + (size_t)c * NB] =
-                sel(identity, S<T>::zero(), S<T>::conj(rcj[u]) + S<T>::conj(dotv));
-          }
-        }
+      // fast: S_j = sum conj(x_r) a_rj ; slow: S_j = sum conj(v_r) a_rj
+      const T dotv = sel(fast, S<T>::conj(inv) * Sj[u], Sj[u]);
+      const T w = rcj[u] + dotv;
+      ctw[u] = sel(j > c && j < nbp && !identity, ctau * w, S<T>::zero());
+      if (g == 0 && wave == 0 && j < c) {
+        // gram(j,c) = conj(V(c,j)) + sum_{r>c} conj(V(r,j)) v_r
+        p.gram[(size_t)j + (size_t)c * NB] = sel(identity, S<T>::zero(), S<T>::conj(rcj[u]) + S<T>::conj(dotv));
       }
     }
     if (g == 0 && tid == 0) {
@@ -306,29 +326,28 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
       p.tau[c] = tt;
       p.gram[(size_t)c + (size_t)c * NB] = tt;
     }
-    __syncthreads();
 
     // ---- apply H^H to the rows this thread owns ----
+    // a(r,j) -= coef_r * conj(tau) w_j for j > c (coef = 1 on the diagonal
+    // row, v_r below it, 0 above), column c becomes beta / v.  The
+    // conj(tau) w_j come from lane j by v_readlane (wave-uniform, SGPRs).
     if (!identity) {
+      T coef[RPT], newc[RPT];
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        if (row[q] == c) {
+        const T vq = ldexp_t(x[q], escale) * inv;
+        coef[q] = sel(row[q] == c, S<T>::one(), sel(below[q], vq, S<T>::zero()));
+        newc[q] = sel(row[q] == c, S<T>::make(beta, 0.0), sel(below[q], vq, x[q]));
+      }
 #pragma unroll
-          for (int j = 0; j < NB; ++j) {
-            const T upd = a[q][j] - wsh[j];
-            a[q][j] = sel(j == c, S<T>::make(beta, 0.0), sel(j > c, upd, a[q][j]));
-          }
-        } else if (below[q]) {
-          const T vq = ldexp_t(x[q], escale) * inv;
+      for (int j = 0; j < NB; ++j) {
+        const T cw = readlane_t(ctw[j >> 6], j & 63);
 #pragma unroll
-          for (int j = 0; j < NB; ++j) {
-            const T upd = a[q][j] - vq * wsh[j];
-            a[q][j] = sel(j == c, vq, sel(j > c, upd, a[q][j]));
-          }
-        }
+        for (int q = 0; q < RPT; ++q) a[q][j] = sel(j == c, newc[q], a[q][j] - coef[q] * cw);
       }
     }
-    __syncthreads();
+    // No barrier here: red/gred are next written after the next grid
+    // barrier, whose __syncthreads orders them after these reads.
   }
 
   // ---- write back R/V and the clean V ----

@@ -43,7 +43,7 @@ void Buffer::release() {
 size_t Handle::held_bytes() const {
   size_t s = 0;
   for (const Buffer* b : {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc, &stage1, &stage2,
-                          &stage3})
+                          &stage3, &vbuf2, &gram2, &tbuf2})
     s += b->bytes;
   return s;
 }
@@ -58,19 +58,29 @@ hipEvent_t Handle::take_event() {
   if (hipEventCreate(&e) != hipSuccess) return nullptr;
   return e;
 }
-int Handle::prof_begin(hipEvent_t* a) {
+int Handle::prof_begin(hipEvent_t* a, hipStream_t s) {
   *a = nullptr;
   if (!prof) return 0;
   *a = take_event();
-  if (*a) hipEventRecord(*a, stream);
+  if (*a) hipEventRecord(*a, s ? s : stream);
   return 0;
 }
-void Handle::prof_end(int kid, hipEvent_t a, double flops, double bytes) {
+void Handle::prof_end(int kid, hipEvent_t a, double flops, double bytes, hipStream_t s) {
   if (!prof || !a) return;
   hipEvent_t b = take_event();
   if (!b) return;
-  hipEventRecord(b, stream);
+  hipEventRecord(b, s ? s : stream);
   recs.push_back(ProfRec{kid, a, b, flops, bytes});
+}
+int Handle::ensure_pipeline() {
+  if (pstream) return RNPX_OK;
+  hipError_t e = hipStreamCreateWithFlags(&pstream, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_status(e);
+  for (hipEvent_t* ev : {&ev_pan[0], &ev_pan[1], &ev_next, &ev_start}) {
+    e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_status(e);
+  }
+  return RNPX_OK;
 }
 
 Handle::~Handle() {
@@ -80,8 +90,12 @@ Handle::~Handle() {
     hipEventDestroy(r.b);
   }
   for (auto e : event_pool) hipEventDestroy(e);
+  if (pstream) hipStreamSynchronize(pstream);
+  for (hipEvent_t ev : {ev_pan[0], ev_pan[1], ev_next, ev_start})
+    if (ev) hipEventDestroy(ev);
+  if (pstream) hipStreamDestroy(pstream);
   for (Buffer* b : {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc, &stage1, &stage2,
-                          &stage3})
+                          &stage3, &vbuf2, &gram2, &tbuf2})
     b->release();
   if (own_stream && own) hipStreamDestroy(own);
 }
@@ -247,6 +261,38 @@ static int tfactor_from_v(Handle* h, int n, int k, int nb, const T* V, long ldv,
 // Factor (QR.hpp:129-197) on device memory.
 // ---------------------------------------------------------------------------
 template <typename T>
+static int launch_panel_step(Handle* h, int p, int i, int ib, int m, int nb, T* A, long lda, T* tau, T* V, T* gram,
+                             hipStream_t st) {
+  const int mp = m - i;
+  PanelArgs<T> pa;
+  pa.mp = mp;
+  pa.nbp = ib;
+  pa.A = A + (size_t)i + (size_t)i * lda;
+  pa.lda = lda;
+  pa.tau = tau + i;
+  pa.V = V;
+  pa.ldv = mp;
+  pa.gram = gram;
+  pa.part = (T*)h->ppart.p;
+  pa.rowbuf = (T*)h->rowbuf.p;
+  pa.counter = (unsigned*)h->sync.p + 1 + p;
+  pa.abortf = (unsigned*)h->flag.p;
+  const int rpt = pick_rpt(h, mp, nb, S<T>::cplx);
+  hipEvent_t ev;
+  h->prof_begin(&ev, st);
+  RNPX_TRY(launch_panel<T>(pa, nb, rpt, st));
+  // panel traffic: the mp x ib panel read and written once, plus V and tau
+  h->prof_end(K_PANEL, ev, (S<T>::cplx ? 4.0 : 1.0) * (2.0 * mp * ib * ib - 2.0 / 3.0 * ib * ib * ib),
+              sizeof(T) * (3.0 * mp * ib + ib), st);
+  return RNPX_OK;
+}
+
+// Factor (QR.hpp:129-197) on device memory.  With look-ahead (the default)
+// panel p+1 is factored on a second stream while the trailing update of
+// panel p runs: the update of panel p is split into the next panel's columns
+// (done first, then the panel stream is released) and the rest.  V, Gram and
+// T are double-buffered between consecutive panels.
+template <typename T>
 int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
   if (m < 0) return -2;
   if (n < 0) return -3;
@@ -261,44 +307,77 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
   // Every panel workgroup must be resident at once (they meet at grid
   // barriers): one 256-thread workgroup per CU at these register counts.
   if (G0 > std::min(kMaxPanelGroups, h->num_cus)) return RNPX_ERR_UNSUPPORTED;
-  RNPX_TRY(h->vbuf.ensure(sizeof(T) * (size_t)m * nb, h));
-  RNPX_TRY(h->gram.ensure(sizeof(T) * (size_t)nb * nb, h));
-  RNPX_TRY(h->tbuf.ensure(sizeof(T) * (size_t)nb * nb, h));
+  const bool look = (h->tune.lookahead >= 0) && npan > 1;
+  Buffer* vb[2] = {&h->vbuf, &h->vbuf2};
+  Buffer* gb[2] = {&h->gram, &h->gram2};
+  Buffer* tb[2] = {&h->tbuf, &h->tbuf2};
+  for (int q = 0; q < (look ? 2 : 1); ++q) {
+    RNPX_TRY(vb[q]->ensure(sizeof(T) * (size_t)m * nb, h));
+    RNPX_TRY(gb[q]->ensure(sizeof(T) * (size_t)nb * nb, h));
+    RNPX_TRY(tb[q]->ensure(sizeof(T) * (size_t)nb * nb, h));
+  }
   RNPX_TRY(h->ppart.ensure(sizeof(T) * 2 * (size_t)kMaxPanelGroups * nb + 64, h));
   RNPX_TRY(h->rowbuf.ensure(sizeof(T) * 2 * (size_t)nb, h));
   RNPX_TRY(h->sync.ensure(sizeof(unsigned) * (size_t)(npan + 1), h));
   RNPX_TRY(launch_zero_u32((unsigned*)h->sync.p, npan + 1, st));
-  T* V = (T*)h->vbuf.p;
-  for (int p = 0, i = 0; i < k; i += nb, ++p) {
-    const int ib = std::min(nb, k - i);
-    const int mp = m - i;
-    PanelArgs<T> pa;
-    pa.mp = mp;
-    pa.nbp = ib;
-    pa.A = A + (size_t)i + (size_t)i * lda;
-    pa.lda = lda;
-    pa.tau = tau + i;
-    pa.V = V;
-    pa.ldv = mp;
-    pa.gram = (T*)h->gram.p;
-    pa.part = (T*)h->ppart.p;
-    pa.rowbuf = (T*)h->rowbuf.p;
-    pa.counter = (unsigned*)h->sync.p + 1 + p;
-    pa.abortf = (unsigned*)h->flag.p;
-    const int rpt = pick_rpt(h, mp, nb, S<T>::cplx);
+
+  auto update = [&](int p, int i, int ib, int c0, int c1) -> int {
+    // columns [c0, c1) of the trailing matrix, rows [i, m), reflectors of panel p
+    if (c1 <= c0) return RNPX_OK;
+    const int q = look ? (p & 1) : 0;
+    return apply_block_dev<T>(h, 'L', 'C', m - i, c1 - c0, ib, (const T*)vb[q]->p, m - i, (const T*)tb[q]->p, nb,
+                              A + (size_t)i + (size_t)c0 * lda, lda);
+  };
+  auto tfactor = [&](int p, int ib) -> int {
+    const int q = look ? (p & 1) : 0;
     hipEvent_t ev;
     h->prof_begin(&ev);
-    RNPX_TRY(launch_panel<T>(pa, nb, rpt, st));
-    // panel traffic: the mp x ib panel read and written once, plus V and tau
-    h->prof_end(K_PANEL, ev, (S<T>::cplx ? 4.0 : 1.0) * (2.0 * mp * ib * ib - 2.0 / 3.0 * ib * ib * ib),
-                sizeof(T) * (3.0 * mp * ib + ib));
-    if (i + ib < n) {
-      h->prof_begin(&ev);
-      RNPX_TRY(launch_tfactor<T>((const T*)h->gram.p, nb, ib, nb, (T*)h->tbuf.p, nb, st));
-      h->prof_end(K_TFACTOR, ev, 0.0, 0.0);
-      int s = apply_block_dev<T>(h, 'L', 'C', mp, n - i - ib, ib, V, mp, (const T*)h->tbuf.p, nb,
-                                 A + (size_t)i + (size_t)(i + ib) * lda, lda);
+    RNPX_TRY(launch_tfactor<T>((const T*)gb[q]->p, nb, ib, nb, (T*)tb[q]->p, nb, st));
+    h->prof_end(K_TFACTOR, ev, 0.0, 0.0);
+    return RNPX_OK;
+  };
+
+  if (!look) {
+    for (int p = 0, i = 0; i < k; i += nb, ++p) {
+      const int ib = std::min(nb, k - i);
+      int s = launch_panel_step<T>(h, p, i, ib, m, nb, A, lda, tau, (T*)vb[0]->p, (T*)gb[0]->p, st);
       if (s) return s;
+      if (i + ib < n) {
+        if ((s = tfactor(p, ib))) return s;
+        if ((s = update(p, i, ib, i + ib, n))) return s;
+      }
+    }
+    return RNPX_OK;
+  }
+
+  int s = h->ensure_pipeline();
+  if (s) return s;
+  hipStream_t ps = h->pstream;
+  RNPX_TRY(hipEventRecord(h->ev_start, st));
+  RNPX_TRY(hipStreamWaitEvent(ps, h->ev_start, 0));
+  {
+    const int ib0 = std::min(nb, k);
+    if ((s = launch_panel_step<T>(h, 0, 0, ib0, m, nb, A, lda, tau, (T*)vb[0]->p, (T*)gb[0]->p, ps))) return s;
+    RNPX_TRY(hipEventRecord(h->ev_pan[0], ps));
+  }
+  for (int p = 0, i = 0; i < k; i += nb, ++p) {
+    const int ib = std::min(nb, k - i);
+    RNPX_TRY(hipStreamWaitEvent(st, h->ev_pan[p & 1], 0));
+    if (i + ib >= n) break;
+    if ((s = tfactor(p, ib))) return s;
+    const int i1 = i + ib;  // next panel's first column
+    if (i1 < k) {
+      const int ib1 = std::min(nb, k - i1);
+      if ((s = update(p, i, ib, i1, i1 + ib1))) return s;
+      RNPX_TRY(hipEventRecord(h->ev_next, st));
+      RNPX_TRY(hipStreamWaitEvent(ps, h->ev_next, 0));
+      const int q1 = (p + 1) & 1;
+      if ((s = launch_panel_step<T>(h, p + 1, i1, ib1, m, nb, A, lda, tau, (T*)vb[q1]->p, (T*)gb[q1]->p, ps)))
+        return s;
+      RNPX_TRY(hipEventRecord(h->ev_pan[q1], ps));
+      if ((s = update(p, i, ib, i1 + ib1, n))) return s;
+    } else {
+      if ((s = update(p, i, ib, i1, n))) return s;
     }
   }
   return RNPX_OK;
