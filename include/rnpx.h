@@ -43,6 +43,18 @@ enum {
 	RNPX_ERR_NOTCONV = -1005    /* iterative refinement did not reach its target */
 };
 
+/* One problem of a batched call: A is m-by-n at A + a_off (leading dimension
+ * lda), tau has min(m,n) entries at tau + tau_off; with nrhs > 0 (and
+ * m >= n) the least-squares solution of min ||A x - b|| for the m-by-nrhs
+ * right-hand sides at B + b_off (leading dimension ldb) overwrites their
+ * first n rows (Q^H b below them).  Offsets count elements. */
+typedef struct rnpx_qr_desc {
+	int m, n, lda, ldb, nrhs, pad;
+	long long a_off, tau_off, b_off;
+} rnpx_qr_desc;
+
+typedef struct rnpx_batched_plan_s *rnpx_batched_plan;
+
 /* Tuning hook: the counterpart of specialising QR::Tuning<T>
  * (RNP/LA/QR.hpp:41-51).  Zero fields keep the built-in choice. */
 typedef struct rnpx_tuning {
@@ -120,6 +132,34 @@ int rnpx_dorgqr_dev(rnpx_handle h, size_t m, size_t n, size_t k, double *a, size
                     const double *tau);
 int rnpx_zungqr_dev(rnpx_handle h, size_t m, size_t n, size_t k, rnpx_zcomplex *a, size_t lda,
                     const rnpx_zcomplex *tau);
+
+/* Batched QR / least squares (no counterpart in the reference: RNP factors
+ * one matrix per call).  desc is a host array of count descriptors; A, tau,
+ * B are device buffers.  Problems are cost-sorted and run by persistent
+ * teams in one launch per tier; asynchronous on the handle's stream. */
+int rnpx_dgeqrf_batched(rnpx_handle h, int count, const rnpx_qr_desc *desc, double *A, double *tau,
+                        double *B);
+int rnpx_zgeqrf_batched(rnpx_handle h, int count, const rnpx_qr_desc *desc, rnpx_zcomplex *A,
+                        rnpx_zcomplex *tau, rnpx_zcomplex *B);
+/* The same with the descriptor upload done once and the launches captured
+ * in a hipGraph bound to A/tau/B; run replays it on the handle's stream. */
+int rnpx_dgeqrf_batched_plan(rnpx_handle h, int count, const rnpx_qr_desc *desc, double *A,
+                             double *tau, double *B, rnpx_batched_plan *plan);
+int rnpx_zgeqrf_batched_plan(rnpx_handle h, int count, const rnpx_qr_desc *desc,
+                             rnpx_zcomplex *A, rnpx_zcomplex *tau, rnpx_zcomplex *B,
+                             rnpx_batched_plan *plan);
+int rnpx_batched_plan_run(rnpx_batched_plan plan);
+int rnpx_batched_plan_destroy(rnpx_batched_plan plan);
+
+/* Least squares min ||A X - B||, m >= n, device pointers: Factor, then
+ * MultQ("L","C") on B, then back substitution with R (the composition of
+ * RNP/LA/QR.hpp:129, QR.hpp:337 and SolveTrM, RNP/BLAS/BLAS_mix.hpp:2096).
+ * A is overwritten by its factorization (tau, length n), B (m-by-nrhs) by
+ * Q^H B with X in its first n rows. */
+int rnpx_dgels_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, double *a, size_t lda,
+                   double *tau, double *b, size_t ldb);
+int rnpx_zgels_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, rnpx_zcomplex *a, size_t lda,
+                   rnpx_zcomplex *tau, rnpx_zcomplex *b, size_t ldb);
 
 /* Building blocks, device pointers (the reference's Reflector functions):
  * T factor of k <= nb forward columnwise reflectors stored in a (n-by-k,

@@ -73,6 +73,14 @@ SIGNATURES = {
     "rnpx_zunmqr_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
     "rnpx_dorgqr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p]),
     "rnpx_zungqr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p]),
+    "rnpx_dgels_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
+    "rnpx_zgels_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
+    "rnpx_dgeqrf_batched": (_i, [_p, _i, _p, _p, _p, _p]),
+    "rnpx_zgeqrf_batched": (_i, [_p, _i, _p, _p, _p, _p]),
+    "rnpx_dgeqrf_batched_plan": (_i, [_p, _i, _p, _p, _p, _p, ctypes.POINTER(_p)]),
+    "rnpx_zgeqrf_batched_plan": (_i, [_p, _i, _p, _p, _p, _p, ctypes.POINTER(_p)]),
+    "rnpx_batched_plan_run": (_i, [_p]),
+    "rnpx_batched_plan_destroy": (_i, [_p]),
     "rnpx_dlarft_dev": (_i, [_p, _sz, _sz, _p, _sz, _p, _p, _sz]),
     "rnpx_zlarft_dev": (_i, [_p, _sz, _sz, _p, _sz, _p, _p, _sz]),
     "rnpx_dlarfb_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, _p, _sz, _p, _sz, _p, _sz]),
@@ -85,6 +93,25 @@ SIGNATURES = {
     "rnpx_version": (_s, []),
     "rnpx_device_name": (_i, [ctypes.c_char_p, _sz]),
 }
+
+
+# rnpx_qr_desc (include/rnpx.h): six int32 then three int64 offsets.
+QR_DESC = np.dtype([("m", np.int32), ("n", np.int32), ("lda", np.int32), ("ldb", np.int32), ("nrhs", np.int32),
+                    ("pad", np.int32), ("a_off", np.int64), ("tau_off", np.int64), ("b_off", np.int64)])
+
+
+def make_batch(shapes, nrhs=0):
+    """Descriptor table for problems of the given (m, n) shapes packed one after
+    another (lda = m, ldb = m).  Returns (desc, a_elems, tau_elems, b_elems)."""
+    desc = np.zeros(len(shapes), dtype=QR_DESC)
+    a = t = b = 0
+    for i, (m, n) in enumerate(shapes):
+        r = nrhs if (nrhs and m >= n) else 0
+        desc[i] = (m, n, max(m, 1), max(m, 1), r, 0, a, t, b)
+        a += m * n
+        t += min(m, n)
+        b += m * r
+    return desc, a, t, b
 
 
 def lib():
@@ -315,6 +342,28 @@ class Handle:
         fn = "rnpx_dorgqr_dev" if _tpre(a) == "d" else "rnpx_zungqr_dev"
         _check(getattr(lib(), fn)(self.h, m, n, k, _dptr(a), _ld(a), _dptr(tau)), "orgqr_dev")
 
+    def gels(self, a, tau, b):
+        """min ||A X - B|| in place: A -> factor, B -> Q^H B with X on top."""
+        m, n = a.shape
+        nrhs = b.shape[1] if b.dim() == 2 else 1
+        fn = "rnpx_dgels_dev" if _tpre(a) == "d" else "rnpx_zgels_dev"
+        _check(getattr(lib(), fn)(self.h, m, n, nrhs, _dptr(a), _ld(a), _dptr(tau), _dptr(b), _ld(b)), "gels_dev")
+
+    def geqrf_batched(self, desc, A, tau, B=None):
+        """Batched QR (and LS where desc['nrhs'] > 0) over packed device buffers."""
+        desc = np.ascontiguousarray(desc, dtype=QR_DESC)
+        fn = "rnpx_dgeqrf_batched" if _tpre(A) == "d" else "rnpx_zgeqrf_batched"
+        _check(getattr(lib(), fn)(self.h, len(desc), desc.ctypes.data_as(_p), _dptr(A), _dptr(tau),
+                                  None if B is None else _dptr(B)), "geqrf_batched")
+
+    def batched_plan(self, desc, A, tau, B=None):
+        desc = np.ascontiguousarray(desc, dtype=QR_DESC)
+        plan = ctypes.c_void_p()
+        fn = "rnpx_dgeqrf_batched_plan" if _tpre(A) == "d" else "rnpx_zgeqrf_batched_plan"
+        _check(getattr(lib(), fn)(self.h, len(desc), desc.ctypes.data_as(_p), _dptr(A), _dptr(tau),
+                                  None if B is None else _dptr(B), ctypes.byref(plan)), "batched_plan")
+        return BatchedPlan(plan)
+
     def larft(self, v, tau, t):
         n, k = v.shape
         fn = "rnpx_dlarft_dev" if _tpre(v) == "d" else "rnpx_zlarft_dev"
@@ -343,3 +392,24 @@ class Handle:
         seed = (_i * 4)(*[int(s) for s in iseed])
         fn = "rnpx_dlarnv_dev" if _tpre(x) == "d" else "rnpx_zlarnv_dev"
         _check(getattr(lib(), fn)(self.h, dist, seed, x.numel(), _dptr(x)), "larnv_dev")
+
+
+class BatchedPlan:
+    """A captured batched launch (hipGraph) bound to its device buffers."""
+
+    def __init__(self, p):
+        self.p = p
+
+    def run(self):
+        _check(lib().rnpx_batched_plan_run(self.p), "batched_plan_run")
+
+    def close(self):
+        if self.p:
+            lib().rnpx_batched_plan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -303,6 +303,44 @@ int rnpx_zungqr_dev(rnpx_handle h, size_t m, size_t n, size_t k, rnpx_zcomplex* 
                          reinterpret_cast<const zd*>(tau));
   return st < 0 && st > -100 ? st - 1 : st;
 }
+int rnpx_dgels_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, double* a, size_t lda, double* tau, double* b,
+                   size_t ldb) {
+  RNPX_DEV_CHECK(h, m, n);
+  if (!fits_int(nrhs)) return RNPX_ERR_UNSUPPORTED;
+  return gels_dev<double>(H(h), (int)m, (int)n, (int)nrhs, a, (long)lda, tau, b, (long)ldb);
+}
+int rnpx_zgels_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau,
+                   rnpx_zcomplex* b, size_t ldb) {
+  RNPX_DEV_CHECK(h, m, n);
+  if (!fits_int(nrhs)) return RNPX_ERR_UNSUPPORTED;
+  return gels_dev<zd>(H(h), (int)m, (int)n, (int)nrhs, reinterpret_cast<zd*>(a), (long)lda,
+                      reinterpret_cast<zd*>(tau), reinterpret_cast<zd*>(b), (long)ldb);
+}
+int rnpx_dgeqrf_batched(rnpx_handle h, int count, const rnpx_qr_desc* desc, double* A, double* tau, double* B) {
+  if (!h) return -1;
+  return geqrf_batched<double>(H(h), count, desc, A, tau, B);
+}
+int rnpx_zgeqrf_batched(rnpx_handle h, int count, const rnpx_qr_desc* desc, rnpx_zcomplex* A, rnpx_zcomplex* tau,
+                        rnpx_zcomplex* B) {
+  if (!h) return -1;
+  return geqrf_batched<zd>(H(h), count, desc, reinterpret_cast<zd*>(A), reinterpret_cast<zd*>(tau),
+                           reinterpret_cast<zd*>(B));
+}
+int rnpx_dgeqrf_batched_plan(rnpx_handle h, int count, const rnpx_qr_desc* desc, double* A, double* tau, double* B,
+                             rnpx_batched_plan* plan) {
+  if (!h) return -1;
+  return geqrf_batched_plan<double>(H(h), count, desc, A, tau, B, reinterpret_cast<BatchedPlan**>(plan));
+}
+int rnpx_zgeqrf_batched_plan(rnpx_handle h, int count, const rnpx_qr_desc* desc, rnpx_zcomplex* A,
+                             rnpx_zcomplex* tau, rnpx_zcomplex* B, rnpx_batched_plan* plan) {
+  if (!h) return -1;
+  return geqrf_batched_plan<zd>(H(h), count, desc, reinterpret_cast<zd*>(A), reinterpret_cast<zd*>(tau),
+                                reinterpret_cast<zd*>(B), reinterpret_cast<BatchedPlan**>(plan));
+}
+int rnpx_batched_plan_run(rnpx_batched_plan plan) { return batched_plan_run(reinterpret_cast<BatchedPlan*>(plan)); }
+int rnpx_batched_plan_destroy(rnpx_batched_plan plan) {
+  return batched_plan_destroy(reinterpret_cast<BatchedPlan*>(plan));
+}
 int rnpx_dlarft_dev(rnpx_handle h, size_t n, size_t k, const double* v, size_t ldv, const double* tau, double* t,
                     size_t ldt) {
   RNPX_DEV_CHECK(h, n, k);

@@ -139,3 +139,57 @@ RNPX_INST(zd)
 #undef RNPX_INST
 
 }  // namespace rnpx
+
+namespace rnpx {
+
+// Back substitution R X = B for an upper-triangular n x n R (BLAS xtrsm
+// "L","U","N","N", the SolveTrM of RNP/BLAS/BLAS_mix.hpp:2096, in place on the
+// first n rows of B).  One workgroup per right-hand side; 64-row blocks from
+// the bottom: wave 0 solves the diagonal block with the row values in its
+// lanes (each x_i broadcast by v_readlane), then all threads update the rows
+// above with the block's contribution.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_un_kernel(int n, const T* __restrict__ R, long ldr, T* __restrict__ B,
+                                                      long ldb) {
+  T* b = B + (size_t)blockIdx.x * ldb;
+  __shared__ T xs[64];
+  const int t = threadIdx.x, lane = t & 63;
+  for (int i1 = n; i1 > 0; i1 -= 64) {
+    const int i0 = i1 > 64 ? i1 - 64 : 0;
+    const int bs = i1 - i0;
+    if (t < 64) {
+      T y = S<T>::zero();
+      if (lane < bs) y = b[i0 + lane];
+      for (int i = bs - 1; i >= 0; --i) {
+        const T rii = R[(size_t)(i0 + i) + (size_t)(i0 + i) * ldr];
+        T yi = readlane_t(y, i);
+        yi = S<T>::cplx ? yi * recip_t(rii) : S<T>::make(S<T>::re(yi) / S<T>::re(rii), 0.0);
+        if (lane == i) y = yi;
+        if (lane < i) y = y - R[(size_t)(i0 + lane) + (size_t)(i0 + i) * ldr] * yi;
+      }
+      if (lane < bs) {
+        xs[lane] = y;
+        b[i0 + lane] = y;
+      }
+    }
+    __syncthreads();
+    for (int r = t; r < i0; r += 256) {
+      T s = S<T>::zero();
+      for (int j = 0; j < bs; ++j) s = s + R[(size_t)r + (size_t)(i0 + j) * ldr] * xs[j];
+      b[r] = b[r] - s;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+hipError_t launch_trsm_un(int n, int nrhs, const T* R, long ldr, T* B, long ldb, hipStream_t st) {
+  if (n < 0 || nrhs < 0 || ldr < n || ldb < n) return hipErrorInvalidValue;
+  if (n == 0 || nrhs == 0) return hipSuccess;
+  hipLaunchKernelGGL((trsm_un_kernel<T>), dim3(nrhs), dim3(256), 0, st, n, R, ldr, B, ldb);
+  return hipGetLastError();
+}
+template hipError_t launch_trsm_un<double>(int, int, const double*, long, double*, long, hipStream_t);
+template hipError_t launch_trsm_un<zd>(int, int, const zd*, long, zd*, long, hipStream_t);
+
+}  // namespace rnpx

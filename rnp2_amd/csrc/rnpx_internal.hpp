@@ -42,6 +42,8 @@ struct Handle {
   hipStream_t pstream = nullptr;  // panel stream of the look-ahead pipeline
   hipEvent_t ev_pan[2] = {nullptr, nullptr}, ev_next = nullptr, ev_start = nullptr;
   Buffer stage1, stage2, stage3;  // host-argument staging for the drop-in calls
+  void* batch = nullptr;               // batched-call descriptor buffers
+  void (*batch_free)(void*) = nullptr;
   bool prof = false;
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> event_pool;
@@ -63,6 +65,13 @@ int prof_collect(Handle* h, int kid, long* launches, double* ms, double* flops, 
 void prof_reset(Handle* h);
 
 template <typename T> int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau);
+template <typename T> int gels_dev(Handle* h, int m, int n, int nrhs, T* A, long lda, T* tau, T* B, long ldb);
+template <typename T> int geqrf_batched(Handle* h, int count, const rnpx_qr_desc* desc, T* A, T* tau, T* B);
+struct BatchedPlan;
+template <typename T>
+int geqrf_batched_plan(Handle* h, int count, const rnpx_qr_desc* desc, T* A, T* tau, T* B, BatchedPlan** out);
+int batched_plan_run(BatchedPlan* p);
+int batched_plan_destroy(BatchedPlan* p);
 template <typename T>
 int ormqr_dev(Handle* h, char side, char trans, int m, int n, int k, const T* A, long lda, const T* tau, T* C,
               long ldc);

@@ -4,6 +4,13 @@
 // hipErrorInvalidValue instead of launching when they do not match.
 #pragma once
 #include "rnpx_device.hpp"
+#include "../../include/rnpx.h"
+
+#define RNPX_LAUNCH_CHECK(x)            \
+  do {                                  \
+    hipError_t _le = (x);               \
+    if (_le != hipSuccess) return _le;  \
+  } while (0)
 
 namespace rnpx {
 
@@ -73,5 +80,15 @@ hipError_t launch_zero_u32(unsigned* p, int n, hipStream_t st);
 // 2 Normal01 (pairs of uniforms through sqrt(-2 log1p(-a)) cos/sin 2 pi b).
 template <typename T>
 hipError_t launch_larnv(int dist, const int iseed[4], long long count, T* x, hipStream_t st);
+
+// R X = B, R upper triangular n x n, B's first n rows (n x nrhs) in place.
+template <typename T>
+hipError_t launch_trsm_un(int n, int nrhs, const T* R, long ldr, T* B, long ldb, hipStream_t st);
+
+// Batched QR / LS: tier 0 = one wave per problem in LDS (lds_bytes of
+// dynamic LDS per team), tier 1 = 256 threads per problem in place.
+template <typename T>
+hipError_t launch_batched(int tier, int count, const int* order, const rnpx_qr_desc* desc, T* A, T* tau, T* B,
+                          unsigned* counter, int teams, int lds_bytes, hipStream_t st);
 
 }  // namespace rnpx

@@ -1,0 +1,207 @@
+// rnpx_ls.cpp -- least-squares drivers and the batched form.
+//
+// Dense least squares min ||A X - B|| (m >= n) is the composition the
+// reference leaves to its callers: QR::Factor (RNP/LA/QR.hpp:129),
+// QR::MultQ("L","C") on B (QR.hpp:337) and back substitution with R
+// (SolveTrM "L","U","N","N", RNP/BLAS/BLAS_mix.hpp:2096).
+//
+// Batched: descriptors are validated and cost-sorted on the host (largest
+// first, ties by index), split into the LDS tier and the in-place tier, and
+// run by persistent teams pulling from an atomic counter.  A plan uploads the
+// table once and captures both tier launches (fork/join over the handle's
+// two streams) in a hipGraph that replays with one call.
+#include "rnpx_internal.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace rnpx {
+
+#define RNPX_TRY(x)                               \
+  do {                                            \
+    hipError_t _e = (x);                          \
+    if (_e != hipSuccess) return hip_status(_e);  \
+  } while (0)
+
+template <typename T>
+int gels_dev(Handle* h, int m, int n, int nrhs, T* A, long lda, T* tau, T* B, long ldb) {
+  if (m < 0) return -2;
+  if (n < 0 || n > m) return -3;
+  if (nrhs < 0) return -4;
+  if (lda < std::max(1, m)) return -6;
+  if (ldb < std::max(1, m)) return -9;
+  if (n == 0 || nrhs == 0) return RNPX_OK;
+  int s = geqrf_dev<T>(h, m, n, A, lda, tau);
+  if (s) return s;
+  s = ormqr_dev<T>(h, 'L', 'C', m, nrhs, n, A, lda, tau, B, ldb);
+  if (s) return s;
+  RNPX_TRY(launch_trsm_un<T>(n, nrhs, A, lda, B, ldb, h->stream));
+  return RNPX_OK;
+}
+
+template int gels_dev<double>(Handle*, int, int, int, double*, long, double*, double*, long);
+template int gels_dev<zd>(Handle*, int, int, int, zd*, long, zd*, zd*, long);
+
+// ---------------------------------------------------------------------------
+// Batched
+// ---------------------------------------------------------------------------
+constexpr int kBatchLdsBytes = 40 * 1024;  // per one-wave team: four teams per CU
+
+struct BatchSetup {
+  std::vector<int> small, large;  // cost-sorted problem indices per tier
+};
+
+template <typename T>
+static int batch_setup(int count, const rnpx_qr_desc* desc, BatchSetup* out) {
+  std::vector<double> cost(count);
+  for (int i = 0; i < count; ++i) {
+    const rnpx_qr_desc& d = desc[i];
+    if (d.m < 0 || d.n < 0 || d.lda < std::max(1, d.m) || d.nrhs < 0) return -3;
+    if (d.nrhs > 0 && (d.ldb < std::max(1, d.m) || d.m < d.n)) return -3;
+    if (d.a_off < 0 || d.tau_off < 0 || d.b_off < 0) return -3;
+    const double k = std::min(d.m, d.n);
+    cost[i] = (double)d.m * d.n * k + (double)d.nrhs * d.m * d.n;
+    const long need = (long)(d.m | 1) * d.n * (long)sizeof(T);
+    (need <= kBatchLdsBytes ? out->small : out->large).push_back(i);
+  }
+  auto by_cost = [&](int a, int b) { return cost[a] != cost[b] ? cost[a] > cost[b] : a < b; };
+  std::sort(out->small.begin(), out->small.end(), by_cost);
+  std::sort(out->large.begin(), out->large.end(), by_cost);
+  return RNPX_OK;
+}
+
+struct BatchBuffers {
+  Buffer desc, order, counters;
+};
+
+template <typename T>
+static int batch_upload(Handle* h, int count, const rnpx_qr_desc* desc, const BatchSetup& bs, BatchBuffers* bb) {
+  RNPX_TRY(bb->desc.ensure(sizeof(rnpx_qr_desc) * (size_t)std::max(count, 1), h));
+  RNPX_TRY(bb->order.ensure(sizeof(int) * (size_t)std::max(count, 1), h));
+  RNPX_TRY(bb->counters.ensure(64, h));
+  std::vector<int> order(bs.large);
+  order.insert(order.end(), bs.small.begin(), bs.small.end());
+  RNPX_TRY(hipMemcpyAsync(bb->desc.p, desc, sizeof(rnpx_qr_desc) * count, hipMemcpyHostToDevice, h->stream));
+  RNPX_TRY(hipMemcpyAsync(bb->order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, h->stream));
+  // the host vectors die with this call: make the copies complete first
+  RNPX_TRY(hipStreamSynchronize(h->stream));
+  return RNPX_OK;
+}
+
+// Enqueue both tiers: the in-place tier on the panel stream, the LDS tier on
+// the main stream, joined back into the main stream.
+template <typename T>
+static int batch_launch(Handle* h, const BatchSetup& bs, BatchBuffers* bb, T* A, T* tau, T* B) {
+  const int* order = (const int*)bb->order.p;
+  const rnpx_qr_desc* d = (const rnpx_qr_desc*)bb->desc.p;
+  unsigned* ctr = (unsigned*)bb->counters.p;
+  const int nl = (int)bs.large.size(), ns = (int)bs.small.size();
+  int s = h->ensure_pipeline();
+  if (s) return s;
+  if (nl > 0) {
+    RNPX_TRY(hipEventRecord(h->ev_start, h->stream));
+    RNPX_TRY(hipStreamWaitEvent(h->pstream, h->ev_start, 0));
+    RNPX_TRY(launch_batched<T>(1, nl, order, d, A, tau, B, ctr, std::min(nl, h->num_cus), 0, h->pstream));
+  }
+  if (ns > 0) {
+    const int teams = std::min(ns, 4 * h->num_cus);
+    RNPX_TRY(launch_batched<T>(0, ns, order + nl, d, A, tau, B, ctr + 1, teams, kBatchLdsBytes, h->stream));
+  }
+  if (nl > 0) {
+    RNPX_TRY(hipEventRecord(h->ev_next, h->pstream));
+    RNPX_TRY(hipStreamWaitEvent(h->stream, h->ev_next, 0));
+  }
+  return RNPX_OK;
+}
+
+template <typename T>
+int geqrf_batched(Handle* h, int count, const rnpx_qr_desc* desc, T* A, T* tau, T* B) {
+  if (count < 0) return -2;
+  if (count == 0) return RNPX_OK;
+  if (!desc) return -3;
+  if (!A) return -4;
+  if (!tau) return -5;
+  BatchSetup bs;
+  int s = batch_setup<T>(count, desc, &bs);
+  if (s) return s;
+  // one set of descriptor buffers per handle
+  BatchBuffers* bb = reinterpret_cast<BatchBuffers*>(h->batch);
+  if (!bb) {
+    bb = new BatchBuffers();
+    h->batch = bb;
+    h->batch_free = [](void* p) {
+      BatchBuffers* b = reinterpret_cast<BatchBuffers*>(p);
+      b->desc.release();
+      b->order.release();
+      b->counters.release();
+      delete b;
+    };
+  }
+  if ((s = batch_upload<T>(h, count, desc, bs, bb))) return s;
+  return batch_launch<T>(h, bs, bb, A, tau, B);
+}
+
+struct BatchedPlan {
+  Handle* h = nullptr;
+  BatchBuffers bb;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+template <typename T>
+int geqrf_batched_plan(Handle* h, int count, const rnpx_qr_desc* desc, T* A, T* tau, T* B, BatchedPlan** out) {
+  if (!out) return -7;
+  if (count <= 0) return -2;
+  if (!desc) return -3;
+  BatchSetup bs;
+  int s = batch_setup<T>(count, desc, &bs);
+  if (s) return s;
+  BatchedPlan* p = new BatchedPlan();
+  p->h = h;
+  if ((s = batch_upload<T>(h, count, desc, bs, &p->bb)) || (s = h->ensure_pipeline())) {
+    delete p;
+    return s;
+  }
+  hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    s = batch_launch<T>(h, bs, &p->bb, A, tau, B);
+    hipError_t e2 = hipStreamEndCapture(h->stream, &p->graph);
+    if (!s && e2 == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    else e = (e2 != hipSuccess) ? e2 : hipErrorUnknown;
+  }
+  if (e != hipSuccess || s) {
+    if (p->graph) hipGraphDestroy(p->graph);
+    p->bb.desc.release();
+    p->bb.order.release();
+    p->bb.counters.release();
+    delete p;
+    return s ? s : hip_status(e);
+  }
+  *out = p;
+  return RNPX_OK;
+}
+
+int batched_plan_run(BatchedPlan* p) {
+  if (!p) return -1;
+  return hip_status(hipGraphLaunch(p->exec, p->h->stream));
+}
+
+int batched_plan_destroy(BatchedPlan* p) {
+  if (!p) return -1;
+  hipStreamSynchronize(p->h->stream);
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  p->bb.desc.release();
+  p->bb.order.release();
+  p->bb.counters.release();
+  delete p;
+  return RNPX_OK;
+}
+
+template int geqrf_batched<double>(Handle*, int, const rnpx_qr_desc*, double*, double*, double*);
+template int geqrf_batched<zd>(Handle*, int, const rnpx_qr_desc*, zd*, zd*, zd*);
+template int geqrf_batched_plan<double>(Handle*, int, const rnpx_qr_desc*, double*, double*, double*, BatchedPlan**);
+template int geqrf_batched_plan<zd>(Handle*, int, const rnpx_qr_desc*, zd*, zd*, zd*, BatchedPlan**);
+
+}  // namespace rnpx

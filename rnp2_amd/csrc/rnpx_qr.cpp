@@ -85,6 +85,7 @@ int Handle::ensure_pipeline() {
 
 Handle::~Handle() {
   if (stream) hipStreamSynchronize(stream);
+  if (batch && batch_free) batch_free(batch);
   for (auto& r : recs) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
