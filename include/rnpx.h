@@ -161,6 +161,22 @@ int rnpx_dgels_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, double *a, si
 int rnpx_zgels_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, rnpx_zcomplex *a, size_t lda,
                    rnpx_zcomplex *tau, rnpx_zcomplex *b, size_t ldb);
 
+/* Mixed-precision least squares (no counterpart in the reference): QR with
+ * the trailing-update products on FP32 MFMA (reflector generation, T,
+ * residuals, triangular solves and the update of x in FP64), refined by
+ * corrected semi-normal equations in FP64 until the correction reaches
+ * working precision (||d|| <= 2^-50 ||x||), stops halving, or max_iter
+ * corrections.  berr receives the normwise backward error estimate
+ * eta = ||A^T r|| / (||A||_F (||A||_F ||x|| + ||b||)) of the returned x;
+ * RNPX_ERR_NOTCONV when eta > tol.  A and B are left intact; X is
+ * n-by-nrhs.  Real only. */
+int rnpx_dgels_mixed_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, const double *a,
+                         size_t lda, const double *b, size_t ldb, double *x, size_t ldx,
+                         int max_iter, double tol, int *iters, double *berr);
+/* eta (as above) of a solution x of min ||A x - b||, worst over the nrhs. */
+int rnpx_dls_berr_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, const double *a, size_t lda,
+                      const double *b, size_t ldb, const double *x, size_t ldx, double *berr);
+
 /* Building blocks, device pointers (the reference's Reflector functions):
  * T factor of k <= nb forward columnwise reflectors stored in a (n-by-k,
  * unit diagonal implicit) -- Reflector::GenerateBlockTr("F","C"),

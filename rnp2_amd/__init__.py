@@ -75,6 +75,9 @@ SIGNATURES = {
     "rnpx_zungqr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p]),
     "rnpx_dgels_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
     "rnpx_zgels_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
+    "rnpx_dgels_mixed_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p, _sz, _p, _sz, _i, ctypes.c_double,
+                                  ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
+    "rnpx_dls_berr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p, _sz, _p, _sz, ctypes.POINTER(ctypes.c_double)]),
     "rnpx_dgeqrf_batched": (_i, [_p, _i, _p, _p, _p, _p]),
     "rnpx_zgeqrf_batched": (_i, [_p, _i, _p, _p, _p, _p]),
     "rnpx_dgeqrf_batched_plan": (_i, [_p, _i, _p, _p, _p, _p, ctypes.POINTER(_p)]),
@@ -348,6 +351,25 @@ class Handle:
         nrhs = b.shape[1] if b.dim() == 2 else 1
         fn = "rnpx_dgels_dev" if _tpre(a) == "d" else "rnpx_zgels_dev"
         _check(getattr(lib(), fn)(self.h, m, n, nrhs, _dptr(a), _ld(a), _dptr(tau), _dptr(b), _ld(b)), "gels_dev")
+
+    def gels_mixed(self, a, b, x, max_iter=30, tol=1e-14, allow_notconv=False):
+        """Mixed-precision LS with FP64 refinement; returns (status, iters, berr)."""
+        m, n = a.shape
+        nrhs = b.shape[1] if b.dim() == 2 else 1
+        it, be = ctypes.c_int(), ctypes.c_double()
+        st = lib().rnpx_dgels_mixed_dev(self.h, m, n, nrhs, _dptr(a), _ld(a), _dptr(b), _ld(b), _dptr(x), _ld(x),
+                                        max_iter, tol, ctypes.byref(it), ctypes.byref(be))
+        if st != RNPX_OK and not (allow_notconv and st == RNPX_ERR_NOTCONV):
+            raise RnpxError(st, "gels_mixed_dev")
+        return st, it.value, be.value
+
+    def ls_berr(self, a, b, x):
+        m, n = a.shape
+        nrhs = b.shape[1] if b.dim() == 2 else 1
+        be = ctypes.c_double()
+        _check(lib().rnpx_dls_berr_dev(self.h, m, n, nrhs, _dptr(a), _ld(a), _dptr(b), _ld(b), _dptr(x), _ld(x),
+                                       ctypes.byref(be)), "ls_berr_dev")
+        return be.value
 
     def geqrf_batched(self, desc, A, tau, B=None):
         """Batched QR (and LS where desc['nrhs'] > 0) over packed device buffers."""

@@ -316,6 +316,20 @@ int rnpx_zgels_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, rnpx_zcomplex
   return gels_dev<zd>(H(h), (int)m, (int)n, (int)nrhs, reinterpret_cast<zd*>(a), (long)lda,
                       reinterpret_cast<zd*>(tau), reinterpret_cast<zd*>(b), (long)ldb);
 }
+int rnpx_dgels_mixed_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, const double* a, size_t lda, const double* b,
+                         size_t ldb, double* x, size_t ldx, int max_iter, double tol, int* iters, double* berr) {
+  RNPX_DEV_CHECK(h, m, n);
+  if (!fits_int(nrhs)) return RNPX_ERR_UNSUPPORTED;
+  if (!iters || !berr) return -13;
+  return gels_mixed_dev(H(h), (int)m, (int)n, (int)nrhs, a, (long)lda, b, (long)ldb, x, (long)ldx, max_iter, tol,
+                        iters, berr);
+}
+int rnpx_dls_berr_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, const double* a, size_t lda, const double* b,
+                      size_t ldb, const double* x, size_t ldx, double* berr) {
+  RNPX_DEV_CHECK(h, m, n);
+  if (!berr) return -11;
+  return ls_berr_dev(H(h), (int)m, (int)n, (int)nrhs, a, (long)lda, b, (long)ldb, x, (long)ldx, berr);
+}
 int rnpx_dgeqrf_batched(rnpx_handle h, int count, const rnpx_qr_desc* desc, double* A, double* tau, double* B) {
   if (!h) return -1;
   return geqrf_batched<double>(H(h), count, desc, A, tau, B);

@@ -42,8 +42,10 @@ struct Handle {
   hipStream_t pstream = nullptr;  // panel stream of the look-ahead pipeline
   hipEvent_t ev_pan[2] = {nullptr, nullptr}, ev_next = nullptr, ev_start = nullptr;
   Buffer stage1, stage2, stage3;  // host-argument staging for the drop-in calls
+  Buffer lsbuf, lsnrm;            // least-squares refinement scratch
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
+  int update_prec = 0;                 // 1: trailing-update products on FP32 MFMA
   bool prof = false;
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> event_pool;
@@ -66,6 +68,10 @@ void prof_reset(Handle* h);
 
 template <typename T> int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau);
 template <typename T> int gels_dev(Handle* h, int m, int n, int nrhs, T* A, long lda, T* tau, T* B, long ldb);
+int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb, double* X,
+                   long ldx, int max_iter, double tol, int* iters, double* berr);
+int ls_berr_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb,
+                const double* X, long ldx, double* berr);
 template <typename T> int geqrf_batched(Handle* h, int count, const rnpx_qr_desc* desc, T* A, T* tau, T* B);
 struct BatchedPlan;
 template <typename T>

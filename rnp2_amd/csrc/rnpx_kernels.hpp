@@ -85,6 +85,18 @@ hipError_t launch_larnv(int dist, const int iseed[4], long long count, T* x, hip
 template <typename T>
 hipError_t launch_trsm_un(int n, int nrhs, const T* R, long ldr, T* B, long ldb, hipStream_t st);
 
+// Mixed precision: FP64 operands, FP32 MFMA product (FP32 accumulation),
+// FP64 epilogue; split-K slices reduced in FP64.
+hipError_t gemm32(char ta, char tb, int M, int N, int K, double alpha, const double* A, long lda, const double* B,
+                  long ldb, double beta, double* C, long ldc, int ksplit, double* part, hipStream_t st);
+// R^H Y = S (R upper), S's first n rows in place.
+template <typename T>
+hipError_t launch_trsm_uc(int n, int nrhs, const T* R, long ldr, T* B, long ldb, hipStream_t st);
+// out[j] = ||X(:,j)||_2^2
+hipError_t launch_colnorm2(int m, int n, const double* X, long ldx, double* out, hipStream_t st);
+// Y += X
+hipError_t launch_axpy_mat(int m, int n, const double* X, long ldx, double* Y, long ldy, hipStream_t st);
+
 // Batched QR / LS: tier 0 = one wave per problem in LDS (lds_bytes of
 // dynamic LDS per team), tier 1 = 256 threads per problem in place.
 template <typename T>

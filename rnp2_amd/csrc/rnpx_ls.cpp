@@ -41,6 +41,138 @@ int gels_dev(Handle* h, int m, int n, int nrhs, T* A, long lda, T* tau, T* B, lo
 }
 
 template int gels_dev<double>(Handle*, int, int, int, double*, long, double*, double*, long);
+
+// Normwise backward error of a least-squares solution, per right-hand side,
+// worst over the columns:
+//   eta = ||A^T r|| / (||A||_F (||A||_F ||x|| + ||b||)),  r = b - A x,
+// all in FP64 on the device (scratch: the handle's lsbuf/lsnrm buffers).
+struct LsScratch {
+  double *R, *S, *nrm;  // m x nrhs residual, n x nrhs normal residual, norms
+};
+
+static int ls_scratch(Handle* h, int m, int n, int nrhs, LsScratch* sc) {
+  RNPX_TRY(h->lsbuf.ensure(sizeof(double) * ((size_t)m * nrhs + (size_t)n * nrhs), h));
+  RNPX_TRY(h->lsnrm.ensure(sizeof(double) * (size_t)(4 * nrhs + n + 64), h));
+  sc->R = (double*)h->lsbuf.p;
+  sc->S = sc->R + (size_t)m * nrhs;
+  sc->nrm = (double*)h->lsnrm.p;
+  return RNPX_OK;
+}
+
+// r = b - A x; s = A^T r; host gets per-column ||s||^2, ||x||^2, ||b||^2 and
+// the total ||A||_F^2 (when anorm2 < 0 it is computed).
+static int ls_residual(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb,
+                       const double* X, long ldx, const LsScratch& sc, double* anorm2, double* eta) {
+  hipStream_t st = h->stream;
+  RNPX_TRY(launch_copy<double>(m, nrhs, B, ldb, sc.R, m, st));
+  RNPX_TRY(gemm<double>('N', 'N', m, nrhs, n, -1.0, A, lda, X, ldx, 1.0, sc.R, m, 1, nullptr, st));
+  const int ks = std::max(1, std::min(32, m / 512));
+  RNPX_TRY(h->part.ensure(sizeof(double) * (size_t)ks * n * nrhs, h));
+  RNPX_TRY(gemm<double>('T', 'N', n, nrhs, m, 1.0, A, lda, sc.R, m, 0.0, sc.S, n, ks, (double*)h->part.p, st));
+  double* nr = sc.nrm;
+  RNPX_TRY(launch_colnorm2(n, nrhs, sc.S, n, nr, st));
+  RNPX_TRY(launch_colnorm2(n, nrhs, X, ldx, nr + nrhs, st));
+  RNPX_TRY(launch_colnorm2(m, nrhs, B, ldb, nr + 2 * nrhs, st));
+  const bool need_a = (*anorm2 < 0);
+  if (need_a) RNPX_TRY(launch_colnorm2(m, n, A, lda, nr + 3 * nrhs, st));
+  std::vector<double> hn(3 * nrhs + (need_a ? n : 0));
+  RNPX_TRY(hipMemcpyAsync(hn.data(), nr, sizeof(double) * hn.size(), hipMemcpyDeviceToHost, st));
+  RNPX_TRY(hipStreamSynchronize(st));
+  if (need_a) {
+    double a2 = 0.0;
+    for (int j = 0; j < n; ++j) a2 += hn[3 * nrhs + j];
+    *anorm2 = a2;
+  }
+  const double an = sqrt(*anorm2);
+  double worst = 0.0;
+  for (int q = 0; q < nrhs; ++q) {
+    const double den = an * (an * sqrt(hn[nrhs + q]) + sqrt(hn[2 * nrhs + q]));
+    const double e = den > 0 ? sqrt(hn[q]) / den : sqrt(hn[q]);
+    worst = std::max(worst, e);
+  }
+  *eta = worst;
+  return RNPX_OK;
+}
+
+int ls_berr_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb,
+                const double* X, long ldx, double* berr) {
+  if (m < 0 || n < 0 || n > m || nrhs < 0) return -2;
+  if (nrhs == 0 || n == 0) {
+    *berr = 0.0;
+    return RNPX_OK;
+  }
+  LsScratch sc;
+  int s = ls_scratch(h, m, n, nrhs, &sc);
+  if (s) return s;
+  double a2 = -1.0;
+  return ls_residual(h, m, n, nrhs, A, lda, B, ldb, X, ldx, sc, &a2, berr);
+}
+
+// Mixed-precision least squares: QR with the trailing-update products on
+// FP32 MFMA (panel: norms, beta, tau in FP64; T factor in FP64), then
+// refinement by corrected semi-normal equations in FP64:
+//   r = b - A x,  s = A^T r,  R^T R d = s,  x += d
+// until the correction is at working precision (||d|| <= 2^-50 ||x|| in
+// every column), or ||d|| stops halving, or max_iter corrections; then eta
+// of the final x is checked against tol.  A and B are not modified; X
+// (n x nrhs) receives the solution.
+int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb, double* X,
+                   long ldx, int max_iter, double tol, int* iters, double* berr) {
+  if (m < 0) return -2;
+  if (n < 0 || n > m) return -3;
+  if (nrhs < 0) return -4;
+  if (lda < std::max(1, m)) return -6;
+  if (ldb < std::max(1, m)) return -8;
+  if (ldx < std::max(1, n)) return -10;
+  *iters = 0;
+  *berr = 0.0;
+  if (n == 0 || nrhs == 0) return RNPX_OK;
+  hipStream_t st = h->stream;
+  RNPX_TRY(h->vfull.ensure(sizeof(double) * ((size_t)m * n + n + (size_t)m * nrhs), h));
+  double* Af = (double*)h->vfull.p;
+  double* tau = Af + (size_t)m * n;
+  double* Bq = tau + n;
+  RNPX_TRY(launch_copy<double>(m, n, A, lda, Af, m, st));
+  h->update_prec = 1;
+  int s = geqrf_dev<double>(h, m, n, Af, m, tau);
+  h->update_prec = 0;
+  if (s) return s;
+  // x0 = R^{-1} (Q^T b)(0:n)
+  RNPX_TRY(launch_copy<double>(m, nrhs, B, ldb, Bq, m, st));
+  if ((s = ormqr_dev<double>(h, 'L', 'C', m, nrhs, n, Af, m, tau, Bq, m))) return s;
+  RNPX_TRY(launch_trsm_un<double>(n, nrhs, Af, m, Bq, m, st));
+  RNPX_TRY(launch_copy<double>(n, nrhs, Bq, m, X, ldx, st));
+  LsScratch sc;
+  if ((s = ls_scratch(h, m, n, nrhs, &sc))) return s;
+  double a2 = -1.0, eta = 0.0, prev = 0.0;
+  std::vector<double> dn(2 * nrhs);
+  int it = 0;
+  for (;; ++it) {
+    if ((s = ls_residual(h, m, n, nrhs, A, lda, B, ldb, X, ldx, sc, &a2, &eta))) return s;
+    if (it >= max_iter) break;
+    // d = R^{-1} R^{-T} s  (s is in sc.S, n x nrhs), x += d
+    RNPX_TRY(launch_trsm_uc<double>(n, nrhs, Af, m, sc.S, n, st));
+    RNPX_TRY(launch_trsm_un<double>(n, nrhs, Af, m, sc.S, n, st));
+    RNPX_TRY(launch_axpy_mat(n, nrhs, sc.S, n, X, ldx, st));
+    RNPX_TRY(launch_colnorm2(n, nrhs, sc.S, n, sc.nrm, st));
+    RNPX_TRY(launch_colnorm2(n, nrhs, X, ldx, sc.nrm + nrhs, st));
+    RNPX_TRY(hipMemcpyAsync(dn.data(), sc.nrm, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, st));
+    RNPX_TRY(hipStreamSynchronize(st));
+    double rel = 0.0;
+    for (int q = 0; q < nrhs; ++q) rel = std::max(rel, dn[nrhs + q] > 0 ? sqrt(dn[q] / dn[nrhs + q]) : sqrt(dn[q]));
+    const bool small = rel <= 0x1p-50;
+    const bool stalled = it > 0 && !(rel < 0.5 * prev);
+    prev = rel;
+    if (small || stalled) {
+      ++it;
+      if ((s = ls_residual(h, m, n, nrhs, A, lda, B, ldb, X, ldx, sc, &a2, &eta))) return s;
+      break;
+    }
+  }
+  *iters = it;
+  *berr = eta;
+  return (eta <= tol) ? RNPX_OK : RNPX_ERR_NOTCONV;
+}
 template int gels_dev<zd>(Handle*, int, int, int, zd*, long, zd*, zd*, long);
 
 // ---------------------------------------------------------------------------

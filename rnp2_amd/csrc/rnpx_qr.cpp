@@ -43,7 +43,7 @@ void Buffer::release() {
 size_t Handle::held_bytes() const {
   size_t s = 0;
   for (const Buffer* b : {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc, &stage1, &stage2,
-                          &stage3, &vbuf2, &gram2, &tbuf2})
+                          &stage3, &vbuf2, &gram2, &tbuf2, &lsbuf, &lsnrm})
     s += b->bytes;
   return s;
 }
@@ -96,7 +96,7 @@ Handle::~Handle() {
     if (ev) hipEventDestroy(ev);
   if (pstream) hipStreamDestroy(pstream);
   for (Buffer* b : {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc, &stage1, &stage2,
-                          &stage3, &vbuf2, &gram2, &tbuf2})
+                          &stage3, &vbuf2, &gram2, &tbuf2, &lsbuf, &lsnrm})
     b->release();
   if (own_stream && own) hipStreamDestroy(own);
 }
@@ -224,12 +224,21 @@ static int apply_block_dev(Handle* h, char side, char trans, int mc, int nc, int
     const double fl = (S<T>::cplx ? 8.0 : 2.0) * (double)mc * nc * k;
     const double esz = sizeof(T);
     hipEvent_t ev;
+    // Mixed precision (real only): the two big products on FP32 MFMA.
+    bool f32 = false;
+    if constexpr (!S<T>::cplx) f32 = (h->update_prec == 1);
     h->prof_begin(&ev);
-    RNPX_TRY(gemm<T>('C', 'N', k, nc, mc, one, V, ldv, C, ldc, zero, W, k, ks, (T*)h->part.p, st));
+    if constexpr (!S<T>::cplx) {
+      if (f32) RNPX_TRY(gemm32('C', 'N', k, nc, mc, 1.0, V, ldv, C, ldc, 0.0, W, k, ks, (T*)h->part.p, st));
+    }
+    if (!f32) RNPX_TRY(gemm<T>('C', 'N', k, nc, mc, one, V, ldv, C, ldc, zero, W, k, ks, (T*)h->part.p, st));
     h->prof_end(K_WGEMM, ev, fl, esz * ((double)mc * nc + (double)mc * k + (double)k * nc));
     RNPX_TRY(gemm<T>(trans == 'N' ? 'N' : 'C', 'N', k, nc, k, one, Tm, ldt, W, k, zero, Z, k, 1, nullptr, st));
     h->prof_begin(&ev);
-    RNPX_TRY(gemm<T>('N', 'N', mc, nc, k, mone, V, ldv, Z, k, one, C, ldc, 1, nullptr, st));
+    if constexpr (!S<T>::cplx) {
+      if (f32) RNPX_TRY(gemm32('N', 'N', mc, nc, k, -1.0, V, ldv, Z, k, 1.0, C, ldc, 1, nullptr, st));
+    }
+    if (!f32) RNPX_TRY(gemm<T>('N', 'N', mc, nc, k, mone, V, ldv, Z, k, one, C, ldc, 1, nullptr, st));
     h->prof_end(K_UPDATE, ev, fl, esz * (2.0 * mc * nc + (double)mc * k + (double)k * nc));
   } else {
     // W = C V (mc x k), Z = W op(T), C -= Z V^H
