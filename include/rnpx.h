@@ -177,6 +177,46 @@ int rnpx_dgels_mixed_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, const d
 int rnpx_dls_berr_dev(rnpx_handle h, size_t m, size_t n, size_t nrhs, const double *a, size_t lda,
                       const double *b, size_t ldb, const double *x, size_t ldx, double *berr);
 
+/* ---------------------------------------------------------------------
+ * Least-squares request server (no counterpart in the reference).
+ * Requests are submitted from the host and solved on the device by worker
+ * threads (one HIP stream each): small requests (m <= 256) are batched into
+ * one persistent launch, larger ones run concurrently; admission is gated by
+ * the HBM budget.  mode 0: FP64; mode 1: mixed precision with FP64
+ * refinement and FP64 fallback when refinement does not reach tol.
+ * ------------------------------------------------------------------- */
+typedef struct rnpx_server_s *rnpx_server;
+typedef struct rnpx_server_config {
+	size_t hbm_budget;  /* bytes for in-flight requests (0 = unbounded) */
+	int workers;        /* concurrent requests / batches on the device, 1..16 */
+	int mode;           /* 0 FP64, 1 mixed + refinement (FP64 fallback) */
+	int max_iter;       /* refinement corrections (0 = 30) */
+	double tol;         /* backward-error target of the mixed path (0 = 1e-13) */
+} rnpx_server_config;
+typedef struct rnpx_request_info {
+	int status;   /* RNPX_OK or an error code */
+	int path;     /* 0 FP64 dense, 1 mixed, 2 mixed fell back to FP64, 3 batched */
+	int iters;    /* refinement corrections (mixed) */
+	int worker;   /* worker that ran it */
+	double berr;  /* eta of the solution (dense paths) */
+	double t_submit, t_admit, t_done;  /* steady-clock seconds */
+} rnpx_request_info;
+typedef struct rnpx_server_stats {
+	long long submitted, admitted, completed, rejected, failed, batches, queued;
+	size_t bytes_in_use;
+} rnpx_server_stats;
+int rnpx_server_create(const rnpx_server_config *cfg, rnpx_server *server);
+/* Copies A (m-by-n, lda) and B (m-by-nrhs, ldb); m >= n. */
+int rnpx_server_submit(rnpx_server s, size_t m, size_t n, size_t nrhs, const double *a, size_t lda,
+                       const double *b, size_t ldb, long long *id);
+/* 0: done (X n-by-nrhs written when status is OK), 1: pending (wait == 0). */
+int rnpx_server_poll(rnpx_server s, long long id, int wait, double *x, size_t ldx,
+                     rnpx_request_info *info);
+int rnpx_server_release(rnpx_server s, long long id);
+int rnpx_server_stats_get(rnpx_server s, rnpx_server_stats *stats);
+/* Waits for every submitted request, then stops the workers. */
+int rnpx_server_destroy(rnpx_server s);
+
 /* Building blocks, device pointers (the reference's Reflector functions):
  * T factor of k <= nb forward columnwise reflectors stored in a (n-by-k,
  * unit diagonal implicit) -- Reflector::GenerateBlockTr("F","C"),

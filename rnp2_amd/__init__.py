@@ -48,6 +48,21 @@ class Tuning(ctypes.Structure):
     _fields_ = [("nb", _i), ("panel_rpt", _i), ("ksplit", _i), ("lookahead", _i)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", _sz), ("workers", _i), ("mode", _i), ("max_iter", _i), ("tol", ctypes.c_double)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", _i), ("path", _i), ("iters", _i), ("worker", _i), ("berr", ctypes.c_double),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("admitted", ctypes.c_longlong), ("completed", ctypes.c_longlong),
+                ("rejected", ctypes.c_longlong), ("failed", ctypes.c_longlong), ("batches", ctypes.c_longlong),
+                ("queued", ctypes.c_longlong), ("bytes_in_use", _sz)]
+
+
 # name -> (restype, argtypes); kept in the order of include/rnpx.h
 SIGNATURES = {
     "rnpx_dgeqrf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
@@ -93,6 +108,12 @@ SIGNATURES = {
     "rnpx_zgemm_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, None, _p, _sz, _p, _sz, None, _p, _sz]),
     "rnpx_dlarnv_dev": (_i, [_p, _i, ctypes.POINTER(_i), _sz, _p]),
     "rnpx_zlarnv_dev": (_i, [_p, _i, ctypes.POINTER(_i), _sz, _p]),
+    "rnpx_server_create": (_i, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_p)]),
+    "rnpx_server_submit": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p, _sz, ctypes.POINTER(ctypes.c_longlong)]),
+    "rnpx_server_poll": (_i, [_p, ctypes.c_longlong, _i, _p, _sz, ctypes.POINTER(RequestInfo)]),
+    "rnpx_server_release": (_i, [_p, ctypes.c_longlong]),
+    "rnpx_server_stats_get": (_i, [_p, ctypes.POINTER(ServerStats)]),
+    "rnpx_server_destroy": (_i, [_p]),
     "rnpx_version": (_s, []),
     "rnpx_device_name": (_i, [ctypes.c_char_p, _sz]),
 }
@@ -428,6 +449,60 @@ class BatchedPlan:
     def close(self):
         if self.p:
             lib().rnpx_batched_plan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+PATHS = {0: "fp64", 1: "mixed", 2: "mixed->fp64", 3: "batched"}
+
+
+class Server:
+    """Least-squares request server (rnpx_server_*): submit host arrays, poll
+    for X.  mode 0 = FP64, 1 = mixed precision with FP64 fallback."""
+
+    def __init__(self, hbm_budget=0, workers=4, mode=1, max_iter=30, tol=1e-13):
+        cfg = ServerConfig(hbm_budget, workers, mode, max_iter, tol)
+        p = ctypes.c_void_p()
+        _check(lib().rnpx_server_create(ctypes.byref(cfg), ctypes.byref(p)), "rnpx_server_create")
+        self.p = p
+        self.shapes = {}
+
+    def submit(self, a, b):
+        a = np.asfortranarray(a, dtype=np.float64)
+        b2 = np.asfortranarray(b.reshape(b.shape[0], -1), dtype=np.float64)
+        m, n = a.shape
+        rid = ctypes.c_longlong()
+        st = lib().rnpx_server_submit(self.p, m, n, b2.shape[1], _ptr(a), m, _ptr(b2), m, ctypes.byref(rid))
+        self.shapes[rid.value] = (n, b2.shape[1])
+        return st, rid.value
+
+    def poll(self, rid, wait=True):
+        n, q = self.shapes[rid]
+        x = np.zeros((n, q), dtype=np.float64, order="F")
+        info = RequestInfo()
+        st = lib().rnpx_server_poll(self.p, rid, 1 if wait else 0, _ptr(x), n, ctypes.byref(info))
+        if st == 1:
+            return None, None
+        _check(st, "rnpx_server_poll")
+        return x, info
+
+    def release(self, rid):
+        lib().rnpx_server_release(self.p, rid)
+        self.shapes.pop(rid, None)
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().rnpx_server_stats_get(self.p, ctypes.byref(s)), "rnpx_server_stats_get")
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.p:
+            lib().rnpx_server_destroy(self.p)
             self.p = None
 
     def __del__(self):

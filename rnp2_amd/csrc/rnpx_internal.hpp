@@ -40,6 +40,7 @@ struct Handle {
   Buffer vbuf, vfull, gram, tbuf, wbuf, zbuf, part, ppart, rowbuf, sync, flag, misc;
   Buffer vbuf2, gram2, tbuf2;    // second set for the look-ahead pipeline
   hipStream_t pstream = nullptr;  // panel stream of the look-ahead pipeline
+  hipStream_t ext_pstream = nullptr;  // shared panel stream supplied by the owner (not destroyed)
   hipEvent_t ev_pan[2] = {nullptr, nullptr}, ev_next = nullptr, ev_start = nullptr;
   Buffer stage1, stage2, stage3;  // host-argument staging for the drop-in calls
   Buffer lsbuf, lsnrm;            // least-squares refinement scratch

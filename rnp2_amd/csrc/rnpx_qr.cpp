@@ -74,8 +74,13 @@ void Handle::prof_end(int kid, hipEvent_t a, double flops, double bytes, hipStre
 }
 int Handle::ensure_pipeline() {
   if (pstream) return RNPX_OK;
-  hipError_t e = hipStreamCreateWithFlags(&pstream, hipStreamNonBlocking);
-  if (e != hipSuccess) return hip_status(e);
+  hipError_t e = hipSuccess;
+  if (ext_pstream) {
+    pstream = ext_pstream;
+  } else {
+    e = hipStreamCreateWithFlags(&pstream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_status(e);
+  }
   for (hipEvent_t* ev : {&ev_pan[0], &ev_pan[1], &ev_next, &ev_start}) {
     e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
     if (e != hipSuccess) return hip_status(e);
@@ -94,7 +99,7 @@ Handle::~Handle() {
   if (pstream) hipStreamSynchronize(pstream);
   for (hipEvent_t ev : {ev_pan[0], ev_pan[1], ev_next, ev_start})
     if (ev) hipEventDestroy(ev);
-  if (pstream) hipStreamDestroy(pstream);
+  if (pstream && pstream != ext_pstream) hipStreamDestroy(pstream);
   for (Buffer* b : {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc, &stage1, &stage2,
                           &stage3, &vbuf2, &gram2, &tbuf2, &lsbuf, &lsnrm})
     b->release();
