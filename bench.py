@@ -13,6 +13,12 @@ roofline: the trailing-update GEMM (C -= V Z, FP64 MFMA), timed live with
 cpu_baseline: the oracle (a C restatement of RNP's blocked QR) run on the
           host cores over the first panels of the same 8192 x 8192 matrix.
 
+Extra legs (in "extra", same line): complex<double> 4096^2 and tall
+32768 x 4096 Factor; the batched form (10,000 least-squares problems, m
+log-uniform in [16,256], n uniform in [8,m]); mixed-precision vs plain FP64
+least squares on 16384 x 2048 with s log-spaced 1..1e-6; a Poisson stream of
+least-squares requests through the server with p50/p99 latency.
+
 Prints one JSON line.  Only one GPU: --gpus N with N > 1 exits with a message.
 """
 import argparse
@@ -61,6 +67,188 @@ def cpu_baseline(m, n, seed, budget_s=15.0):
                       f"panels ({K} columns) of the same {m}x{n} matrix: {fl / 1e9:.1f} GFLOP in {dt:.2f} s"}
 
 
+def _ev_time(fn, reps, stream_sync):
+    """Mean wall time of fn() over reps runs (each ended by a device sync)."""
+    fn()
+    stream_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    stream_sync()
+    return (time.perf_counter() - t0) / reps
+
+
+def leg_factor(R, torch, m, n, cplx, reps=3):
+    h = R.Handle()
+    dt = torch.complex128 if cplx else torch.float64
+    a0 = R.colmajor_empty(m, n, dt)
+    h.larnv(2, [3, 1, 4, 1], a0)
+    a = R.colmajor_empty(m, n, dt)
+    tau = torch.empty(min(m, n), dtype=dt, device="cuda")
+    h.synchronize()
+    stream = torch.cuda.ExternalStream(h.stream)
+
+    def run():
+        with torch.cuda.stream(stream):
+            a.copy_(a0)
+        h.geqrf(a, tau)
+
+    t = _ev_time(run, reps, h.synchronize)
+    fl = qr_flops(m, n, cplx)
+    return {"shape": [m, n], "dtype": "c128" if cplx else "f64", "ms": round(t * 1e3, 3),
+            "gflops": round(fl / t / 1e9, 1)}
+
+
+def c3_shapes(count, seed):
+    rng = np.random.default_rng(seed)
+    m = np.rint(np.exp(rng.uniform(np.log(16), np.log(256), count))).astype(int)
+    n = np.array([rng.integers(8, mm + 1) for mm in m])
+    return list(zip(m.tolist(), n.tolist()))
+
+
+def leg_batched(R, torch, cplx, count=10000, reps=5):
+    shapes = c3_shapes(count, 0)
+    desc, na, nt, nbb = R.make_batch(shapes, nrhs=1)
+    dt = torch.complex128 if cplx else torch.float64
+    h = R.Handle()
+    A0 = torch.empty(na, dtype=dt, device="cuda")
+    B0 = torch.empty(nbb, dtype=dt, device="cuda")
+    h.larnv(2, [7, 7, 7, 7], A0)
+    h.larnv(2, [9, 9, 9, 9], B0)
+    A, B = A0.clone(), B0.clone()
+    tau = torch.empty(nt, dtype=dt, device="cuda")
+    h.synchronize()
+    plan = h.batched_plan(desc, A, tau, B)
+    stream = torch.cuda.ExternalStream(h.stream)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for r in range(reps + 1):
+        with torch.cuda.stream(stream):
+            A.copy_(A0)
+            B.copy_(B0)
+            start.record(stream)
+        plan.run()
+        with torch.cuda.stream(stream):
+            end.record(stream)
+        h.synchronize()
+        if r:
+            times.append(start.elapsed_time(end) * 1e-3)
+    plan.close()
+    t = float(np.median(times))
+    fl = sum(qr_flops(m, nn, cplx) + (4 if cplx else 1) * (4.0 * m * nn - 2.0 * nn * nn + nn * nn) for m, nn in shapes)
+    by = (16 if cplx else 8) * sum(2 * m * nn + 2 * m for m, nn in shapes)
+    return {"problems": count, "dtype": "c128" if cplx else "f64", "ms": round(t * 1e3, 3),
+            "solves_per_s": round(count / t, 1), "gflops": round(fl / t / 1e9, 1),
+            "hbm_gbs": round(by / t / 1e9, 1), "fp64_frac": round(fl / t / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4)}
+
+
+def conditioned_dev(R, torch, h, m, n, cond, seed):
+    """A = U diag(s) V^T on the device: U, V the Q factors (this library's QR)
+    of Gaussian matrices, s log-spaced from 1 to 1/cond."""
+    def q_of(rows, cols, sd):
+        g = R.colmajor_empty(rows, cols, torch.float64)
+        h.larnv(2, [sd % 4096, 5, 7, 11], g)
+        tau = torch.empty(cols, dtype=torch.float64, device="cuda")
+        h.geqrf(g, tau)
+        h.orgqr(g, tau)
+        return g
+    u = q_of(m, n, seed)
+    v = q_of(n, n, seed + 1)
+    h.synchronize()
+    s = torch.logspace(0, -np.log10(cond), n, dtype=torch.float64, device="cuda")
+    a = R.colmajor_empty(m, n, torch.float64)
+    a.copy_((u * s) @ v.t())
+    torch.cuda.synchronize()
+    return a
+
+
+def leg_mixed_ls(R, torch, m=16384, n=2048, cond=1e6, reps=3):
+    h = R.Handle()
+    a = conditioned_dev(R, torch, h, m, n, cond, 21)
+    xt = torch.randn(n, 1, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    b = R.colmajor_empty(m, 1, torch.float64)
+    b.copy_(a @ xt + 1e-8 * torch.randn(m, 1, dtype=torch.float64, device="cuda",
+                                        generator=torch.Generator("cuda").manual_seed(2)))
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(h.stream)
+    af = R.colmajor_empty(m, n, torch.float64)
+    bf = R.colmajor_empty(m, 1, torch.float64)
+    tau = torch.empty(n, dtype=torch.float64, device="cuda")
+
+    def plain():
+        with torch.cuda.stream(stream):
+            af.copy_(a)
+            bf.copy_(b)
+        h.gels(af, tau, bf)
+
+    t_plain = _ev_time(plain, reps, h.synchronize)
+    x64 = bf[:n].clone()
+    be64 = h.ls_berr(a, b, x64)
+    x = R.colmajor_empty(n, 1, torch.float64)
+    info = {}
+
+    def mixed():
+        info["r"] = h.gels_mixed(a, b, x, max_iter=30, tol=1e-13, allow_notconv=True)
+
+    t_mixed = _ev_time(mixed, reps, h.synchronize)
+    st, iters, bem = info["r"]
+    fwd = float((x - x64).abs().max() / x64.abs().max())
+    return {"shape": [m, n], "cond": cond,
+            "plain_fp64": {"ms": round(t_plain * 1e3, 3), "solves_per_s": round(1.0 / t_plain, 2),
+                           "berr": be64},
+            "mixed": {"ms": round(t_mixed * 1e3, 3), "solves_per_s": round(1.0 / t_mixed, 2), "iters": iters,
+                      "berr": bem, "status": st, "rel_diff_vs_fp64": fwd},
+            "speedup": round(t_plain / t_mixed, 3)}
+
+
+def leg_serving(R, torch, count=120, rate=40.0, seed=5, mode=1):
+    """Poisson arrivals at `rate` requests/s of least-squares requests: m
+    log-uniform in [64, 8192], n uniform in [m/8, m/2], 1-16 right-hand
+    sides, condition number log-uniform in [1e2, 1e8] (s log-spaced)."""
+    rng = np.random.default_rng(seed)
+    h = R.Handle()
+    reqs = []
+    for i in range(count):
+        m = int(np.rint(np.exp(rng.uniform(np.log(64), np.log(8192)))))
+        n = int(rng.integers(max(1, m // 8), max(2, m // 2) + 1))
+        q = int(rng.integers(1, 17))
+        cond = float(np.exp(rng.uniform(np.log(1e2), np.log(1e8))))
+        a = conditioned_dev(R, torch, h, m, n, cond, 100 + 3 * i)
+        xt = torch.from_numpy(rng.standard_normal((n, q))).cuda()
+        b = a @ xt + 1e-8 * torch.from_numpy(rng.standard_normal((m, q))).cuda()
+        reqs.append((a.cpu().numpy(), b.cpu().numpy(), cond))
+    gaps = rng.exponential(1.0 / rate, count)
+    srv = R.Server(hbm_budget=64 << 30, workers=4, mode=mode, max_iter=30, tol=1e-13)
+    t0 = time.monotonic() + 0.05
+    arrivals, ids = [], []
+    t_next = t0
+    for (a, b, _), g in zip(reqs, gaps):
+        t_next += g
+        d = t_next - time.monotonic()
+        if d > 0:
+            time.sleep(d)
+        arrivals.append(t_next)
+        ids.append(srv.submit(a, b)[1])
+    lat, worst, paths = [], 0.0, {}
+    t_end = 0.0
+    for (a, b, _), arr, rid in zip(reqs, arrivals, ids):
+        x, info = srv.poll(rid)
+        lat.append(info.t_done - arr)
+        t_end = max(t_end, info.t_done)
+        paths[R.PATHS[info.path]] = paths.get(R.PATHS[info.path], 0) + 1
+        r = b - a @ x
+        an = np.linalg.norm(a)
+        e = np.max(np.linalg.norm(a.T @ r, axis=0) / (an * (an * np.linalg.norm(x, axis=0) + np.linalg.norm(b, axis=0))))
+        worst = max(worst, float(e))
+    stats = srv.stats()
+    srv.close()
+    lat = np.array(lat) * 1e3
+    return {"requests": count, "rate_per_s": rate, "mode": "mixed+fp64-fallback" if mode == 1 else "fp64",
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(float(lat.mean()), 3), "throughput_rps": round(count / (t_end - arrivals[0]), 2),
+            "max_berr": worst, "paths": paths, "rejected": stats["rejected"], "batches": stats["batches"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +261,8 @@ def main():
     ap.add_argument("--no-lookahead", action="store_true", help="serialize panels and updates")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--legs", default="all",
+                    help="comma list of extra legs: complex,tall,batched,mixed,serving; 'none' or 'all'")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -156,6 +346,26 @@ def main():
         "panel_hbm_gbs": round(pan["bytes"] / (pan["ms"] * 1e-3) / 1e9, 1) if pan["ms"] > 0 else None,
         "tau_sane": ok,
     }
+    legs = [] if args.legs == "none" else (["complex", "tall", "batched", "mixed", "serving"]
+                                           if args.legs == "all" else args.legs.split(","))
+    extra = {}
+    for leg in legs:
+        print(f"[bench] leg {leg}", file=sys.stderr, flush=True)
+        try:
+            if leg == "complex":
+                extra["complex_4096"] = leg_factor(R, torch, 4096, 4096, True)
+            elif leg == "tall":
+                extra["tall_32768x4096"] = leg_factor(R, torch, 32768, 4096, False)
+            elif leg == "batched":
+                extra["batched_10k_ls"] = {"d": leg_batched(R, torch, False), "z": leg_batched(R, torch, True)}
+            elif leg == "mixed":
+                extra["mixed_ls_16384x2048"] = leg_mixed_ls(R, torch)
+            elif leg == "serving":
+                extra["serving"] = leg_serving(R, torch)
+        except Exception as e:  # a failing leg is reported, the headline line still prints
+            extra[leg] = {"error": repr(e)}
+    if extra:
+        line["extra"] = extra
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(m, n, seed, args.cpu_budget)
     print(json.dumps(line), flush=True)
