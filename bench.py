@@ -36,6 +36,21 @@ FP64_MFMA_PEAK_TFLOPS = 78.6   # MI355X dense FP64 matrix peak (published)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E (published)
 
 
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "qr_factor_8192_kernels.md")
+
+
+def update_traffic_bytes():
+    """HBM bytes per trailing-update launch from the committed FETCH_SIZE /
+    WRITE_SIZE counter summary (8192^2 workload), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def qr_flops(m, n, cplx=False):
     k = min(m, n)
     f = 4.0 * m * n * k - 2.0 * (m + n) * k * k + 4.0 * k ** 3 / 3.0
@@ -334,7 +349,10 @@ def main():
                                f"(RNP Normal01 stream, iseed {seed}), resident in HBM",
                    "m": m, "n": n, "nb": args.nb or 64, "flops_per_step": fl},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
+                     "traffic": update_traffic_bytes() if (m, n) == (8192, 8192) else None,
+                     "algorithmic_bytes_per_launch": round(upd["bytes"] / max(1, upd["launches"])),
+                     "traffic_source": "profiles/qr_factor_8192_kernels.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE)",
                      "kernel": "gemm_kernel<double,'N','N'> (trailing update C -= V Z)",
                      "launches": upd["launches"], "kernel_ms_per_step": round(upd["ms"] / args.steps, 3)},
         "breakdown_ms_per_step": {

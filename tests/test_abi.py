@@ -1,0 +1,204 @@
+"""The C ABI and the host-side surface, on the CPU (no kernel is launched).
+
+librnpx.so must load here, export every function include/rnpx.h declares,
+answer RNP's workspace queries exactly as the reference does
+(RNP/LA/QR.hpp:141-144, 355-358, 519-522), reject bad arguments LAPACK
+style, and report RNPX_ERR_NOGPU (never a CPU fallback) when no GPU is
+present.  The C++ surface include/rnpx_qr.hpp is compiled and run the same
+way.
+"""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import rnp2_amd as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rnpx.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rnpx_\w+)\s*\(", text)))
+
+
+def has_gpu():
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 40
+    L = ctypes.CDLL(R.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, f"declared in include/rnpx.h but not exported: {missing}"
+    out = subprocess.run(["nm", "-D", "--defined-only", R.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\b(rnpx_\w+)$", out, flags=re.M))
+    assert set(names) <= exported
+
+
+def test_python_signatures_cover_header():
+    names = set(declared_functions())
+    assert names <= set(R.SIGNATURES), sorted(names - set(R.SIGNATURES))
+
+
+def test_version_string():
+    assert R.version().startswith("rnpx")
+
+
+def test_workspace_queries_match_reference():
+    L = R.lib()
+    sz = ctypes.c_size_t
+    for m, n in ((512, 256), (8192, 8192), (100, 300)):
+        lw = sz(0)
+        assert L.rnpx_dgeqrf(m, n, None, m, None, ctypes.byref(lw), None) == 0
+        assert lw.value == 32 * n
+        lw = sz(0)
+        assert L.rnpx_zgeqrf(m, n, None, m, None, ctypes.byref(lw), None) == 0
+        assert lw.value == 32 * n
+    lw = sz(0)
+    assert L.rnpx_dormqr(b"L", b"C", 100, 40, 30, None, 100, None, None, 100, ctypes.byref(lw), None) == 0
+    assert lw.value == 32 * 40 + 32 * 32
+    lw = sz(0)
+    assert L.rnpx_zunmqr(b"R", b"N", 40, 100, 30, None, 100, None, None, 40, ctypes.byref(lw), None) == 0
+    assert lw.value == 32 * 40 + 32 * 32
+    lw = sz(0)
+    assert L.rnpx_dorgqr(100, 40, 30, None, 100, None, ctypes.byref(lw), None) == 0
+    assert lw.value == 40 * 32
+    # zero sizes return before the query, as in the reference
+    lw = sz(7)
+    assert L.rnpx_dgeqrf(0, 5, None, 1, None, ctypes.byref(lw), None) == 0 and lw.value == 7
+
+
+def test_argument_errors_lapack_style():
+    L = R.lib()
+    sz = ctypes.c_size_t
+    lw = sz(0)
+    assert L.rnpx_dgeqrf(10, 5, None, 9, None, ctypes.byref(lw), None) == -4       # lda < m
+    lw = sz(5)
+    a = np.zeros(50)
+    assert L.rnpx_dgeqrf(10, 5, a.ctypes.data_as(ctypes.c_void_p), 10, None, ctypes.byref(lw),
+                         a.ctypes.data_as(ctypes.c_void_p)) == -6                     # 0 < lwork < m
+    assert L.rnpx_dormqr(b"X", b"N", 10, 5, 3, None, 10, None, None, 10, ctypes.byref(sz(0)), None) == -1
+    assert L.rnpx_dormqr(b"L", b"Q", 10, 5, 3, None, 10, None, None, 10, ctypes.byref(sz(0)), None) == -2
+    assert L.rnpx_dormqr(b"L", b"N", 10, 5, 11, None, 10, None, None, 10, ctypes.byref(sz(0)), None) == -5
+    assert L.rnpx_dorgqr(10, 5, 6, None, 10, None, ctypes.byref(sz(0)), None) == -3   # k > n
+
+
+@pytest.mark.skipif(has_gpu(), reason="checks the no-GPU error path")
+def test_no_gpu_fails_loudly():
+    L = R.lib()
+    h = ctypes.c_void_p()
+    assert L.rnpx_create(ctypes.byref(h), 0) == R.RNPX_ERR_NOGPU
+    a = np.asfortranarray(np.random.default_rng(0).standard_normal((64, 32)))
+    a0 = a.copy()
+    tau = np.zeros(32)
+    lw = ctypes.c_size_t(32 * 32)
+    work = np.zeros(32 * 32)
+    st = L.rnpx_dgeqrf(64, 32, a.ctypes.data_as(ctypes.c_void_p), 64, tau.ctypes.data_as(ctypes.c_void_p),
+                       ctypes.byref(lw), work.ctypes.data_as(ctypes.c_void_p))
+    assert st == R.RNPX_ERR_NOGPU
+    np.testing.assert_array_equal(a, a0)          # nothing computed on the CPU
+    assert R.Factor(64, 32, a, 64, tau, np.array([32 * 32], dtype=np.uint64), work) == R.RNPX_ERR_NOGPU
+    with pytest.raises(R.RnpxError):
+        R.Handle()
+
+
+def test_descriptor_layout_matches_c():
+    """rnp2_amd.QR_DESC mirrors struct rnpx_qr_desc byte for byte."""
+    src = r'''
+    #include <stdio.h>
+    #include <stddef.h>
+    #include "rnpx.h"
+    int main(void) {
+      printf("%zu %zu %zu %zu %zu %zu\n", sizeof(rnpx_qr_desc), offsetof(rnpx_qr_desc, nrhs),
+             offsetof(rnpx_qr_desc, a_off), offsetof(rnpx_qr_desc, tau_off), offsetof(rnpx_qr_desc, b_off),
+             sizeof(rnpx_request_info));
+      return 0;
+    }'''
+    got = _run_c(src, "gcc", "c")
+    size, o_nrhs, o_a, o_tau, o_b, rinfo = map(int, got.split())
+    d = R.QR_DESC
+    assert d.itemsize == size
+    assert d.fields["nrhs"][1] == o_nrhs and d.fields["a_off"][1] == o_a
+    assert d.fields["tau_off"][1] == o_tau and d.fields["b_off"][1] == o_b
+    assert ctypes.sizeof(R.RequestInfo) == rinfo
+
+
+def test_make_batch_descriptors():
+    shapes = [(20, 10), (7, 7), (256, 9)]
+    desc, a_len, tau_len, b_len = R.make_batch(shapes, nrhs=2)
+    assert list(desc["m"]) == [20, 7, 256] and list(desc["n"]) == [10, 7, 9]
+    assert np.all(desc["lda"] >= desc["m"])
+    offs = desc["a_off"]
+    for i in range(len(shapes) - 1):            # packed without overlap
+        assert offs[i] + desc["lda"][i] * desc["n"][i] <= offs[i + 1]
+    assert a_len >= offs[-1] + desc["lda"][-1] * desc["n"][-1]
+    assert tau_len == sum(min(m, n) for m, n in shapes)
+    assert b_len >= sum(m * 2 for m, n in shapes)
+
+
+def _run_c(src, cc, ext, extra=()):
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "t." + ext)
+        open(p, "w").write(src)
+        exe = os.path.join(d, "t")
+        cmd = [cc, p, "-I", os.path.join(ROOT, "include"), "-o", exe, *extra]
+        subprocess.run(cmd, check=True, capture_output=True, text=True)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=60,
+                           env={**os.environ, "LD_LIBRARY_PATH": os.path.dirname(R.LIB_PATH)})
+        return r.stdout + r.stderr
+
+
+def test_cpp_surface_compiles_and_queries():
+    """include/rnpx_qr.hpp: the reference's template signatures, queries and
+    assertion messages, for double, complex<double>, float."""
+    src = r'''
+    #include <complex>
+    #include <cstdio>
+    #include "rnpx_qr.hpp"
+    using namespace RNP::LA;
+    int main() {
+      size_t lw = 0;
+      QR::Factor<double>(512, 256, NULL, 512, NULL, &lw, NULL);
+      std::printf("factor %zu\n", lw);
+      lw = 0;
+      QR::Factor<std::complex<double> >(100, 300, NULL, 100, NULL, &lw, NULL);
+      std::printf("zfactor %zu\n", lw);
+      lw = 0;
+      QR::MultQ<double>("L", "C", 100, 40, 30, NULL, 100, NULL, NULL, 100, &lw, NULL);
+      std::printf("multq %zu\n", lw);
+      lw = 0;
+      QR::GenerateQ<float>(100, 40, 30, NULL, 100, NULL, &lw, NULL);
+      std::printf("genq %zu\n", lw);
+      QR::Factor<double>(10, 5, NULL, 9, NULL, &lw, NULL);        /* lda < m */
+      double a[50] = {1}, tau[5], work[100];
+      lw = 100;
+      QR::Factor(10, 5, a, 10, tau, &lw, work);
+      std::printf("status %d\n", QR::LastStatus());
+      return 0;
+    }'''
+    out = _run_c(src, "g++", "cpp", ("-std=c++11", "-L", os.path.dirname(R.LIB_PATH), "-lrnpx",
+                                     f"-Wl,-rpath,{os.path.dirname(R.LIB_PATH)}"))
+    assert "factor 8192" in out and "zfactor 9600" in out
+    assert "multq 2304" in out and "genq 1280" in out
+    assert "Assertion failed in" in out and "lda >= m" in out
+    if not has_gpu():
+        assert f"status {R.RNPX_ERR_NOGPU}" in out and "rnpx: Factor: status" in out
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
