@@ -62,6 +62,8 @@ typedef struct rnpx_tuning {
 	int panel_rpt;    /* panel rows per thread: 1, 2, 4 */
 	int ksplit;       /* split-K slices for W = V^H C (0 = automatic) */
 	int lookahead;    /* 1 = overlap the next panel with the trailing update */
+	int panel;        /* real nb = 64 panels: 0 automatic, 1 column by column only,
+	                     2 CholeskyQR2 + Householder reconstruction where it applies */
 } rnpx_tuning;
 
 /* ---------------------------------------------------------------------

@@ -45,7 +45,7 @@ class ZComplex(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     """rnpx_tuning: the QR::Tuning<T> hook (QR.hpp:41-51)."""
-    _fields_ = [("nb", _i), ("panel_rpt", _i), ("ksplit", _i), ("lookahead", _i)]
+    _fields_ = [("nb", _i), ("panel_rpt", _i), ("ksplit", _i), ("lookahead", _i), ("panel", _i)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -145,6 +145,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rnp2_amd.build` "
                                "(there is no CPU fallback)")
+        # librnpx and torch both need libamdhip64.so.7; whichever loads first
+        # serves the whole process, and torch only initialises on its own
+        # bundled runtime.  Load torch's first when torch is installed.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name, None)
@@ -325,8 +332,10 @@ class Handle:
     def stream(self):
         return lib().rnpx_get_stream(self.h)
 
-    def set_tuning(self, nb=0, panel_rpt=0, ksplit=0, lookahead=0):
-        t = Tuning(nb, panel_rpt, ksplit, lookahead)
+    def set_tuning(self, nb=0, panel_rpt=0, ksplit=0, lookahead=0, panel=0):
+        """panel: 0 automatic, 1 column-by-column reflectors only, 2 CholeskyQR2 +
+        Householder reconstruction for real 64-column panels where it applies."""
+        t = Tuning(nb, panel_rpt, ksplit, lookahead, panel)
         _check(lib().rnpx_set_tuning(self.h, ctypes.byref(t)), "rnpx_set_tuning")
 
     KERNELS = {"update": 0, "wgemm": 1, "panel": 2, "tfactor": 3}

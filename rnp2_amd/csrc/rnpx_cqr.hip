@@ -1,0 +1,473 @@
+// rnpx_cqr.hip -- panel factorization by CholeskyQR2 with Householder
+// reconstruction, for real panels of 64 columns.
+//
+// The column-by-column panel (rnpx_panel.hip) needs one grid-wide reduction
+// per column.  This kernel needs two grid-wide reductions per panel:
+//
+//   1. Q1 R1 = A     (CholeskyQR: Gram A^T A over the grid, Cholesky, A R1^-1)
+//   2. Q  R2 = Q1    (the second pass restores orthogonality to working
+//                     precision when the first pass was accurate enough)
+//   3. Householder reconstruction: Q - [D; 0] = Y U (LU without pivoting,
+//      D = diag(+-1) chosen column by column), so that
+//        Y = the Householder vectors V (unit lower trapezoidal),
+//        tau_j = -U_jj D_j,  T = -U D Y1^-T,  R_householder = D R2 R1.
+//
+// In exact arithmetic this is the factorization the column-by-column
+// reflectors produce (RNP/LA/QR.hpp:78-102 with Reflector::Generate,
+// RNP/LA/Reflector.hpp:117-158): with the panel's Householder form
+// H = I - Y T Y^T, H [I; 0] = Q D, hence Q D - [I; 0] = -Y T Y1^T, an LU
+// factorization with a unit lower factor.  The sign D_j = -sign of the
+// running pivot makes |U_jj| = tau_j in [1, 2] and matches Generate's
+// choice beta_j = -sign(alpha_j) |.|; a zero pivot takes D_j = +1, the
+// reference's beta > 0 when real(alpha) == 0 (Reflector.hpp:131).
+//
+// The second pass's Gram G2 = Q1^T Q1 = I + E is within 2^-27 of I or the
+// panel is rejected.  When |E| <= 2^-40 its Cholesky factor is taken to
+// first order, R2 = I + E_U with E_U = triu(E) - diag(E)/2, and
+// R2^-1 = I - E_U (both exact to O(|E|^2) <= 2^-68); otherwise it is
+// factored.
+//
+// The kernel gives up -- without writing anything but the fail flag --
+// when the first Cholesky meets a pivot below 2^-40 of its diagonal entry, a
+// column whose squared norm is outside [2^-600, 2^600] (so the Gram entries
+// cannot under/overflow), a non-finite value, or the second pass's test
+// above fails.  The column-by-column kernel then runs on the panel.  Every
+// workgroup takes the same decision from the same reduced Gram.
+//
+// Layout: G workgroups of 256 threads; workgroup g stages rows
+// [g*RW, (g+1)*RW) of the panel in LDS (row-major, padded), so the panel is
+// read from HBM once and written once.  Gram partials use FP64 MFMA over the
+// 10 upper 16x16 tiles; they are summed in groups of 16 workgroups by the
+// last workgroup of each group to arrive, then across groups by the last
+// group, always in workgroup order (deterministic); everybody else waits on
+// a done flag.  The 64x64 Cholesky, triangular inverses and the LU run in
+// 16x16 diagonal blocks (one wave each step, v_readlane broadcasts) with the
+// off-diagonal blocks as MFMA block products over the four waves
+// (rnpx_small64.hpp); workgroup 0 does the LU and publishes U^-1, workgroup 1
+// forms R, one more workgroup forms T.
+#include "rnpx_kernels.hpp"
+#include "rnpx_small64.hpp"
+
+namespace rnpx {
+
+namespace {
+
+constexpr int NB = 64;
+constexpr int LDA_S = NB + 1;  // padded LDS row stride (doubles)
+constexpr int kTiles = 10;     // upper 16x16 tiles of a 64x64 symmetric matrix
+constexpr int kGramElems = kTiles * 256;
+constexpr int kGroup = 16;     // workgroups per first-level reduction group
+constexpr double kFirstOrder = 0x1p-40;
+constexpr double kSecondPass = 0x1p-27;
+// sync words: [pass*32 + grp] group arrivals, [pass*32 + 16] group-leader
+// arrivals, [pass*32 + 17] done flag; [64] U^-1 published.
+constexpr int kSyncTop = 16, kSyncDone = 17, kSyncPub = 64;
+
+__device__ __forceinline__ void tile_of(int t, int& ti, int& tj) {
+  // (0,0) (0,1) (0,2) (0,3) (1,1) (1,2) (1,3) (2,2) (2,3) (3,3)
+  ti = (t >= 4) + (t >= 7) + (t >= 9);
+  const int first = (ti == 0) ? 0 : (ti == 1) ? 4 : (ti == 2) ? 7 : 9;
+  tj = ti + (t - first);
+}
+
+// Wait until this wave's global stores have completed (vmcnt(0)).
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// The workgroup arrives on an agent-scope counter after all its global
+// stores are done (release by thread 0, acquire after); returns the count
+// before this arrival.
+__device__ __forceinline__ unsigned arrive(unsigned* counter, unsigned* slot) {
+  wait_stores();
+  __syncthreads();
+  if (threadIdx.x == 0) *slot = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  return *slot;
+}
+
+// Set a flag after all the workgroup's global stores are done.
+__device__ __forceinline__ void publish(unsigned* flag) {
+  wait_stores();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded wait for a flag (abort flag and ~2 s wall-clock limit), then
+// acquire for the whole workgroup.
+__device__ __forceinline__ void wait_flag(unsigned* flag, unsigned* abortf) {
+  if (threadIdx.x == 0) {
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+      if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+      if (wall_clock64() - t0 > kBarrierTimeoutTicks) {
+        __hip_atomic_store(abortf, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// Partial Gram P = Ab^T Ab of this workgroup's RW rows; tile t of the 10 is
+// written row-major (16x16) to out[t*256 ...].
+template <int RW>
+__device__ __forceinline__ void gram_partial(const double (*Ab)[LDA_S], double* out, int lane, int wave) {
+  const int t0 = (wave == 0) ? 0 : (wave == 1) ? 3 : (wave == 2) ? 6 : 8;  // 3, 3, 2, 2 tiles
+  const int nt = (wave < 2) ? 3 : 2;
+  for (int q = 0; q < nt; ++q) {
+    int ti, tj;
+    tile_of(t0 + q, ti, tj);
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < RW; k0 += 4) {
+      const double x = Ab[k0 + (lane >> 4)][16 * ti + (lane & 15)];
+      const double y = Ab[k0 + (lane >> 4)][16 * tj + (lane & 15)];
+      acc = mfma16(x, y, acc);
+    }
+    // item r = P[16 ti + lane/16 + 4r][16 tj + lane%16]
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[(t0 + q) * 256 + ((lane >> 4) + 4 * r) * 16 + (lane & 15)] = acc[r];
+  }
+}
+
+// Sum of partials src[w * kGramElems + e], w = 0..n-1 (n <= 16), in order.
+// Thread tid owns the entry pairs 2 tid + 512 q; eight partials' loads are
+// in flight at once.
+__device__ __forceinline__ void sum_partials(const double* src, int n, double* dst, int tid) {
+  constexpr int Q = kGramElems / 512;  // pairs per thread
+  constexpr int W = 8;                 // partials per batch
+  double2 acc[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) acc[q] = make_double2(0.0, 0.0);
+  for (int w0 = 0; w0 < n; w0 += W) {
+    double2 v[W][Q];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const double2* p = reinterpret_cast<const double2*>(src + (size_t)(w0 + w) * kGramElems) + tid;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) v[w][q] = (w0 + w < n) ? p[256 * q] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (w0 + w < n) {
+          acc[q].x += v[w][q].x;
+          acc[q].y += v[w][q].y;
+        }
+  }
+  double2* d = reinterpret_cast<double2*>(dst) + tid;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) d[256 * q] = acc[q];
+}
+
+// After this workgroup's partial is in part[g]: the reduced Gram of the
+// pass lands in gs for every workgroup.
+__device__ __forceinline__ void gram_allreduce(const CqrArgs& a, int pass, int G, int g, int tid, unsigned* slot) {
+  unsigned* sw = a.sync + pass * 32;
+  double* gs = a.gsum + pass * kGramElems;
+  const int grp = g / kGroup, ngroups = (G + kGroup - 1) / kGroup;
+  const int gbeg = grp * kGroup, gsize = min(kGroup, G - gbeg);
+  if (arrive(&sw[grp], slot) == (unsigned)(gsize - 1)) {
+    // last of its group
+    if (ngroups == 1) {
+      sum_partials(a.part, gsize, gs, tid);
+      publish(&sw[kSyncDone]);
+    } else {
+      sum_partials(a.part + (size_t)gbeg * kGramElems, gsize, a.gpart + (size_t)grp * kGramElems, tid);
+      if (arrive(&sw[kSyncTop], slot) == (unsigned)(ngroups - 1)) {
+        sum_partials(a.gpart, ngroups, gs, tid);
+        publish(&sw[kSyncDone]);
+      }
+    }
+  }
+  wait_flag(&sw[kSyncDone], a.abortf);
+}
+
+// Load the reduced symmetric Gram into M (full 64x64).
+__device__ __forceinline__ void load_gram(const double* gs, double (*M)[LDA_S], int tid) {
+  for (int e = tid; e < kGramElems; e += 256) {
+    const int t = e >> 8, r = (e >> 4) & 15, c = e & 15;
+    int ti, tj;
+    tile_of(t, ti, tj);
+    if (ti == tj && r > c) continue;  // one writer per entry of a diagonal tile
+    const double v = gs[e];
+    M[16 * ti + r][16 * tj + c] = v;
+    M[16 * tj + c][16 * ti + r] = v;
+  }
+}
+
+// Rows [0, RW) of Ab := Ab * X for an upper triangular X (64x64) on MFMA;
+// Ab is overwritten after every wave has read its operands.
+template <int RW>
+__device__ __forceinline__ void times_upper(double (*Ab)[LDA_S], const double (*X)[LDA_S], int lane, int wave) {
+  constexpr int RT = RW / 64;  // row tiles per wave
+  f64x4 acc[RT][4];
+#pragma unroll
+  for (int q = 0; q < RT; ++q) {
+    const int t = wave + 4 * q;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      acc[q][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+      for (int k0 = 0; k0 < 16 * (c + 1); k0 += 4) {  // X[k][16c..] = 0 for k >= 16(c+1)
+        const double x = Ab[16 * t + (lane & 15)][k0 + (lane >> 4)];
+        const double y = X[k0 + (lane >> 4)][16 * c + (lane & 15)];
+        acc[q][c] = mfma16(x, y, acc[q][c]);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < RT; ++q) {
+    const int t = wave + 4 * q;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Ab[16 * t + (lane >> 4) + 4 * r][16 * c + (lane & 15)] = acc[q][c][r];
+  }
+  __syncthreads();
+}
+
+// Wave w's tile row of C = A B for 64x64 upper triangular A, B (row-major in
+// LDS; entries of A below its diagonal are treated as zero when mask_a, and
+// column k of A is scaled by dcol[k] when dcol is given):
+// acc[tj] item r = C[16 w + lane/16 + 4r][16 tj + lane%16].
+__device__ __forceinline__ void upper_product_row(const double (*A)[LDA_S], const double (*B)[LDA_S], bool mask_a, const double* dcol,
+                                  f64x4 (&acc)[4], int lane, int wave) {
+  const int ti = wave;
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    acc[tj] = f64x4{0.0, 0.0, 0.0, 0.0};
+    if (tj < ti) continue;
+    for (int k0 = 16 * ti; k0 < 16 * (tj + 1); k0 += 4) {
+      const int i = 16 * ti + (lane & 15), k = k0 + (lane >> 4);
+      double x = A[i][k];
+      if (mask_a && k < i) x = 0.0;
+      if (dcol) x *= dcol[k];
+      const double y = B[k][16 * tj + (lane & 15)];
+      acc[tj] = mfma16(x, y, acc[tj]);
+    }
+  }
+}
+
+// Largest |M - I| over the workgroup (a NaN entry makes the result NaN).
+__device__ __forceinline__ double dev_from_identity(const double (*M)[LDA_S], double* red4, int tid) {
+  double d = 0.0;
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e / NB, j = e % NB;
+    const double x = fabs(M[i][j] - (i == j ? 1.0 : 0.0));
+    d = (x > d || x != x) ? x : d;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(d, off, 64);
+    d = (o > d || o != o) ? o : d;
+  }
+  if ((tid & 63) == 0) red4[tid >> 6] = d;
+  __syncthreads();
+  double r = red4[0];
+#pragma unroll
+  for (int w = 1; w < 4; ++w) r = (red4[w] > r || red4[w] != red4[w]) ? red4[w] : r;
+  __syncthreads();
+  return r;
+}
+
+}  // namespace
+
+template <int RW>
+__global__ __launch_bounds__(256) void cqr_panel_kernel(CqrArgs a) {
+  __shared__ double Ab[RW][LDA_S];
+  __shared__ double M1[NB][LDA_S];
+  __shared__ double M2[NB][LDA_S];
+  __shared__ double rowbuf[NB];
+  __shared__ double red4[4];
+  __shared__ double gdiag[NB];
+  __shared__ double tmpb[4 * 16 * small64::TS];
+  __shared__ unsigned slot;
+  __shared__ int okf;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = blockIdx.x, G = gridDim.x;
+  const int r0 = g * RW;
+  const int mp = a.mp;
+  double* rsave = a.scratch;          // R1 (saved and used by workgroup 1)
+  double* uinv = a.scratch + 4096;    // U^-1
+  double* ulp = a.scratch + 8192;     // U and L packed
+  double* dpub = a.scratch + 12288;   // D
+  const int tgrp = (G > 2) ? 2 : 1;   // the workgroup that forms T
+
+  // Stage this workgroup's rows (coalesced down each column), zero padding.
+  {
+    constexpr int PER = RW * NB / 256;
+    double v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + 256 * q, r = e % RW, c = e / RW;
+      v[q] = (r0 + r < mp) ? a.A[(size_t)(r0 + r) + (size_t)c * a.lda] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + 256 * q;
+      Ab[e % RW][e / RW] = v[q];
+    }
+  }
+  __syncthreads();
+
+  // Pass 1: Q1 = A R1^-1.
+  gram_partial<RW>(Ab, a.part + (size_t)g * kGramElems, lane, wave);
+  gram_allreduce(a, 0, G, g, tid, &slot);
+  load_gram(a.gsum, M1, tid);
+  __syncthreads();
+  const bool ok1 = small64::chol64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, gdiag, &okf, tid);  // M1 = R1, M2 = R1^-1
+  if (!ok1) {
+    // Every workgroup factored the same Gram and leaves here together;
+    // nothing of the panel has been written.
+    if (g == 0 && tid == 0) *a.fail = 1u;
+    return;
+  }
+  if (g == 1)
+    for (int e = tid; e < NB * NB; e += 256) rsave[e] = M1[e / NB][e % NB];
+  times_upper<RW>(Ab, M2, lane, wave);
+
+  // Pass 2: Q = Q1 R2^-1.
+  gram_partial<RW>(Ab, a.part + (size_t)g * kGramElems, lane, wave);
+  gram_allreduce(a, 1, G, g, tid, &slot);
+  load_gram(a.gsum + kGramElems, M1, tid);
+  __syncthreads();
+  const double dev = dev_from_identity(M1, red4, tid);
+  if (!(dev <= kSecondPass)) {
+    if (g == 0 && tid == 0) *a.fail = 1u;
+    return;
+  }
+  if (dev <= kFirstOrder) {
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int i = e / NB, j = e % NB;
+      const double E = M1[i][j] - (i == j ? 1.0 : 0.0);
+      const double eu = (i < j) ? E : ((i == j) ? 0.5 * E : 0.0);
+      M1[i][j] = (i == j ? 1.0 : 0.0) + eu;  // R2
+      M2[i][j] = (i == j ? 1.0 : 0.0) - eu;  // R2^-1
+    }
+    __syncthreads();
+  } else {
+    small64::chol64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, gdiag, &okf, tid);
+  }
+  times_upper<RW>(Ab, M2, lane, wave);
+  // Ab = Q (this workgroup's rows), M1 = R2.
+
+  if (g == 0) {
+    // LU of Q_top - D, published with U^-1.
+    for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = Ab[e / NB][e % NB];
+    __syncthreads();
+    small64::lu64<LDA_S>(&M1[0][0], rowbuf, &M2[0][0], tmpb, tid);  // M1 = U and L, M2 = U^-1
+    for (int e = tid; e < NB * NB; e += 256) {
+      ulp[e] = M1[e / NB][e % NB];
+      uinv[e] = M2[e / NB][e % NB];
+    }
+    if (tid < NB) dpub[tid] = rowbuf[tid];
+    publish(&a.sync[kSyncPub]);
+    // Rows 64..RW-1 of this workgroup: Y = Q U^-1.
+    if constexpr (RW > NB) {
+      __syncthreads();
+      times_upper<RW>(Ab, M2, lane, wave);
+      for (int e = tid; e < (RW - NB) * NB; e += 256) {
+        const int r = NB + e % (RW - NB), c = e / (RW - NB);
+        const int gr = r0 + r;
+        if (gr < mp) {
+          const double v = Ab[r][c];
+          a.A[(size_t)gr + (size_t)c * a.lda] = v;
+          a.V[(size_t)gr + (size_t)c * a.ldv] = v;
+        }
+      }
+    }
+    // Top block: L below the diagonal (R above is written by workgroup 1),
+    // V's unit lower block, tau.
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int i = e % NB, j = e / NB;
+      const double l = M1[i][j];
+      if (i > j) a.A[(size_t)i + (size_t)j * a.lda] = l;
+      a.V[(size_t)i + (size_t)j * a.ldv] = (i == j) ? 1.0 : ((i > j) ? l : 0.0);
+    }
+    if (tid < NB) a.tau[tid] = -M1[tid][tid] * rowbuf[tid];
+    return;
+  }
+
+  f64x4 racc[4];
+  if (g == 1) {
+    // R = R2 R1 while workgroup 0 reconstructs.
+    for (int e = tid; e < NB * NB; e += 256) M2[e / NB][e % NB] = rsave[e];
+    __syncthreads();
+    upper_product_row(M1, M2, false, nullptr, racc, lane, wave);
+  }
+  wait_flag(&a.sync[kSyncPub], a.abortf);
+  if (g == 1) {
+    // R_householder = D R, the upper triangle of the top block.
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
+        if (i <= j) a.A[(size_t)i + (size_t)j * a.lda] = dpub[i] * racc[tj][r];
+      }
+  }
+  // Y = Q U^-1 for this workgroup's rows.
+  __syncthreads();
+  for (int e = tid; e < NB * NB; e += 256) M2[e / NB][e % NB] = uinv[e];
+  __syncthreads();
+  times_upper<RW>(Ab, M2, lane, wave);
+  for (int e = tid; e < RW * NB; e += 256) {
+    const int r = e % RW, c = e / RW;
+    const int gr = r0 + r;
+    if (gr < mp) {
+      const double v = Ab[r][c];
+      a.A[(size_t)gr + (size_t)c * a.lda] = v;
+      a.V[(size_t)gr + (size_t)c * a.ldv] = v;
+    }
+  }
+  if (g == tgrp) {
+    // T = -U D Z^T with Z = L^-1.
+    for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = ulp[e];
+    if (tid < NB) rowbuf[tid] = dpub[tid];
+    __syncthreads();
+    small64::inv_unit_lower64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, tid);  // M2 = L^-1 (lower blocks)
+    // Z^T is upper: stage it transposed in Ab's top rows.
+    for (int e = tid; e < NB * NB; e += 256) Ab[e % NB][e / NB] = M2[e / NB][e % NB];
+    __syncthreads();
+    f64x4 tacc[4];
+    upper_product_row(M1, Ab, true, rowbuf, tacc, lane, wave);
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
+        a.T[(size_t)i + (size_t)j * NB] = (i <= j) ? -tacc[tj][r] : 0.0;
+      }
+  }
+}
+
+int cqr_panel_groups(int mp, int max_groups, int* rw) {
+  if (mp < 2 * NB) return 0;
+  if (max_groups > kCqrMaxGroups) max_groups = kCqrMaxGroups;
+  for (int r : {64, 128}) {
+    const int G = (mp + r - 1) / r;
+    if (G <= max_groups) {
+      *rw = r;
+      return G;
+    }
+  }
+  return 0;
+}
+
+hipError_t launch_cqr_panel(const CqrArgs& a, int G, int rw, hipStream_t st) {
+  if (a.mp < 2 * NB || G < 2 || G > kCqrMaxGroups || (long)G * rw < a.mp || (long)(G - 1) * rw >= a.mp ||
+      a.lda < a.mp || a.ldv < a.mp)
+    return hipErrorInvalidValue;
+  if (rw == 64)
+    hipLaunchKernelGGL((cqr_panel_kernel<64>), dim3(G), dim3(256), 0, st, a);
+  else if (rw == 128)
+    hipLaunchKernelGGL((cqr_panel_kernel<128>), dim3(G), dim3(256), 0, st, a);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+}  // namespace rnpx

@@ -44,6 +44,7 @@ struct Handle {
   hipEvent_t ev_pan[2] = {nullptr, nullptr}, ev_next = nullptr, ev_start = nullptr;
   Buffer stage1, stage2, stage3;  // host-argument staging for the drop-in calls
   Buffer lsbuf, lsnrm;            // least-squares refinement scratch
+  Buffer cqr_part, cqr_gsum, cqr_scratch;  // CholeskyQR2 panel scratch
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
   int update_prec = 0;                 // 1: trailing-update products on FP32 MFMA

@@ -19,6 +19,29 @@ constexpr int kMaxPanelGroups = 128;
 // Grid-barrier timeout: 2 s of the 100 MHz wall clock.
 constexpr uint64_t kBarrierTimeoutTicks = 200000000ull;
 
+// Bounded grid barrier: arrive with release, spin on a relaxed load, then an
+// acquire fence.  Gives up (and raises the abort flag, which every spinner
+// also watches) after ~2 s of wall clock so that no wave can spin forever.
+__device__ inline void grid_barrier(unsigned* counter, unsigned* abortf, unsigned target) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t t0 = wall_clock64();
+    while (true) {
+      unsigned v = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (v >= target) break;
+      if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+      if (wall_clock64() - t0 > kBarrierTimeoutTicks) {
+        __hip_atomic_store(abortf, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
 template <typename T>
 struct PanelArgs {
   int mp, nbp;         // panel rows, panel columns (nbp <= NB)
@@ -30,17 +53,46 @@ struct PanelArgs {
   T* rowbuf;           // scratch: 2 * NB elements
   unsigned* counter;   // scratch: arrival counter, zero at launch
   unsigned* abortf;    // sticky error flag: set when a barrier times out
+  const unsigned* run_if = nullptr;  // when set, the kernel runs only if *run_if != 0
 };
 
 template <typename T>
 hipError_t launch_panel(const PanelArgs<T>& a, int nb, int rpt, hipStream_t st);
 
+// Real 64-column panel by CholeskyQR2 + Householder reconstruction
+// (rnpx_cqr.hip).  Outputs as the column-wise panel, plus T directly; on
+// rejection it writes only *fail = 1 and the caller runs the column-wise
+// panel (launched gated on fail) and the T kernel instead.
+struct CqrArgs {
+  int mp;               // panel rows (>= 128)
+  double* A; long lda;  // panel top-left
+  double* tau;          // out: 64 scalars
+  double* V; long ldv;  // out: clean V (ldv >= mp)
+  double* T;            // out: 64 x 64 T, leading dimension 64
+  double* part;         // scratch: G * 2560 (partial Gram tiles)
+  double* gpart;        // scratch: 16 * 2560 (group sums)
+  double* gsum;         // scratch: 2 * 2560 (reduced Gram of each pass)
+  double* scratch;      // scratch: kCqrScratch doubles
+  unsigned* sync;       // kCqrSyncWords words, zero at launch
+  unsigned* abortf;     // sticky error flag
+  unsigned* fail;       // zero at launch; 1 = redo this panel column by column
+};
+constexpr int kCqrSyncWords = 128;
+constexpr int kCqrScratch = 3 * 4096 + 64;
+constexpr int kCqrMaxGroups = 256;
+// Workgroups (and rows per workgroup in *rw) for an mp-row panel, 0 if the
+// panel is outside what the kernel takes (mp < 128 or too many rows).
+int cqr_panel_groups(int mp, int max_groups, int* rw);
+hipError_t launch_cqr_panel(const CqrArgs& a, int G, int rw, hipStream_t st);
+
 // T factor of a forward, columnwise block reflector from its Gram matrix
 // (the larft analogue, RNP/LA/Reflector.hpp:573-626): k <= NB reflectors,
 // gram(j,c) = V(:,j)^H V(:,c) for j < c and tau_c on the diagonal.
 // Writes the NB x NB block of T (zeros below the diagonal and beyond k).
+// With run_if set the kernel returns at once unless *run_if != 0.
 template <typename T>
-hipError_t launch_tfactor(const T* gram, int ldg, int k, int nb, T* Tout, int ldt, hipStream_t st);
+hipError_t launch_tfactor(const T* gram, int ldg, int k, int nb, T* Tout, int ldt, hipStream_t st,
+                          const unsigned* run_if = nullptr);
 
 // Gram of a clean V and tau into the layout launch_tfactor reads:
 // gram(j,c) = sum_r conj(V(r,j)) V(r,c) for j < c, tau_c on the diagonal.

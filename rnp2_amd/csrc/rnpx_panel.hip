@@ -30,29 +30,6 @@
 
 namespace rnpx {
 
-// Bounded grid barrier: arrive with release, spin on a relaxed load, then an
-// acquire fence.  Gives up (and raises the abort flag, which every spinner
-// also watches) after ~2 s of wall clock so that no wave can spin forever.
-__device__ inline void grid_barrier(unsigned* counter, unsigned* abortf, unsigned target) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t t0 = wall_clock64();
-    while (true) {
-      unsigned v = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (v >= target) break;
-      if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
-      if (wall_clock64() - t0 > kBarrierTimeoutTicks) {
-        __hip_atomic_store(abortf, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-}
-
 template <typename T, int N>
 __device__ __forceinline__ T select_col(const T (&row)[N], int c) {
   T x = S<T>::zero();
@@ -105,6 +82,9 @@ template <> struct RedChunk<zd> { static constexpr int value = 32; };
 
 template <typename T, int NB, int RPT>
 __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
+  // Gated launch: the whole grid reads the same word and returns together,
+  // before any barrier.
+  if (p.run_if && *p.run_if == 0u) return;
   constexpr int CH = (RedChunk<T>::value < NB) ? RedChunk<T>::value : NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = blockIdx.x, G = gridDim.x;

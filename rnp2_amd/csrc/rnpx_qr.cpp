@@ -275,10 +275,54 @@ static int tfactor_from_v(Handle* h, int n, int k, int nb, const T* V, long ldv,
 // ---------------------------------------------------------------------------
 // Factor (QR.hpp:129-197) on device memory.
 // ---------------------------------------------------------------------------
+// Whether panel p (rows [i, m), ib columns) takes the CholeskyQR2 +
+// reconstruction kernel; G and rows per workgroup in *g / *rw.
 template <typename T>
-static int launch_panel_step(Handle* h, int p, int i, int ib, int m, int nb, T* A, long lda, T* tau, T* V, T* gram,
-                             hipStream_t st) {
+static bool use_cqr(const Handle* h, int mp, int ib, int nb, int* g, int* rw) {
+  if constexpr (S<T>::cplx) {
+    return false;
+  } else {
+    if (h->tune.panel == 1 || nb != 64 || ib != 64) return false;
+    *g = cqr_panel_groups(mp, std::min(kCqrMaxGroups, h->num_cus), rw);
+    return *g > 0;
+  }
+}
+
+// One panel step.  sync holds, per panel p: [1 + p] the column-wise kernel's
+// barrier counter, [1 + npan + p] the CholeskyQR2 kernel's fail flag, and
+// kCqrSyncWords words at 1 + 2 npan + kCqrSyncWords p for its reductions.  With the CholeskyQR2 kernel the column-wise kernel is
+// launched behind it gated on the fail flag; *gated receives that flag's
+// address (the T-factor kernel is gated the same way), or null.
+template <typename T>
+static int launch_panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, T* A, long lda, T* tau, T* V,
+                             T* gram, T* Tm, hipStream_t st, const unsigned** gated) {
   const int mp = m - i;
+  unsigned* sync = (unsigned*)h->sync.p;
+  *gated = nullptr;
+  hipEvent_t ev;
+  h->prof_begin(&ev, st);
+  int G = 0, rw = 0;
+  if (use_cqr<T>(h, mp, ib, nb, &G, &rw)) {
+    if constexpr (!S<T>::cplx) {
+      CqrArgs ca;
+      ca.mp = mp;
+      ca.A = A + (size_t)i + (size_t)i * lda;
+      ca.lda = lda;
+      ca.tau = tau + i;
+      ca.V = V;
+      ca.ldv = mp;
+      ca.T = Tm;
+      ca.part = (double*)h->cqr_part.p;
+      ca.gpart = (double*)h->cqr_gsum.p + 2 * 2560;
+      ca.gsum = (double*)h->cqr_gsum.p;
+      ca.scratch = (double*)h->cqr_scratch.p;
+      ca.sync = sync + 1 + 2 * npan + (size_t)kCqrSyncWords * p;
+      ca.abortf = (unsigned*)h->flag.p;
+      ca.fail = sync + 1 + npan + p;
+      RNPX_TRY(launch_cqr_panel(ca, G, rw, st));
+      *gated = ca.fail;
+    }
+  }
   PanelArgs<T> pa;
   pa.mp = mp;
   pa.nbp = ib;
@@ -290,11 +334,10 @@ static int launch_panel_step(Handle* h, int p, int i, int ib, int m, int nb, T* 
   pa.gram = gram;
   pa.part = (T*)h->ppart.p;
   pa.rowbuf = (T*)h->rowbuf.p;
-  pa.counter = (unsigned*)h->sync.p + 1 + p;
+  pa.counter = sync + 1 + p;
   pa.abortf = (unsigned*)h->flag.p;
+  pa.run_if = *gated;
   const int rpt = pick_rpt(h, mp, nb, S<T>::cplx);
-  hipEvent_t ev;
-  h->prof_begin(&ev, st);
   RNPX_TRY(launch_panel<T>(pa, nb, rpt, st));
   // panel traffic: the mp x ib panel read and written once, plus V and tau
   h->prof_end(K_PANEL, ev, (S<T>::cplx ? 4.0 : 1.0) * (2.0 * mp * ib * ib - 2.0 / 3.0 * ib * ib * ib),
@@ -333,8 +376,19 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
   }
   RNPX_TRY(h->ppart.ensure(sizeof(T) * 2 * (size_t)kMaxPanelGroups * nb + 64, h));
   RNPX_TRY(h->rowbuf.ensure(sizeof(T) * 2 * (size_t)nb, h));
-  RNPX_TRY(h->sync.ensure(sizeof(unsigned) * (size_t)(npan + 1), h));
-  RNPX_TRY(launch_zero_u32((unsigned*)h->sync.p, npan + 1, st));
+  const int sync_words = 1 + 2 * npan + kCqrSyncWords * npan;
+  RNPX_TRY(h->sync.ensure(sizeof(unsigned) * (size_t)sync_words, h));
+  {
+    int g0 = 0, rw0 = 0;
+    if (use_cqr<T>(h, m, std::min(nb, k), nb, &g0, &rw0)) {
+      // later panels may use more (smaller) workgroups than the first
+      RNPX_TRY(h->cqr_part.ensure(sizeof(double) * (size_t)std::min(kCqrMaxGroups, h->num_cus) * 2560, h));
+      RNPX_TRY(h->cqr_gsum.ensure(sizeof(double) * (2 + 16) * 2560, h));  // gsum, then group sums
+      RNPX_TRY(h->cqr_scratch.ensure(sizeof(double) * kCqrScratch, h));
+    }
+  }
+  RNPX_TRY(launch_zero_u32((unsigned*)h->sync.p, sync_words, st));
+  const unsigned* gate[2] = {nullptr, nullptr};  // T-factor gate of the panel in each buffer
 
   auto update = [&](int p, int i, int ib, int c0, int c1) -> int {
     // columns [c0, c1) of the trailing matrix, rows [i, m), reflectors of panel p
@@ -347,15 +401,20 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
     const int q = look ? (p & 1) : 0;
     hipEvent_t ev;
     h->prof_begin(&ev);
-    RNPX_TRY(launch_tfactor<T>((const T*)gb[q]->p, nb, ib, nb, (T*)tb[q]->p, nb, st));
+    RNPX_TRY(launch_tfactor<T>((const T*)gb[q]->p, nb, ib, nb, (T*)tb[q]->p, nb, st, gate[q]));
     h->prof_end(K_TFACTOR, ev, 0.0, 0.0);
     return RNPX_OK;
+  };
+  auto panel = [&](int p, int i, int ib, hipStream_t ps) -> int {
+    const int q = look ? (p & 1) : 0;
+    return launch_panel_step<T>(h, p, npan, i, ib, m, nb, A, lda, tau, (T*)vb[q]->p, (T*)gb[q]->p, (T*)tb[q]->p,
+                                ps, &gate[q]);
   };
 
   if (!look) {
     for (int p = 0, i = 0; i < k; i += nb, ++p) {
       const int ib = std::min(nb, k - i);
-      int s = launch_panel_step<T>(h, p, i, ib, m, nb, A, lda, tau, (T*)vb[0]->p, (T*)gb[0]->p, st);
+      int s = panel(p, i, ib, st);
       if (s) return s;
       if (i + ib < n) {
         if ((s = tfactor(p, ib))) return s;
@@ -372,7 +431,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
   RNPX_TRY(hipStreamWaitEvent(ps, h->ev_start, 0));
   {
     const int ib0 = std::min(nb, k);
-    if ((s = launch_panel_step<T>(h, 0, 0, ib0, m, nb, A, lda, tau, (T*)vb[0]->p, (T*)gb[0]->p, ps))) return s;
+    if ((s = panel(0, 0, ib0, ps))) return s;
     RNPX_TRY(hipEventRecord(h->ev_pan[0], ps));
   }
   for (int p = 0, i = 0; i < k; i += nb, ++p) {
@@ -387,8 +446,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
       RNPX_TRY(hipEventRecord(h->ev_next, st));
       RNPX_TRY(hipStreamWaitEvent(ps, h->ev_next, 0));
       const int q1 = (p + 1) & 1;
-      if ((s = launch_panel_step<T>(h, p + 1, i1, ib1, m, nb, A, lda, tau, (T*)vb[q1]->p, (T*)gb[q1]->p, ps)))
-        return s;
+      if ((s = panel(p + 1, i1, ib1, ps))) return s;
       RNPX_TRY(hipEventRecord(h->ev_pan[q1], ps));
       if ((s = update(p, i, ib, i1 + ib1, n))) return s;
     } else {

@@ -17,7 +17,8 @@ namespace rnpx {
 
 template <typename T, int NB>
 __global__ __launch_bounds__(256) void tfactor_kernel(const T* __restrict__ gram, int ldg, int k,
-                                                       T* __restrict__ Tout, int ldt) {
+                                                       T* __restrict__ Tout, int ldt, const unsigned* run_if) {
+  if (run_if && *run_if == 0u) return;
   constexpr int LEAF = 16;
   __shared__ T Gs[NB][NB + 1];  // Gs[l][c] = G(l,c), l < c; diagonal = tau
   __shared__ T Ts[NB][NB + 1];
@@ -79,11 +80,15 @@ __global__ __launch_bounds__(256) void tfactor_kernel(const T* __restrict__ gram
 }
 
 template <typename T>
-hipError_t launch_tfactor(const T* gram, int ldg, int k, int nb, T* Tout, int ldt, hipStream_t st) {
+hipError_t launch_tfactor(const T* gram, int ldg, int k, int nb, T* Tout, int ldt, hipStream_t st,
+                          const unsigned* run_if) {
   if (k < 0 || k > nb || ldg < nb || ldt < nb) return hipErrorInvalidValue;
-  if (nb == 16) hipLaunchKernelGGL((tfactor_kernel<T, 16>), dim3(1), dim3(256), 0, st, gram, ldg, k, Tout, ldt);
-  else if (nb == 32) hipLaunchKernelGGL((tfactor_kernel<T, 32>), dim3(1), dim3(256), 0, st, gram, ldg, k, Tout, ldt);
-  else if (nb == 64) hipLaunchKernelGGL((tfactor_kernel<T, 64>), dim3(1), dim3(256), 0, st, gram, ldg, k, Tout, ldt);
+  if (nb == 16)
+    hipLaunchKernelGGL((tfactor_kernel<T, 16>), dim3(1), dim3(256), 0, st, gram, ldg, k, Tout, ldt, run_if);
+  else if (nb == 32)
+    hipLaunchKernelGGL((tfactor_kernel<T, 32>), dim3(1), dim3(256), 0, st, gram, ldg, k, Tout, ldt, run_if);
+  else if (nb == 64)
+    hipLaunchKernelGGL((tfactor_kernel<T, 64>), dim3(1), dim3(256), 0, st, gram, ldg, k, Tout, ldt, run_if);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -139,8 +144,9 @@ hipError_t launch_gram(int n, int k, const T* V, long ldv, const T* tau, T* gram
   return hipGetLastError();
 }
 
-template hipError_t launch_tfactor<double>(const double*, int, int, int, double*, int, hipStream_t);
-template hipError_t launch_tfactor<zd>(const zd*, int, int, int, zd*, int, hipStream_t);
+template hipError_t launch_tfactor<double>(const double*, int, int, int, double*, int, hipStream_t,
+                                           const unsigned*);
+template hipError_t launch_tfactor<zd>(const zd*, int, int, int, zd*, int, hipStream_t, const unsigned*);
 template hipError_t launch_gram<double>(int, int, const double*, long, const double*, double*, int, double*, int,
                                         hipStream_t);
 template hipError_t launch_gram<zd>(int, int, const zd*, long, const zd*, zd*, int, zd*, int, hipStream_t);
