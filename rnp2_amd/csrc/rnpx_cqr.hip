@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256) void cqr_panel_kernel(CqrArgs a) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
-        a.T[(size_t)i + (size_t)j * NB] = (i <= j) ? -tacc[tj][r] : 0.0;
+        a.T[(size_t)i + (size_t)j * a.ldt] = (i <= j) ? -tacc[tj][r] : 0.0;
       }
   }
 }
@@ -459,7 +459,7 @@ int cqr_panel_groups(int mp, int max_groups, int* rw) {
 
 hipError_t launch_cqr_panel(const CqrArgs& a, int G, int rw, hipStream_t st) {
   if (a.mp < 2 * NB || G < 2 || G > kCqrMaxGroups || (long)G * rw < a.mp || (long)(G - 1) * rw >= a.mp ||
-      a.lda < a.mp || a.ldv < a.mp)
+      a.lda < a.mp || a.ldv < a.mp || a.ldt < NB)
     return hipErrorInvalidValue;
   if (rw == 64)
     hipLaunchKernelGGL((cqr_panel_kernel<64>), dim3(G), dim3(256), 0, st, a);

@@ -45,6 +45,7 @@ struct Handle {
   Buffer stage1, stage2, stage3;  // host-argument staging for the drop-in calls
   Buffer lsbuf, lsnrm;            // least-squares refinement scratch
   Buffer cqr_part, cqr_gsum, cqr_scratch;  // CholeskyQR2 panel scratch
+  Buffer wbuf2, zbuf2, part2, misc2;       // GEMM scratch of the panel stream
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
   int update_prec = 0;                 // 1: trailing-update products on FP32 MFMA

@@ -68,7 +68,7 @@ struct CqrArgs {
   double* A; long lda;  // panel top-left
   double* tau;          // out: 64 scalars
   double* V; long ldv;  // out: clean V (ldv >= mp)
-  double* T;            // out: 64 x 64 T, leading dimension 64
+  double* T; long ldt;  // out: 64 x 64 T (ldt >= 64)
   double* part;         // scratch: G * 2560 (partial Gram tiles)
   double* gpart;        // scratch: 16 * 2560 (group sums)
   double* gsum;         // scratch: 2 * 2560 (reduced Gram of each pass)

@@ -211,50 +211,65 @@ static int pick_ksplit(const Handle* h, int rows, int cols_tiles) {
 //   side R: C(mc x nc) = C op(H),   V is nc x k
 // op = H for trans 'N', H^H for trans 'C'.
 // ---------------------------------------------------------------------------
+// Where a block application runs: stream, GEMM scratch, whether it is
+// profiled.  The default is the handle's stream and main scratch.
+struct GemmWork {
+  hipStream_t st;
+  Buffer* w;
+  Buffer* z;
+  Buffer* part;
+  bool prof;
+};
+
 template <typename T>
 static int apply_block_dev(Handle* h, char side, char trans, int mc, int nc, int k, const T* V, long ldv,
-                           const T* Tm, long ldt, T* C, long ldc) {
+                           const T* Tm, long ldt, T* C, long ldc, const GemmWork* gw = nullptr) {
   if (mc <= 0 || nc <= 0 || k <= 0) return RNPX_OK;
-  hipStream_t st = h->stream;
+  const GemmWork main_work = {h->stream, &h->wbuf, &h->zbuf, &h->part, true};
+  if (!gw) gw = &main_work;
+  hipStream_t st = gw->st;
+  Buffer& wb = *gw->w;
+  Buffer& zb = *gw->z;
+  Buffer& pb = *gw->part;
   const T one = S<T>::one(), zero = S<T>::zero(), mone = S<T>::zero() - S<T>::one();
   if (side == 'L') {
     // W = V^H C (k x nc), Z = op(T) W, C -= V Z
     const int tiles = (nc + 127) / 128;
     int ks = pick_ksplit(h, mc, tiles);
-    RNPX_TRY(h->wbuf.ensure(sizeof(T) * (size_t)k * nc, h));
-    RNPX_TRY(h->zbuf.ensure(sizeof(T) * (size_t)k * nc, h));
-    if (ks > 1) RNPX_TRY(h->part.ensure(sizeof(T) * (size_t)ks * k * nc, h));
-    T* W = (T*)h->wbuf.p;
-    T* Z = (T*)h->zbuf.p;
+    RNPX_TRY(wb.ensure(sizeof(T) * (size_t)k * nc, h));
+    RNPX_TRY(zb.ensure(sizeof(T) * (size_t)k * nc, h));
+    if (ks > 1) RNPX_TRY(pb.ensure(sizeof(T) * (size_t)ks * k * nc, h));
+    T* W = (T*)wb.p;
+    T* Z = (T*)zb.p;
     const double fl = (S<T>::cplx ? 8.0 : 2.0) * (double)mc * nc * k;
     const double esz = sizeof(T);
-    hipEvent_t ev;
+    hipEvent_t ev = nullptr;
     // Mixed precision (real only): the two big products on FP32 MFMA.
     bool f32 = false;
     if constexpr (!S<T>::cplx) f32 = (h->update_prec == 1);
-    h->prof_begin(&ev);
+    if (gw->prof) h->prof_begin(&ev, st);
     if constexpr (!S<T>::cplx) {
-      if (f32) RNPX_TRY(gemm32('C', 'N', k, nc, mc, 1.0, V, ldv, C, ldc, 0.0, W, k, ks, (T*)h->part.p, st));
+      if (f32) RNPX_TRY(gemm32('C', 'N', k, nc, mc, 1.0, V, ldv, C, ldc, 0.0, W, k, ks, (T*)pb.p, st));
     }
-    if (!f32) RNPX_TRY(gemm<T>('C', 'N', k, nc, mc, one, V, ldv, C, ldc, zero, W, k, ks, (T*)h->part.p, st));
-    h->prof_end(K_WGEMM, ev, fl, esz * ((double)mc * nc + (double)mc * k + (double)k * nc));
+    if (!f32) RNPX_TRY(gemm<T>('C', 'N', k, nc, mc, one, V, ldv, C, ldc, zero, W, k, ks, (T*)pb.p, st));
+    if (gw->prof) h->prof_end(K_WGEMM, ev, fl, esz * ((double)mc * nc + (double)mc * k + (double)k * nc), st);
     RNPX_TRY(gemm<T>(trans == 'N' ? 'N' : 'C', 'N', k, nc, k, one, Tm, ldt, W, k, zero, Z, k, 1, nullptr, st));
-    h->prof_begin(&ev);
+    if (gw->prof) h->prof_begin(&ev, st);
     if constexpr (!S<T>::cplx) {
       if (f32) RNPX_TRY(gemm32('N', 'N', mc, nc, k, -1.0, V, ldv, Z, k, 1.0, C, ldc, 1, nullptr, st));
     }
     if (!f32) RNPX_TRY(gemm<T>('N', 'N', mc, nc, k, mone, V, ldv, Z, k, one, C, ldc, 1, nullptr, st));
-    h->prof_end(K_UPDATE, ev, fl, esz * (2.0 * mc * nc + (double)mc * k + (double)k * nc));
+    if (gw->prof) h->prof_end(K_UPDATE, ev, fl, esz * (2.0 * mc * nc + (double)mc * k + (double)k * nc), st);
   } else {
     // W = C V (mc x k), Z = W op(T), C -= Z V^H
     const int tiles = (mc + 127) / 128;
     int ks = pick_ksplit(h, nc, tiles);
-    RNPX_TRY(h->wbuf.ensure(sizeof(T) * (size_t)k * mc, h));
-    RNPX_TRY(h->zbuf.ensure(sizeof(T) * (size_t)k * mc, h));
-    if (ks > 1) RNPX_TRY(h->part.ensure(sizeof(T) * (size_t)ks * k * mc, h));
-    T* W = (T*)h->wbuf.p;
-    T* Z = (T*)h->zbuf.p;
-    RNPX_TRY(gemm<T>('N', 'N', mc, k, nc, one, C, ldc, V, ldv, zero, W, mc, ks, (T*)h->part.p, st));
+    RNPX_TRY(wb.ensure(sizeof(T) * (size_t)k * mc, h));
+    RNPX_TRY(zb.ensure(sizeof(T) * (size_t)k * mc, h));
+    if (ks > 1) RNPX_TRY(pb.ensure(sizeof(T) * (size_t)ks * k * mc, h));
+    T* W = (T*)wb.p;
+    T* Z = (T*)zb.p;
+    RNPX_TRY(gemm<T>('N', 'N', mc, k, nc, one, C, ldc, V, ldv, zero, W, mc, ks, (T*)pb.p, st));
     RNPX_TRY(gemm<T>('N', trans == 'N' ? 'N' : 'C', mc, k, k, one, W, mc, Tm, ldt, zero, Z, mc, 1, nullptr, st));
     RNPX_TRY(gemm<T>('N', 'C', mc, nc, k, mone, Z, mc, V, ldv, one, C, ldc, 1, nullptr, st));
   }
@@ -288,19 +303,20 @@ static bool use_cqr(const Handle* h, int mp, int ib, int nb, int* g, int* rw) {
   }
 }
 
-// One panel step.  sync holds, per panel p: [1 + p] the column-wise kernel's
-// barrier counter, [1 + npan + p] the CholeskyQR2 kernel's fail flag, and
-// kCqrSyncWords words at 1 + 2 npan + kCqrSyncWords p for its reductions.  With the CholeskyQR2 kernel the column-wise kernel is
-// launched behind it gated on the fail flag; *gated receives that flag's
-// address (the T-factor kernel is gated the same way), or null.
+// One inner panel: rows [i, m), columns [i, i + ib) of A factored in place
+// (R and V packed, tau), its clean V written at V (leading dimension ldv)
+// and its T (ib x ib, leading dimension ldt) at Tm, all on stream st.  sync
+// holds, per inner panel p: [1 + p] the column-wise kernel's barrier
+// counter, [1 + npan + p] the CholeskyQR2 kernel's fail flag, and
+// kCqrSyncWords words at 1 + 2 npan + kCqrSyncWords p for its reductions.
+// Where the CholeskyQR2 kernel applies it runs first; the column-wise panel
+// and the T-factor kernel behind it are gated on its fail flag.
 template <typename T>
-static int launch_panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, T* A, long lda, T* tau, T* V,
-                             T* gram, T* Tm, hipStream_t st, const unsigned** gated) {
+static int panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, T* A, long lda, T* tau, T* V, long ldv,
+                      T* Tm, long ldt, hipStream_t st) {
   const int mp = m - i;
   unsigned* sync = (unsigned*)h->sync.p;
-  *gated = nullptr;
-  hipEvent_t ev;
-  h->prof_begin(&ev, st);
+  const unsigned* gated = nullptr;
   int G = 0, rw = 0;
   if (use_cqr<T>(h, mp, ib, nb, &G, &rw)) {
     if constexpr (!S<T>::cplx) {
@@ -310,8 +326,9 @@ static int launch_panel_step(Handle* h, int p, int npan, int i, int ib, int m, i
       ca.lda = lda;
       ca.tau = tau + i;
       ca.V = V;
-      ca.ldv = mp;
+      ca.ldv = ldv;
       ca.T = Tm;
+      ca.ldt = ldt;
       ca.part = (double*)h->cqr_part.p;
       ca.gpart = (double*)h->cqr_gsum.p + 2 * 2560;
       ca.gsum = (double*)h->cqr_gsum.p;
@@ -320,7 +337,7 @@ static int launch_panel_step(Handle* h, int p, int npan, int i, int ib, int m, i
       ca.abortf = (unsigned*)h->flag.p;
       ca.fail = sync + 1 + npan + p;
       RNPX_TRY(launch_cqr_panel(ca, G, rw, st));
-      *gated = ca.fail;
+      gated = ca.fail;
     }
   }
   PanelArgs<T> pa;
@@ -330,26 +347,32 @@ static int launch_panel_step(Handle* h, int p, int npan, int i, int ib, int m, i
   pa.lda = lda;
   pa.tau = tau + i;
   pa.V = V;
-  pa.ldv = mp;
-  pa.gram = gram;
+  pa.ldv = ldv;
+  pa.gram = (T*)h->gram.p;
   pa.part = (T*)h->ppart.p;
   pa.rowbuf = (T*)h->rowbuf.p;
   pa.counter = sync + 1 + p;
   pa.abortf = (unsigned*)h->flag.p;
-  pa.run_if = *gated;
+  pa.run_if = gated;
   const int rpt = pick_rpt(h, mp, nb, S<T>::cplx);
   RNPX_TRY(launch_panel<T>(pa, nb, rpt, st));
-  // panel traffic: the mp x ib panel read and written once, plus V and tau
-  h->prof_end(K_PANEL, ev, (S<T>::cplx ? 4.0 : 1.0) * (2.0 * mp * ib * ib - 2.0 / 3.0 * ib * ib * ib),
-              sizeof(T) * (3.0 * mp * ib + ib), st);
+  RNPX_TRY(launch_tfactor<T>((const T*)h->gram.p, nb, ib, nb, Tm, (int)ldt, st, gated));
   return RNPX_OK;
 }
 
-// Factor (QR.hpp:129-197) on device memory.  With look-ahead (the default)
-// panel p+1 is factored on a second stream while the trailing update of
-// panel p runs: the update of panel p is split into the next panel's columns
-// (done first, then the panel stream is released) and the rest.  V, Gram and
-// T are double-buffered between consecutive panels.
+// Factor (QR.hpp:129-197) on device memory.
+//
+// Outer blocks of nbo = 2 nb columns (the default; nb = 64) are factored as
+// two inner panels of nb columns: panel 1, its block reflector applied to
+// the next nb columns, panel 2, then the two compact-WY forms are merged,
+//   V = [V1 | (0; V2)],  T = [T1, -T1 (V1^H V2) T2; 0, T2],
+// so that the trailing matrix sees one rank-nbo update (K = 128 GEMMs run
+// half as many passes over it at higher arithmetic intensity than K = 64).
+// With look-ahead (the default) the panel phase of block b+1 runs on a
+// second stream while the trailing update of block b runs: the update of
+// block b is split into block b+1's columns (done first, then the panel
+// stream is released) and the rest.  V and T are double-buffered between
+// consecutive outer blocks; V uses leading dimension m.
 template <typename T>
 int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
   if (m < 0) return -2;
@@ -359,21 +382,23 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
   if (k == 0) return RNPX_OK;
   hipStream_t st = h->stream;
   const int nb = pick_nb<T>(h, m, n);
-  const int npan = (k + nb - 1) / nb;
+  const int nbo = (h->tune.nb == 0 || h->tune.nb == 128) ? 2 * nb : nb;
+  const int npan = (k + nb - 1) / nb;     // inner panels
+  const int nob = (k + nbo - 1) / nbo;    // outer blocks
   const int rpt0 = pick_rpt(h, m, nb, S<T>::cplx);
   const int G0 = (m + 256 * rpt0 - 1) / (256 * rpt0);
   // Every panel workgroup must be resident at once (they meet at grid
   // barriers): one 256-thread workgroup per CU at these register counts.
   if (G0 > std::min(kMaxPanelGroups, h->num_cus)) return RNPX_ERR_UNSUPPORTED;
-  const bool look = (h->tune.lookahead >= 0) && npan > 1;
+  const bool look = (h->tune.lookahead >= 0) && nob > 1;
+  const long ldv = m, ldt = nbo;
   Buffer* vb[2] = {&h->vbuf, &h->vbuf2};
-  Buffer* gb[2] = {&h->gram, &h->gram2};
   Buffer* tb[2] = {&h->tbuf, &h->tbuf2};
   for (int q = 0; q < (look ? 2 : 1); ++q) {
-    RNPX_TRY(vb[q]->ensure(sizeof(T) * (size_t)m * nb, h));
-    RNPX_TRY(gb[q]->ensure(sizeof(T) * (size_t)nb * nb, h));
-    RNPX_TRY(tb[q]->ensure(sizeof(T) * (size_t)nb * nb, h));
+    RNPX_TRY(vb[q]->ensure(sizeof(T) * (size_t)m * nbo, h));
+    RNPX_TRY(tb[q]->ensure(sizeof(T) * (size_t)nbo * nbo, h));
   }
+  RNPX_TRY(h->gram.ensure(sizeof(T) * (size_t)nb * nb, h));
   RNPX_TRY(h->ppart.ensure(sizeof(T) * 2 * (size_t)kMaxPanelGroups * nb + 64, h));
   RNPX_TRY(h->rowbuf.ensure(sizeof(T) * 2 * (size_t)nb, h));
   const int sync_words = 1 + 2 * npan + kCqrSyncWords * npan;
@@ -387,39 +412,70 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
       RNPX_TRY(h->cqr_scratch.ensure(sizeof(double) * kCqrScratch, h));
     }
   }
+  RNPX_TRY(h->misc2.ensure(sizeof(T) * (size_t)3 * nb * nb, h));
   RNPX_TRY(launch_zero_u32((unsigned*)h->sync.p, sync_words, st));
-  const unsigned* gate[2] = {nullptr, nullptr};  // T-factor gate of the panel in each buffer
+  if (nbo > nb && k > nb) {
+    // Parts of the merged V and T no panel writes: V's rows above the second
+    // panel, T's block below the diagonal (m >= k > nb here).
+    for (int q = 0; q < (look ? 2 : 1); ++q) {
+      RNPX_TRY(launch_set<T>(nb, nb, S<T>::zero(), S<T>::zero(), (T*)vb[q]->p + (size_t)nb * ldv, ldv, st));
+      RNPX_TRY(launch_set<T>(nb, nb, S<T>::zero(), S<T>::zero(), (T*)tb[q]->p + nb, ldt, st));
+    }
+  }
 
-  auto update = [&](int p, int i, int ib, int c0, int c1) -> int {
-    // columns [c0, c1) of the trailing matrix, rows [i, m), reflectors of panel p
-    if (c1 <= c0) return RNPX_OK;
-    const int q = look ? (p & 1) : 0;
-    return apply_block_dev<T>(h, 'L', 'C', m - i, c1 - c0, ib, (const T*)vb[q]->p, m - i, (const T*)tb[q]->p, nb,
-                              A + (size_t)i + (size_t)c0 * lda, lda);
-  };
-  auto tfactor = [&](int p, int ib) -> int {
-    const int q = look ? (p & 1) : 0;
+  // The panel phase of outer block b (columns [i, i + wo)) on stream ps.
+  auto panel_phase = [&](int b, hipStream_t ps, const GemmWork* gw) -> int {
+    const int i = b * nbo, wo = std::min(nbo, k - i);
+    const int q = look ? (b & 1) : 0;
+    T* Vo = (T*)vb[q]->p;
+    T* To = (T*)tb[q]->p;
     hipEvent_t ev;
-    h->prof_begin(&ev);
-    RNPX_TRY(launch_tfactor<T>((const T*)gb[q]->p, nb, ib, nb, (T*)tb[q]->p, nb, st, gate[q]));
-    h->prof_end(K_TFACTOR, ev, 0.0, 0.0);
+    h->prof_begin(&ev, ps);
+    const int ib1 = std::min(nb, wo);
+    int s = panel_step<T>(h, i / nb, npan, i, ib1, m, nb, A, lda, tau, Vo, ldv, To, ldt, ps);
+    if (s) return s;
+    double fl = (S<T>::cplx ? 4.0 : 1.0) * (2.0 * (m - i) * ib1 * ib1 - 2.0 / 3.0 * ib1 * ib1 * ib1);
+    if (wo > nb) {
+      const int i2 = i + nb, ib2 = wo - nb;
+      // panel 1's reflectors applied to panel 2's columns
+      if ((s = apply_block_dev<T>(h, 'L', 'C', m - i, ib2, ib1, Vo, ldv, To, ldt, A + (size_t)i + (size_t)i2 * lda,
+                                  lda, gw)))
+        return s;
+      T* V2 = Vo + (size_t)nb + (size_t)nb * ldv;
+      T* T2 = To + (size_t)nb + (size_t)nb * ldt;
+      if ((s = panel_step<T>(h, i2 / nb, npan, i2, ib2, m, nb, A, lda, tau, V2, ldv, T2, ldt, ps))) return s;
+      // T12 = -T1 (V1^H V2) T2, V1^H V2 over the rows of V2
+      const T one = S<T>::one(), zero = S<T>::zero(), mone = S<T>::zero() - S<T>::one();
+      T* Sm = (T*)h->misc2.p;
+      T* Xm = Sm + (size_t)nb * nb;
+      const int rows2 = m - i2;
+      const int ks = std::max(1, std::min(32, rows2 / 256));
+      RNPX_TRY(gw->part->ensure(sizeof(T) * (size_t)ks * nb * nb, h));
+      RNPX_TRY(gemm<T>('C', 'N', ib1, ib2, rows2, one, Vo + nb, ldv, V2, ldv, zero, Sm, nb, ks, (T*)gw->part->p, ps));
+      RNPX_TRY(gemm<T>('N', 'N', ib1, ib2, ib1, one, To, ldt, Sm, nb, zero, Xm, nb, 1, nullptr, ps));
+      RNPX_TRY(gemm<T>('N', 'N', ib1, ib2, ib2, mone, Xm, nb, T2, ldt, zero, To + (size_t)nb * ldt, ldt, 1, nullptr,
+                       ps));
+      fl += (S<T>::cplx ? 4.0 : 1.0) * (2.0 * rows2 * ib2 * ib2 - 2.0 / 3.0 * ib2 * ib2 * ib2);
+    }
+    h->prof_end(K_PANEL, ev, fl, sizeof(T) * 3.0 * (m - i) * wo, ps);
     return RNPX_OK;
   };
-  auto panel = [&](int p, int i, int ib, hipStream_t ps) -> int {
-    const int q = look ? (p & 1) : 0;
-    return launch_panel_step<T>(h, p, npan, i, ib, m, nb, A, lda, tau, (T*)vb[q]->p, (T*)gb[q]->p, (T*)tb[q]->p,
-                                ps, &gate[q]);
+  // Trailing update by outer block b of columns [c0, c1) on the main stream.
+  auto update = [&](int b, int c0, int c1) -> int {
+    if (c1 <= c0) return RNPX_OK;
+    const int i = b * nbo, wo = std::min(nbo, k - i);
+    const int q = look ? (b & 1) : 0;
+    return apply_block_dev<T>(h, 'L', 'C', m - i, c1 - c0, wo, (const T*)vb[q]->p, ldv, (const T*)tb[q]->p, ldt,
+                              A + (size_t)i + (size_t)c0 * lda, lda);
   };
 
+  const GemmWork main_work = {st, &h->wbuf, &h->zbuf, &h->part, false};
   if (!look) {
-    for (int p = 0, i = 0; i < k; i += nb, ++p) {
-      const int ib = std::min(nb, k - i);
-      int s = panel(p, i, ib, st);
+    for (int b = 0; b < nob; ++b) {
+      int s = panel_phase(b, st, &main_work);
       if (s) return s;
-      if (i + ib < n) {
-        if ((s = tfactor(p, ib))) return s;
-        if ((s = update(p, i, ib, i + ib, n))) return s;
-      }
+      const int c0 = b * nbo + std::min(nbo, k - b * nbo);
+      if ((s = update(b, c0, n))) return s;
     }
     return RNPX_OK;
   }
@@ -427,30 +483,25 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
   int s = h->ensure_pipeline();
   if (s) return s;
   hipStream_t ps = h->pstream;
+  const GemmWork panel_work = {ps, &h->wbuf2, &h->zbuf2, &h->part2, false};
   RNPX_TRY(hipEventRecord(h->ev_start, st));
   RNPX_TRY(hipStreamWaitEvent(ps, h->ev_start, 0));
-  {
-    const int ib0 = std::min(nb, k);
-    if ((s = panel(0, 0, ib0, ps))) return s;
-    RNPX_TRY(hipEventRecord(h->ev_pan[0], ps));
-  }
-  for (int p = 0, i = 0; i < k; i += nb, ++p) {
-    const int ib = std::min(nb, k - i);
-    RNPX_TRY(hipStreamWaitEvent(st, h->ev_pan[p & 1], 0));
-    if (i + ib >= n) break;
-    if ((s = tfactor(p, ib))) return s;
-    const int i1 = i + ib;  // next panel's first column
-    if (i1 < k) {
-      const int ib1 = std::min(nb, k - i1);
-      if ((s = update(p, i, ib, i1, i1 + ib1))) return s;
+  if ((s = panel_phase(0, ps, &panel_work))) return s;
+  RNPX_TRY(hipEventRecord(h->ev_pan[0], ps));
+  for (int b = 0; b < nob; ++b) {
+    RNPX_TRY(hipStreamWaitEvent(st, h->ev_pan[b & 1], 0));
+    const int c0 = b * nbo + std::min(nbo, k - b * nbo);  // first column after block b
+    if (c0 >= n) break;
+    if (b + 1 < nob) {
+      const int wn = std::min(nbo, k - c0);  // block b+1's columns
+      if ((s = update(b, c0, c0 + wn))) return s;
       RNPX_TRY(hipEventRecord(h->ev_next, st));
       RNPX_TRY(hipStreamWaitEvent(ps, h->ev_next, 0));
-      const int q1 = (p + 1) & 1;
-      if ((s = panel(p + 1, i1, ib1, ps))) return s;
-      RNPX_TRY(hipEventRecord(h->ev_pan[q1], ps));
-      if ((s = update(p, i, ib, i1 + ib1, n))) return s;
+      if ((s = panel_phase(b + 1, ps, &panel_work))) return s;
+      RNPX_TRY(hipEventRecord(h->ev_pan[(b + 1) & 1], ps));
+      if ((s = update(b, c0 + wn, n))) return s;
     } else {
-      if ((s = update(p, i, ib, i1, n))) return s;
+      if ((s = update(b, c0, n))) return s;
     }
   }
   return RNPX_OK;
