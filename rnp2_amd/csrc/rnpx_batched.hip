@@ -55,7 +55,10 @@ __device__ inline void team_sync() {
 
 // Factor one m x n matrix held at W (leading dimension ld) in place, tau out.
 // Team of NT threads (threadIdx.x < NT).  sh: 8 doubles of LDS scratch.
-template <typename T, int NT>
+// WAVECOL: the Householder application gives each wave one column at a time,
+// its lanes splitting the rows (coalesced, for tall panels); otherwise each
+// lane owns whole columns (no cross-lane reduction, for short matrices).
+template <typename T, int NT, bool WAVECOL = false>
 __device__ void team_factor(int m, int n, T* W, int ld, T* tau, double* sh) {
   const int t = threadIdx.x;
   const int k = min(m, n);
@@ -107,7 +110,7 @@ __device__ void team_factor(int m, int n, T* W, int ld, T* tau, double* sh) {
     }
     const T tauc = S<T>::make((beta - sar) / beta, (-sai) / beta);
     const T inv = recip_t(S<T>::make(sar - beta, sai));
-    if (escale) beta *= safmin * safmin;
+    if (escale) beta = (beta * safmin) * safmin;  // safmin * safmin alone underflows to 0
     for (int r = c + 1 + t; r < m; r += NT) {
       T x = W[r + c * ld];
       if (escale) x = S<T>::make(ldexp(S<T>::re(x), escale), ldexp(S<T>::im(x), escale));
@@ -116,13 +119,26 @@ __device__ void team_factor(int m, int n, T* W, int ld, T* tau, double* sh) {
     team_sync<NT>();
     // apply H^H = I - conj(tau) v v^H to columns c+1..n-1 (column per lane)
     const T ctau = S<T>::conj(tauc);
-    for (int j = c + 1 + t; j < n; j += NT) {
-      T* col = W + j * ld;
-      T w = col[c];
-      for (int r = c + 1; r < m; ++r) w = w + S<T>::cmul(W[r + c * ld], col[r]);
-      const T cw = ctau * w;
-      col[c] = col[c] - cw;
-      for (int r = c + 1; r < m; ++r) col[r] = col[r] - W[r + c * ld] * cw;
+    if constexpr (WAVECOL) {
+      const int wave = t >> 6, lane = t & 63;
+      for (int j = c + 1 + wave; j < n; j += NT / 64) {
+        T* col = W + j * ld;
+        T p = S<T>::zero();
+        for (int r = c + 1 + lane; r < m; r += 64) p = p + S<T>::cmul(W[r + c * ld], col[r]);
+        const T w = col[c] + S<T>::make(wave_sum(S<T>::re(p)), wave_sum(S<T>::im(p)));
+        const T cw = ctau * w;
+        if (lane == 0) col[c] = col[c] - cw;
+        for (int r = c + 1 + lane; r < m; r += 64) col[r] = col[r] - W[r + c * ld] * cw;
+      }
+    } else {
+      for (int j = c + 1 + t; j < n; j += NT) {
+        T* col = W + j * ld;
+        T w = col[c];
+        for (int r = c + 1; r < m; ++r) w = w + S<T>::cmul(W[r + c * ld], col[r]);
+        const T cw = ctau * w;
+        col[c] = col[c] - cw;
+        for (int r = c + 1; r < m; ++r) col[r] = col[r] - W[r + c * ld] * cw;
+      }
     }
     if (t == 0) {
       tau[c] = tauc;
@@ -209,6 +225,65 @@ __global__ __launch_bounds__(256) void batched_large_kernel(int count, const int
     if (d.nrhs > 0 && B && d.m >= d.n) team_solve<T, 256>(d.m, d.n, a, d.lda, tau + d.tau_off, d.nrhs, B + d.b_off, d.ldb);
     __syncthreads();
   }
+}
+
+// Fallback for a real 64-column panel the CholeskyQR2 kernel rejected: one
+// workgroup, launched behind it and gated on its fail flag (it returns at once
+// in the common case, so it costs one workgroup slot, not a grid).  When the
+// flag is set it factors the mp x ib panel column by column in place with
+// team_factor (Reflector::Generate's rules, RNP/LA/Reflector.hpp:117-158, and
+// the Householder application of QR::Factor_unblocked, QR.hpp:78-102), writes
+// the clean V, and forms T from V's Gram matrix (GenerateBlockTr("F","C"),
+// Reflector.hpp:580-626): T(c,c) = tau_c, T(0:c,c) = -tau_c T(0:c,0:c)
+// V(:,0:c)^T v_c.  Slow (column-serial over global memory) but exact.
+__global__ __launch_bounds__(256) void panel_fallback_kernel(int mp, int ib, double* A, long lda, double* tau,
+                                                             double* V, long ldv, double* Tm, long ldt,
+                                                             const unsigned* run_if) {
+  if (*run_if == 0u) return;
+  __shared__ double sh[8];
+  __shared__ double Gs[64][65];
+  __shared__ double Ts[64][65];
+  const int t = threadIdx.x;
+  team_factor<double, 256, true>(mp, ib, A, (int)lda, tau, sh);
+  __syncthreads();
+  // clean V
+  for (long e = t; e < (long)mp * ib; e += 256) {
+    const int r = (int)(e % mp), c = (int)(e / mp);
+    V[(size_t)r + (size_t)c * ldv] = (r > c) ? A[(size_t)r + (size_t)c * lda] : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  // Gram of V: G(l,c) = V(:,l)^T V(:,c), l < c
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int l = e / 64, c = e % 64;
+    double g = 0.0;
+    if (l < c && c < ib) {
+      for (int r = c; r < mp; ++r) g = fma(V[(size_t)r + (size_t)l * ldv], V[(size_t)r + (size_t)c * ldv], g);
+    }
+    Gs[l][c] = g;
+    Ts[l][c] = 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < ib; ++c) {
+    const double tc = tau[c];
+    if (t < c) {
+      double s = 0.0;
+      for (int l = t; l < c; ++l) s = fma(Ts[t][l], Gs[l][c], s);
+      Ts[t][c] = -tc * s;
+    }
+    if (t == c) Ts[c][c] = tc;
+    __syncthreads();
+  }
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int i = e % 64, j = e / 64;
+    Tm[(size_t)i + (size_t)j * ldt] = (i < ib && j < ib) ? Ts[i][j] : 0.0;
+  }
+}
+
+hipError_t launch_panel_fallback(int mp, int ib, double* A, long lda, double* tau, double* V, long ldv, double* Tm,
+                                 long ldt, const unsigned* run_if, hipStream_t st) {
+  if (mp < 1 || ib < 1 || ib > 64 || lda < mp || ldv < mp || ldt < 64 || !run_if) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(panel_fallback_kernel, dim3(1), dim3(256), 0, st, mp, ib, A, lda, tau, V, ldv, Tm, ldt, run_if);
+  return hipGetLastError();
 }
 
 template <typename T>

@@ -48,6 +48,8 @@
 #include "rnpx_kernels.hpp"
 #include "rnpx_small64.hpp"
 
+#include <cstdlib>
+
 namespace rnpx {
 
 namespace {
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(256) void cqr_panel_kernel(CqrArgs a) {
   __shared__ double rowbuf[NB];
   __shared__ double red4[4];
   __shared__ double gdiag[NB];
-  __shared__ double tmpb[4 * 16 * small64::TS];
+  __shared__ double tmpb[small64::kTmpDoubles];
   __shared__ unsigned slot;
   __shared__ int okf;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -447,9 +449,13 @@ __global__ __launch_bounds__(256) void cqr_panel_kernel(CqrArgs a) {
 int cqr_panel_groups(int mp, int max_groups, int* rw) {
   if (mp < 2 * NB) return 0;
   if (max_groups > kCqrMaxGroups) max_groups = kCqrMaxGroups;
-  for (int r : {64, 128}) {
+  static const int first_rw = [] {
+    const char* e = getenv("RNPX_CQR_RW");  // 128: fewer, larger workgroups
+    return (e && atoi(e) == 128) ? 128 : 64;
+  }();
+  for (int r : {first_rw, 64, 128}) {
     const int G = (mp + r - 1) / r;
-    if (G <= max_groups) {
+    if (G >= 2 && G <= max_groups) {
       *rw = r;
       return G;
     }

@@ -85,6 +85,12 @@ constexpr int kCqrMaxGroups = 256;
 int cqr_panel_groups(int mp, int max_groups, int* rw);
 hipError_t launch_cqr_panel(const CqrArgs& a, int G, int rw, hipStream_t st);
 
+// One workgroup, gated on run_if: factors a real mp x ib (ib <= 64) panel the
+// CholeskyQR2 kernel rejected, column by column in place, and writes its
+// clean V and its T (64 x 64 block, zeros beyond ib).
+hipError_t launch_panel_fallback(int mp, int ib, double* A, long lda, double* tau, double* V, long ldv, double* Tm,
+                                 long ldt, const unsigned* run_if, hipStream_t st);
+
 // T factor of a forward, columnwise block reflector from its Gram matrix
 // (the larft analogue, RNP/LA/Reflector.hpp:573-626): k <= NB reflectors,
 // gram(j,c) = V(:,j)^H V(:,c) for j < c and tau_c on the diagonal.

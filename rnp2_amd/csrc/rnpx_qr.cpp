@@ -309,14 +309,14 @@ static bool use_cqr(const Handle* h, int mp, int ib, int nb, int* g, int* rw) {
 // holds, per inner panel p: [1 + p] the column-wise kernel's barrier
 // counter, [1 + npan + p] the CholeskyQR2 kernel's fail flag, and
 // kCqrSyncWords words at 1 + 2 npan + kCqrSyncWords p for its reductions.
-// Where the CholeskyQR2 kernel applies it runs first; the column-wise panel
-// and the T-factor kernel behind it are gated on its fail flag.
+// Where the CholeskyQR2 kernel applies it writes V and T itself; a panel it
+// rejects (fail flag set) is redone behind it by one gated workgroup, which
+// returns at once otherwise and so takes no grid from a concurrent update.
 template <typename T>
 static int panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, T* A, long lda, T* tau, T* V, long ldv,
                       T* Tm, long ldt, hipStream_t st) {
   const int mp = m - i;
   unsigned* sync = (unsigned*)h->sync.p;
-  const unsigned* gated = nullptr;
   int G = 0, rw = 0;
   if (use_cqr<T>(h, mp, ib, nb, &G, &rw)) {
     if constexpr (!S<T>::cplx) {
@@ -337,7 +337,7 @@ static int panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, 
       ca.abortf = (unsigned*)h->flag.p;
       ca.fail = sync + 1 + npan + p;
       RNPX_TRY(launch_cqr_panel(ca, G, rw, st));
-      gated = ca.fail;
+      return hip_status(launch_panel_fallback(mp, ib, ca.A, lda, ca.tau, V, ldv, Tm, ldt, ca.fail, st));
     }
   }
   PanelArgs<T> pa;
@@ -353,10 +353,10 @@ static int panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, 
   pa.rowbuf = (T*)h->rowbuf.p;
   pa.counter = sync + 1 + p;
   pa.abortf = (unsigned*)h->flag.p;
-  pa.run_if = gated;
+  pa.run_if = nullptr;
   const int rpt = pick_rpt(h, mp, nb, S<T>::cplx);
   RNPX_TRY(launch_panel<T>(pa, nb, rpt, st));
-  RNPX_TRY(launch_tfactor<T>((const T*)h->gram.p, nb, ib, nb, Tm, (int)ldt, st, gated));
+  RNPX_TRY(launch_tfactor<T>((const T*)h->gram.p, nb, ib, nb, Tm, (int)ldt, st));
   return RNPX_OK;
 }
 
@@ -469,6 +469,17 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
                               A + (size_t)i + (size_t)c0 * lda, lda);
   };
 
+  static const bool report_fallback = getenv("RNPX_REPORT_FALLBACK") != nullptr;
+  auto report = [&]() -> int {
+    if (!report_fallback) return RNPX_OK;
+    std::vector<unsigned> f(npan);
+    RNPX_TRY(hipStreamSynchronize(st));
+    RNPX_TRY(hipMemcpy(f.data(), (unsigned*)h->sync.p + 1 + npan, sizeof(unsigned) * npan, hipMemcpyDeviceToHost));
+    int nf = 0;
+    for (unsigned x : f) nf += (x != 0);
+    fprintf(stderr, "rnpx: geqrf %dx%d: %d of %d panels fell back to the column-wise kernel\n", m, n, nf, npan);
+    return RNPX_OK;
+  };
   const GemmWork main_work = {st, &h->wbuf, &h->zbuf, &h->part, false};
   if (!look) {
     for (int b = 0; b < nob; ++b) {
@@ -477,7 +488,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
       const int c0 = b * nbo + std::min(nbo, k - b * nbo);
       if ((s = update(b, c0, n))) return s;
     }
-    return RNPX_OK;
+    return report();
   }
 
   int s = h->ensure_pipeline();
@@ -504,7 +515,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
       if ((s = update(b, c0, n))) return s;
     }
   }
-  return RNPX_OK;
+  return report();
 }
 
 // ---------------------------------------------------------------------------

@@ -13,6 +13,8 @@ namespace rnpx {
 namespace small64 {
 
 constexpr int TS = 17;  // row stride of a 16x16 scratch block
+// tmp arguments below hold 4 scratch blocks (16 x TS each) and 64 more doubles.
+constexpr int kTmpDoubles = 4 * 16 * TS + 64;
 
 // 1/x and 1/sqrt(x) from the hardware approximations refined by two Newton
 // steps (a few ulp; x normal).  They keep the division and square root off
@@ -90,20 +92,20 @@ __device__ __forceinline__ bool chol16(double* M, int b, const double* gdiag, in
 // (zeros below the diagonal); X is written at block (b, b) of a matrix of
 // stride SX.
 template <int S, int SX>
-__device__ __forceinline__ void inv_upper16(const double* M, int b, double* X, int xb, int lane) {
+__device__ __forceinline__ void inv_upper16(const double* M, int b, double* X, int xb, double* rinv, int lane) {
+  // Row l of X per lane; U is constant here, so its rows are read as LDS
+  // broadcasts (no v_readlane), with the diagonal reciprocals precomputed.
   const int l = lane & 15;
-  double u[16], acc[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) u[k] = M[(16 * b + k) * S + 16 * b + l];  // column l
-  const double rd = rcp_nr(M[(16 * b + l) * S + 16 * b + l]);            // 1 / U_ll
+  if (lane < 16) rinv[l] = rcp_nr(M[(16 * b + l) * S + 16 * b + l]);
+  double acc[16];
 #pragma unroll
   for (int j = 0; j < 16; ++j) acc[j] = (j == l) ? 1.0 : 0.0;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    const double xj = acc[j] * readlane_t(rd, j);
+    const double xj = acc[j] * rinv[j];
     acc[j] = xj;
 #pragma unroll
-    for (int jp = j + 1; jp < 16; ++jp) acc[jp] = fma(-readlane_t(u[j], jp), xj, acc[jp]);
+    for (int jp = j + 1; jp < 16; ++jp) acc[jp] = fma(-M[(16 * b + j) * S + 16 * b + jp], xj, acc[jp]);
   }
   if (lane < 16) {
 #pragma unroll
@@ -211,6 +213,7 @@ __device__ __forceinline__ void inv_lower_offdiag(const double* M, double* Z, do
 template <int S>
 __device__ __forceinline__ bool chol64(double* M, double* X, double* tmp, double* gdiag, int* okf, int tid) {
   const int lane = tid & 63, wave = tid >> 6;
+  double* rv = tmp + 4 * 16 * TS;  // 16 reciprocals
   if (tid < 64) gdiag[tid] = M[tid * S + tid];
   if (tid == 0) *okf = 1;
   __syncthreads();
@@ -219,7 +222,7 @@ __device__ __forceinline__ bool chol64(double* M, double* X, double* tmp, double
   for (int b = 0; b < 4; ++b) {
     if (wave == 0) {
       ok = chol16<S>(M, b, gdiag, lane) && ok;
-      inv_upper16<S, S>(M, b, X, b, lane);
+      inv_upper16<S, S>(M, b, X, b, rv, lane);
     }
     __syncthreads();
     // R_bc = R_bb^-T G_bc for c > b
@@ -257,11 +260,12 @@ __device__ __forceinline__ bool chol64(double* M, double* X, double* tmp, double
 template <int S>
 __device__ __forceinline__ void lu64(double* M, double* dsg, double* X, double* tmp, int tid) {
   const int lane = tid & 63, wave = tid >> 6;
+  double* rv = tmp + 4 * 16 * TS;  // 16 reciprocals
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     if (wave == 0) {
       lu16<S>(M, b, dsg, lane);
-      inv_upper16<S, S>(M, b, X, b, lane);
+      inv_upper16<S, S>(M, b, X, b, rv, lane);
       inv_unit_lower16<S, TS>(M, b, tmp, 0, lane);
     }
     __syncthreads();
@@ -306,7 +310,7 @@ __device__ __forceinline__ void inv_unit_lower64(const double* M, double* Z, dou
 template <int S>
 __device__ __forceinline__ void inv_upper64(const double* M, double* X, double* tmp, int tid) {
   const int lane = tid & 63, wave = tid >> 6;
-  inv_upper16<S, S>(M, wave, X, wave, lane);
+  inv_upper16<S, S>(M, wave, X, wave, tmp + 4 * 16 * TS + 16 * wave, lane);
   __syncthreads();
   inv_upper_offdiag<S>(M, X, tmp, lane, wave);
 }

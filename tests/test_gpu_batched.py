@@ -62,6 +62,55 @@ def test_batched_factor_and_ls(cplx):
     assert worst["R"] < 1e-11 and worst["tau"] < 1e-11 and worst["x"] < 1e-9
 
 
+def _edge_small(m, n, cplx, seed):
+    """Zero, single-entry, underflowing, subnormal, overflowing and tiny columns."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((m, n))
+    if cplx:
+        a = a + 1j * rng.standard_normal((m, n))
+    a[:, 1] = 0.0
+    a[:, 2] = 0.0
+    a[2, 2] = 2.5j if cplx else 2.5
+    a[:, 3] *= 1e-300
+    a[:, 4] *= 1e-310
+    a[:, 5] *= 1e300
+    a[:, 6] *= 1e-160
+    return a
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_batched_edge_columns(cplx):
+    """The scaled-norm and rescale paths of the reflector in both tiers (LDS
+    problems and in-place 256-thread problems): R and tau elementwise."""
+    import torch
+    shapes = [(40, 20), (120, 64), (256, 200), (16, 8)]
+    print(f"[case] batched edge columns cplx={cplx}", flush=True)
+    desc, na, nt, _ = R.make_batch(shapes, 0)
+    dt = np.complex128 if cplx else np.float64
+    a = np.zeros(na, dtype=dt)
+    mats = []
+    for j, d in enumerate(desc):
+        m, n = int(d["m"]), int(d["n"])
+        x = _edge_small(m, n, cplx, 20 + j)
+        mats.append(x)
+        a[d["a_off"]:d["a_off"] + m * n] = x.T.reshape(-1)
+    A = torch.from_numpy(a.copy()).cuda()
+    tau = torch.zeros(max(nt, 1), dtype=torch.complex128 if cplx else torch.float64, device="cuda")
+    h = R.Handle()
+    h.geqrf_batched(desc, A, tau, None)
+    h.synchronize()
+    Ag, tg = A.cpu().numpy(), tau.cpu().numpy()
+    for d, x in zip(desc, mats):
+        m, n = x.shape
+        k = min(m, n)
+        af_o, tau_o = O.factor(x)
+        af_g = Ag[d["a_off"]:d["a_off"] + m * n].reshape(n, m).T
+        e_r = rel_err_colscaled(np.triu(af_g[:k]), np.triu(af_o[:k]))
+        e_t = float(np.max(np.abs(tg[d["tau_off"]:d["tau_off"] + k] - tau_o)))
+        print(f"        {(m, n)}: R {e_r:.3g} tau {e_t:.3g}", flush=True)
+        assert e_r < 1e-11 and e_t < 1e-11, (m, n, e_r, e_t)
+
+
 def test_batched_alone_vs_mix():
     """Each problem's result is bitwise the same alone and inside a batch."""
     import torch
