@@ -274,6 +274,8 @@ def main():
     ap.add_argument("--nb", type=int, default=0)
     ap.add_argument("--rpt", type=int, default=0)
     ap.add_argument("--no-lookahead", action="store_true", help="serialize panels and updates")
+    ap.add_argument("--panel-cus", type=int, default=0,
+                    help="CUs reserved for the look-ahead panel stream (0 automatic, -1 none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--legs", default="all",
@@ -291,8 +293,9 @@ def main():
     m, n = args.m, args.n
     seed = [1, 2, 3, 5]
     h = R.Handle()
-    if args.nb or args.rpt or args.no_lookahead:
-        h.set_tuning(nb=args.nb, panel_rpt=args.rpt, lookahead=-1 if args.no_lookahead else 0)
+    if args.nb or args.rpt or args.no_lookahead or args.panel_cus:
+        h.set_tuning(nb=args.nb, panel_rpt=args.rpt, lookahead=-1 if args.no_lookahead else 0,
+                     panel_cus=args.panel_cus)
     a0 = R.colmajor_empty(m, n, torch.float64)
     h.larnv(2, seed, a0)          # RNP Normal01 stream, column-major fill
     a = R.colmajor_empty(m, n, torch.float64)

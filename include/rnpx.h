@@ -64,6 +64,8 @@ typedef struct rnpx_tuning {
 	int lookahead;    /* 1 = overlap the next panel with the trailing update */
 	int panel;        /* real nb = 64 panels: 0 automatic, 1 column by column only,
 	                     2 CholeskyQR2 + Householder reconstruction where it applies */
+	int panel_cus;    /* look-ahead: CUs reserved for the panel stream, the trailing
+	                     update runs on the others (0 automatic, -1 no partition) */
 } rnpx_tuning;
 
 /* ---------------------------------------------------------------------

@@ -45,7 +45,7 @@ class ZComplex(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     """rnpx_tuning: the QR::Tuning<T> hook (QR.hpp:41-51)."""
-    _fields_ = [("nb", _i), ("panel_rpt", _i), ("ksplit", _i), ("lookahead", _i), ("panel", _i)]
+    _fields_ = [("nb", _i), ("panel_rpt", _i), ("ksplit", _i), ("lookahead", _i), ("panel", _i), ("panel_cus", _i)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -332,10 +332,12 @@ class Handle:
     def stream(self):
         return lib().rnpx_get_stream(self.h)
 
-    def set_tuning(self, nb=0, panel_rpt=0, ksplit=0, lookahead=0, panel=0):
+    def set_tuning(self, nb=0, panel_rpt=0, ksplit=0, lookahead=0, panel=0, panel_cus=0):
         """panel: 0 automatic, 1 column-by-column reflectors only, 2 CholeskyQR2 +
-        Householder reconstruction for real 64-column panels where it applies."""
-        t = Tuning(nb, panel_rpt, ksplit, lookahead, panel)
+        Householder reconstruction for real 64-column panels where it applies.
+        panel_cus: CUs reserved for the look-ahead panel stream (0 automatic,
+        -1 no partition)."""
+        t = Tuning(nb, panel_rpt, ksplit, lookahead, panel, panel_cus)
         _check(lib().rnpx_set_tuning(self.h, ctypes.byref(t)), "rnpx_set_tuning")
 
     KERNELS = {"update": 0, "wgemm": 1, "panel": 2, "tfactor": 3}

@@ -20,7 +20,7 @@ ARCH = "gfx950"
 
 SOURCES = [
     "rnpx_panel.hip",
-    "rnpx_cqr.hip",
+    "rnpx_cqr.hip", "rnpx_trsm.hip",
     "rnpx_tfactor.hip",
     "rnpx_gemm.hip",
     "rnpx_aux.hip",

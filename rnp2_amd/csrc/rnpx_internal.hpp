@@ -42,10 +42,25 @@ struct Handle {
   hipStream_t pstream = nullptr;  // panel stream of the look-ahead pipeline
   hipStream_t ext_pstream = nullptr;  // shared panel stream supplied by the owner (not destroyed)
   hipEvent_t ev_pan[2] = {nullptr, nullptr}, ev_next = nullptr, ev_start = nullptr;
+  // CU-partitioned look-ahead: the panel stream and the update stream run on
+  // disjoint sets of CUs (stream CU masks), so a panel never queues behind
+  // the trailing update's workgroups.  part_cus = CUs of the panel set, 0 when
+  // the partition is off or could not be created.
+  hipStream_t part_ps = nullptr, part_us = nullptr;
+  int part_cus = 0, part_want = -1;
+  int panel_cap = 0;  // while > 0: the most workgroups a panel kernel may use
   Buffer stage1, stage2, stage3;  // host-argument staging for the drop-in calls
   Buffer lsbuf, lsnrm;            // least-squares refinement scratch
   Buffer cqr_part, cqr_gsum, cqr_scratch;  // CholeskyQR2 panel scratch
   Buffer wbuf2, zbuf2, part2, misc2;       // GEMM scratch of the panel stream
+  Buffer trsm_flags;                       // per-block flags of the blocked triangular solve
+  unsigned trsm_epoch = 0;                 // value the flags take in the current solve
+  // Every buffer above, for accounting and release.
+  std::vector<Buffer*> buffers() {
+    return {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc,
+            &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
+            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags};
+  }
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
   int update_prec = 0;                 // 1: trailing-update products on FP32 MFMA
@@ -57,6 +72,7 @@ struct Handle {
   int prof_begin(hipEvent_t* a, hipStream_t s = nullptr);
   void prof_end(int kid, hipEvent_t a, double flops, double bytes, hipStream_t s = nullptr);
   int ensure_pipeline();
+  int ensure_partition(int cus);
   size_t held_bytes() const;
   static int check_gpu();
   ~Handle();
@@ -69,7 +85,10 @@ int sync_handle(Handle* h);
 int prof_collect(Handle* h, int kid, long* launches, double* ms, double* flops, double* bytes);
 void prof_reset(Handle* h);
 
-template <typename T> int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau);
+// B (m x nrhs, optional): Q^H B is formed alongside, each block reflector
+// applied to B right after its trailing update (the least-squares drivers).
+template <typename T>
+int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B = nullptr, int nrhs = 0, long ldb = 0);
 template <typename T> int gels_dev(Handle* h, int m, int n, int nrhs, T* A, long lda, T* tau, T* B, long ldb);
 int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb, double* X,
                    long ldx, int max_iter, double tol, int* iters, double* berr);

@@ -140,6 +140,14 @@ template <typename T>
 hipError_t launch_larnv(int dist, const int iseed[4], long long count, T* x, hipStream_t st);
 
 // R X = B, R upper triangular n x n, B's first n rows (n x nrhs) in place.
+// Blocked triangular solve with upper R, in place on B (rnpx_trsm.hip):
+// trans 'N' R X = B, 'C' R^H X = B.  flags: trsm_flag_words(n, nrhs) words
+// that never hold `epoch` at launch (epoch != 0, a new value per launch).
+int trsm_flag_words(int n, int nrhs);
+template <typename T>
+hipError_t launch_trsm_blocked(char trans, int n, int nrhs, const T* R, long ldr, T* B, long ldb, unsigned* flags,
+                               unsigned epoch, unsigned* abortf, hipStream_t st);
+
 template <typename T>
 hipError_t launch_trsm_un(int n, int nrhs, const T* R, long ldr, T* B, long ldb, hipStream_t st);
 

@@ -24,6 +24,26 @@ namespace rnpx {
     if (_e != hipSuccess) return hip_status(_e);  \
   } while (0)
 
+// Triangular solve with the upper triangle R of A (n x n): trans 'N'
+// R X = B, 'C' R^H X = B, in place on B.  Each call takes a new flag epoch,
+// so a solve is not replayable from a captured graph.
+template <typename T>
+static int trsm_dev(Handle* h, char trans, int n, int nrhs, const T* R, long ldr, T* B, long ldb) {
+  if (n == 0 || nrhs == 0) return RNPX_OK;
+  const size_t words = (size_t)trsm_flag_words(n, nrhs);
+  const size_t before = h->trsm_flags.bytes;
+  RNPX_TRY(h->trsm_flags.ensure(sizeof(unsigned) * words, h));
+  if (h->trsm_flags.bytes != before || h->trsm_epoch == 0xFFFFFFFFu) {
+    // new (uninitialised) or exhausted flags: clear them with a kernel
+    RNPX_TRY(launch_zero_u32((unsigned*)h->trsm_flags.p, (int)(h->trsm_flags.bytes / sizeof(unsigned)), h->stream));
+    h->trsm_epoch = 0;
+  }
+  ++h->trsm_epoch;
+  RNPX_TRY(launch_trsm_blocked<T>(trans, n, nrhs, R, ldr, B, ldb, (unsigned*)h->trsm_flags.p, h->trsm_epoch,
+                                  (unsigned*)h->flag.p, h->stream));
+  return RNPX_OK;
+}
+
 template <typename T>
 int gels_dev(Handle* h, int m, int n, int nrhs, T* A, long lda, T* tau, T* B, long ldb) {
   if (m < 0) return -2;
@@ -32,12 +52,11 @@ int gels_dev(Handle* h, int m, int n, int nrhs, T* A, long lda, T* tau, T* B, lo
   if (lda < std::max(1, m)) return -6;
   if (ldb < std::max(1, m)) return -9;
   if (n == 0 || nrhs == 0) return RNPX_OK;
-  int s = geqrf_dev<T>(h, m, n, A, lda, tau);
+  // Q^H B is formed inside the factorization (each block reflector applied
+  // to B as it is produced), then R X = (Q^H B)(0:n, :).
+  int s = geqrf_dev<T>(h, m, n, A, lda, tau, B, nrhs, ldb);
   if (s) return s;
-  s = ormqr_dev<T>(h, 'L', 'C', m, nrhs, n, A, lda, tau, B, ldb);
-  if (s) return s;
-  RNPX_TRY(launch_trsm_un<T>(n, nrhs, A, lda, B, ldb, h->stream));
-  return RNPX_OK;
+  return trsm_dev<T>(h, 'N', n, nrhs, A, lda, B, ldb);
 }
 
 template int gels_dev<double>(Handle*, int, int, int, double*, long, double*, double*, long);
@@ -133,14 +152,13 @@ int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda,
   double* tau = Af + (size_t)m * n;
   double* Bq = tau + n;
   RNPX_TRY(launch_copy<double>(m, n, A, lda, Af, m, st));
+  // x0 = R^{-1} (Q^T b)(0:n), Q^T b formed inside the factorization
+  RNPX_TRY(launch_copy<double>(m, nrhs, B, ldb, Bq, m, st));
   h->update_prec = 1;
-  int s = geqrf_dev<double>(h, m, n, Af, m, tau);
+  int s = geqrf_dev<double>(h, m, n, Af, m, tau, Bq, nrhs, m);
   h->update_prec = 0;
   if (s) return s;
-  // x0 = R^{-1} (Q^T b)(0:n)
-  RNPX_TRY(launch_copy<double>(m, nrhs, B, ldb, Bq, m, st));
-  if ((s = ormqr_dev<double>(h, 'L', 'C', m, nrhs, n, Af, m, tau, Bq, m))) return s;
-  RNPX_TRY(launch_trsm_un<double>(n, nrhs, Af, m, Bq, m, st));
+  if ((s = trsm_dev<double>(h, 'N', n, nrhs, Af, m, Bq, m))) return s;
   RNPX_TRY(launch_copy<double>(n, nrhs, Bq, m, X, ldx, st));
   LsScratch sc;
   if ((s = ls_scratch(h, m, n, nrhs, &sc))) return s;
@@ -151,8 +169,8 @@ int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda,
     if ((s = ls_residual(h, m, n, nrhs, A, lda, B, ldb, X, ldx, sc, &a2, &eta))) return s;
     if (it >= max_iter) break;
     // d = R^{-1} R^{-T} s  (s is in sc.S, n x nrhs), x += d
-    RNPX_TRY(launch_trsm_uc<double>(n, nrhs, Af, m, sc.S, n, st));
-    RNPX_TRY(launch_trsm_un<double>(n, nrhs, Af, m, sc.S, n, st));
+    if ((s = trsm_dev<double>(h, 'C', n, nrhs, Af, m, sc.S, n))) return s;
+    if ((s = trsm_dev<double>(h, 'N', n, nrhs, Af, m, sc.S, n))) return s;
     RNPX_TRY(launch_axpy_mat(n, nrhs, sc.S, n, X, ldx, st));
     RNPX_TRY(launch_colnorm2(n, nrhs, sc.S, n, sc.nrm, st));
     RNPX_TRY(launch_colnorm2(n, nrhs, X, ldx, sc.nrm + nrhs, st));

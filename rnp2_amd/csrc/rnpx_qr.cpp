@@ -42,9 +42,7 @@ void Buffer::release() {
 
 size_t Handle::held_bytes() const {
   size_t s = 0;
-  for (const Buffer* b : {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc, &stage1, &stage2,
-                          &stage3, &vbuf2, &gram2, &tbuf2, &lsbuf, &lsnrm})
-    s += b->bytes;
+  for (const Buffer* b : const_cast<Handle*>(this)->buffers()) s += b->bytes;
   return s;
 }
 
@@ -88,6 +86,43 @@ int Handle::ensure_pipeline() {
   return RNPX_OK;
 }
 
+// Two streams on complementary CU masks: the first `cus` CUs for the panel
+// stream, the rest for the update stream.  Returns the panel set's size, or 0
+// when the runtime refuses masks (the caller then uses plain streams).
+int Handle::ensure_partition(int cus) {
+  if (part_want == cus) return part_cus;
+  for (hipStream_t* s : {&part_ps, &part_us}) {
+    if (*s) {
+      hipStreamSynchronize(*s);
+      hipStreamDestroy(*s);
+      *s = nullptr;
+    }
+  }
+  part_want = cus;
+  part_cus = 0;
+  if (cus <= 0 || cus >= num_cus) return 0;
+  const int words = (num_cus + 31) / 32;
+  std::vector<uint32_t> mp(words, 0u), mu(words, 0u);
+  for (int c = 0; c < num_cus; ++c) (c < cus ? mp : mu)[c / 32] |= 1u << (c % 32);
+  hipError_t e1 = hipExtStreamCreateWithCUMask(&part_ps, (uint32_t)words, mp.data());
+  if (e1 != hipSuccess) {
+    if (getenv("RNPX_REPORT_FALLBACK")) fprintf(stderr, "rnpx: CU mask refused: %s\n", hipGetErrorString(e1));
+    part_ps = nullptr;
+    (void)hipGetLastError();
+    return 0;
+  }
+  if (hipExtStreamCreateWithCUMask(&part_us, (uint32_t)words, mu.data()) != hipSuccess) {
+    part_us = nullptr;
+    (void)hipGetLastError();
+    hipStreamDestroy(part_ps);
+    part_ps = nullptr;
+    return 0;
+  }
+  part_cus = cus;
+  if (getenv("RNPX_REPORT_FALLBACK")) fprintf(stderr, "rnpx: CU partition %d + %d of %d\n", cus, num_cus - cus, num_cus);
+  return part_cus;
+}
+
 Handle::~Handle() {
   if (stream) hipStreamSynchronize(stream);
   if (batch && batch_free) batch_free(batch);
@@ -100,9 +135,13 @@ Handle::~Handle() {
   for (hipEvent_t ev : {ev_pan[0], ev_pan[1], ev_next, ev_start})
     if (ev) hipEventDestroy(ev);
   if (pstream && pstream != ext_pstream) hipStreamDestroy(pstream);
-  for (Buffer* b : {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc, &stage1, &stage2,
-                          &stage3, &vbuf2, &gram2, &tbuf2, &lsbuf, &lsnrm})
-    b->release();
+  for (hipStream_t ps2 : {part_ps, part_us}) {
+    if (ps2) {
+      hipStreamSynchronize(ps2);
+      hipStreamDestroy(ps2);
+    }
+  }
+  for (Buffer* b : buffers()) b->release();
   if (own_stream && own) hipStreamDestroy(own);
 }
 
@@ -298,7 +337,7 @@ static bool use_cqr(const Handle* h, int mp, int ib, int nb, int* g, int* rw) {
     return false;
   } else {
     if (h->tune.panel == 1 || nb != 64 || ib != 64) return false;
-    *g = cqr_panel_groups(mp, std::min(kCqrMaxGroups, h->num_cus), rw);
+    *g = cqr_panel_groups(mp, std::min(kCqrMaxGroups, h->panel_cap > 0 ? h->panel_cap : h->num_cus), rw);
     return *g > 0;
   }
 }
@@ -374,10 +413,12 @@ static int panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, 
 // stream is released) and the rest.  V and T are double-buffered between
 // consecutive outer blocks; V uses leading dimension m.
 template <typename T>
-int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
+int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, long ldb) {
   if (m < 0) return -2;
   if (n < 0) return -3;
   if (lda < std::max(1, m)) return -5;
+  if (!B) nrhs = 0;
+  if (nrhs > 0 && ldb < std::max(1, m)) return -9;
   const int k = std::min(m, n);
   if (k == 0) return RNPX_OK;
   hipStream_t st = h->stream;
@@ -460,15 +501,24 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
     h->prof_end(K_PANEL, ev, fl, sizeof(T) * 3.0 * (m - i) * wo, ps);
     return RNPX_OK;
   };
-  // Trailing update by outer block b of columns [c0, c1) on the main stream.
-  auto update = [&](int b, int c0, int c1) -> int {
+  // Trailing update by outer block b of columns [c0, c1) (main stream unless
+  // gw names another).
+  auto update = [&](int b, int c0, int c1, const GemmWork* gw) -> int {
     if (c1 <= c0) return RNPX_OK;
     const int i = b * nbo, wo = std::min(nbo, k - i);
     const int q = look ? (b & 1) : 0;
     return apply_block_dev<T>(h, 'L', 'C', m - i, c1 - c0, wo, (const T*)vb[q]->p, ldv, (const T*)tb[q]->p, ldt,
-                              A + (size_t)i + (size_t)c0 * lda, lda);
+                              A + (size_t)i + (size_t)c0 * lda, lda, gw);
   };
 
+  // Q^H B: block b's reflectors applied to B (rows [i, m)).
+  auto update_rhs = [&](int b, const GemmWork* gw) -> int {
+    if (nrhs == 0) return RNPX_OK;
+    const int i = b * nbo, wo = std::min(nbo, k - i);
+    const int q = look ? (b & 1) : 0;
+    return apply_block_dev<T>(h, 'L', 'C', m - i, nrhs, wo, (const T*)vb[q]->p, ldv, (const T*)tb[q]->p, ldt,
+                              B + (size_t)i, ldb, gw);
+  };
   static const bool report_fallback = getenv("RNPX_REPORT_FALLBACK") != nullptr;
   auto report = [&]() -> int {
     if (!report_fallback) return RNPX_OK;
@@ -486,34 +536,65 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {
       int s = panel_phase(b, st, &main_work);
       if (s) return s;
       const int c0 = b * nbo + std::min(nbo, k - b * nbo);
-      if ((s = update(b, c0, n))) return s;
+      if ((s = update(b, c0, n, nullptr))) return s;
+      if ((s = update_rhs(b, nullptr))) return s;
     }
     return report();
   }
 
   int s = h->ensure_pipeline();
   if (s) return s;
-  hipStream_t ps = h->pstream;
+  hipStream_t ps = h->pstream, us = st;
+  // CU partition: every panel kernel of this call must fit the panel set
+  // (their workgroups meet at grid-wide reductions).
+  int pcus = h->tune.panel_cus;
+  if (pcus == 0) {
+    static const int env_cus = [] {
+      const char* e = getenv("RNPX_PANEL_CUS");
+      return e ? atoi(e) : 0;
+    }();
+    pcus = env_cus;
+  }
+  if (pcus > 0 && G0 <= pcus && !h->ext_pstream && h->ensure_partition(pcus) > 0) {
+    ps = h->part_ps;
+    us = h->part_us;
+    h->panel_cap = pcus;
+  }
+  struct CapReset {
+    Handle* h;
+    ~CapReset() { h->panel_cap = 0; }
+  } cap_reset{h};
   const GemmWork panel_work = {ps, &h->wbuf2, &h->zbuf2, &h->part2, false};
+  const GemmWork upd_work = {us, &h->wbuf, &h->zbuf, &h->part, true};
   RNPX_TRY(hipEventRecord(h->ev_start, st));
   RNPX_TRY(hipStreamWaitEvent(ps, h->ev_start, 0));
+  if (us != st) RNPX_TRY(hipStreamWaitEvent(us, h->ev_start, 0));
   if ((s = panel_phase(0, ps, &panel_work))) return s;
   RNPX_TRY(hipEventRecord(h->ev_pan[0], ps));
   for (int b = 0; b < nob; ++b) {
-    RNPX_TRY(hipStreamWaitEvent(st, h->ev_pan[b & 1], 0));
+    RNPX_TRY(hipStreamWaitEvent(us, h->ev_pan[b & 1], 0));
     const int c0 = b * nbo + std::min(nbo, k - b * nbo);  // first column after block b
-    if (c0 >= n) break;
+    if (c0 >= n) {
+      if ((s = update_rhs(b, &upd_work))) return s;
+      break;
+    }
     if (b + 1 < nob) {
       const int wn = std::min(nbo, k - c0);  // block b+1's columns
-      if ((s = update(b, c0, c0 + wn))) return s;
-      RNPX_TRY(hipEventRecord(h->ev_next, st));
+      if ((s = update(b, c0, c0 + wn, &upd_work))) return s;
+      RNPX_TRY(hipEventRecord(h->ev_next, us));
       RNPX_TRY(hipStreamWaitEvent(ps, h->ev_next, 0));
       if ((s = panel_phase(b + 1, ps, &panel_work))) return s;
       RNPX_TRY(hipEventRecord(h->ev_pan[(b + 1) & 1], ps));
-      if ((s = update(b, c0 + wn, n))) return s;
+      if ((s = update(b, c0 + wn, n, &upd_work))) return s;
     } else {
-      if ((s = update(b, c0, n))) return s;
+      if ((s = update(b, c0, n, &upd_work))) return s;
     }
+    if ((s = update_rhs(b, &upd_work))) return s;
+  }
+  if (us != st) {
+    // the last panel is ordered before the update stream's last wait
+    RNPX_TRY(hipEventRecord(h->ev_next, us));
+    RNPX_TRY(hipStreamWaitEvent(st, h->ev_next, 0));
   }
   return report();
 }
@@ -670,8 +751,8 @@ int sync_handle(Handle* h) {
   return RNPX_OK;
 }
 
-template int geqrf_dev<double>(Handle*, int, int, double*, long, double*);
-template int geqrf_dev<zd>(Handle*, int, int, zd*, long, zd*);
+template int geqrf_dev<double>(Handle*, int, int, double*, long, double*, double*, int, long);
+template int geqrf_dev<zd>(Handle*, int, int, zd*, long, zd*, zd*, int, long);
 template int ormqr_dev<double>(Handle*, char, char, int, int, int, const double*, long, const double*, double*, long);
 template int ormqr_dev<zd>(Handle*, char, char, int, int, int, const zd*, long, const zd*, zd*, long);
 template int orgqr_dev<double>(Handle*, int, int, int, double*, long, const double*);
