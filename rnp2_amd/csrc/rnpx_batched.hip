@@ -18,6 +18,8 @@
 // Results do not depend on which team ran a problem, or when.
 #include "rnpx_kernels.hpp"
 
+#include <cstdlib>
+
 namespace rnpx {
 
 __device__ inline double team_sum(double v, double* sh, int nt) {
@@ -120,15 +122,38 @@ __device__ void team_factor(int m, int n, T* W, int ld, T* tau, double* sh) {
     // apply H^H = I - conj(tau) v v^H to columns c+1..n-1 (column per lane)
     const T ctau = S<T>::conj(tauc);
     if constexpr (WAVECOL) {
+      // UNR columns per wave per pass: independent chains in flight.
+      constexpr int NW = NT / 64, UNR = 4;
       const int wave = t >> 6, lane = t & 63;
-      for (int j = c + 1 + wave; j < n; j += NT / 64) {
-        T* col = W + j * ld;
-        T p = S<T>::zero();
-        for (int r = c + 1 + lane; r < m; r += 64) p = p + S<T>::cmul(W[r + c * ld], col[r]);
-        const T w = col[c] + S<T>::make(wave_sum(S<T>::re(p)), wave_sum(S<T>::im(p)));
-        const T cw = ctau * w;
-        if (lane == 0) col[c] = col[c] - cw;
-        for (int r = c + 1 + lane; r < m; r += 64) col[r] = col[r] - W[r + c * ld] * cw;
+      for (int j0 = c + 1 + wave; j0 < n; j0 += NW * UNR) {
+        T p[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) p[u] = S<T>::zero();
+        for (int r = c + 1 + lane; r < m; r += 64) {
+          const T v = W[r + c * ld];
+#pragma unroll
+          for (int u = 0; u < UNR; ++u)
+            if (j0 + u * NW < n) p[u] = p[u] + S<T>::cmul(v, W[r + (j0 + u * NW) * ld]);
+        }
+        T cw[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+          const double sr = wave_sum(S<T>::re(p[u]));
+          const double si = S<T>::cplx ? wave_sum(S<T>::im(p[u])) : 0.0;
+          cw[u] = S<T>::zero();
+          if (j0 + u * NW < n) cw[u] = ctau * (W[c + (j0 + u * NW) * ld] + S<T>::make(sr, si));
+        }
+        if (lane == 0) {
+#pragma unroll
+          for (int u = 0; u < UNR; ++u)
+            if (j0 + u * NW < n) W[c + (j0 + u * NW) * ld] = W[c + (j0 + u * NW) * ld] - cw[u];
+        }
+        for (int r = c + 1 + lane; r < m; r += 64) {
+          const T v = W[r + c * ld];
+#pragma unroll
+          for (int u = 0; u < UNR; ++u)
+            if (j0 + u * NW < n) W[r + (j0 + u * NW) * ld] = W[r + (j0 + u * NW) * ld] - v * cw[u];
+        }
       }
     } else {
       for (int j = c + 1 + t; j < n; j += NT) {
@@ -149,26 +174,42 @@ __device__ void team_factor(int m, int n, T* W, int ld, T* tau, double* sh) {
 }
 
 // Least squares for m >= n: B (m x nrhs, ldb) <- Q^H B, then R X = B(0:n,:)
-// in place (X in the first n rows).  One lane per right-hand side.
+// in place (X in the first n rows), one right-hand side at a time with the
+// team's threads splitting the rows: for each reflector a team dot product
+// and an axpy (Q^H b in the order QR::MultQ("L","C") applies H_0, H_1, ...),
+// then column-oriented back substitution (x_i = b_i / r_ii, then
+// b(0:i) -= R(0:i, i) x_i).
 template <typename T, int NT>
-__device__ void team_solve(int m, int n, const T* W, int ld, const T* tau, int nrhs, T* B, int ldb) {
+__device__ void team_solve(int m, int n, const T* W, int ld, const T* tau, int nrhs, T* B, int ldb, double* sh,
+                           T* xs) {
   const int t = threadIdx.x;
-  for (int q = t; q < nrhs; q += NT) {
+  for (int q = 0; q < nrhs; ++q) {
     T* b = B + (size_t)q * ldb;
     for (int c = 0; c < n; ++c) {
       const T tc = tau[c];
       if (S<T>::is_zero(tc)) continue;
-      T w = b[c];
-      for (int r = c + 1; r < m; ++r) w = w + S<T>::cmul(W[r + c * ld], b[r]);
+      T p = S<T>::zero();
+      for (int r = c + 1 + t; r < m; r += NT) p = p + S<T>::cmul(W[r + c * ld], b[r]);
+      double pr = team_sum(S<T>::re(p), sh, NT);
+      double pi = 0.0;
+      if constexpr (S<T>::cplx) pi = team_sum(S<T>::im(p), sh, NT);
+      const T w = b[c] + S<T>::make(pr, pi);
       const T cw = S<T>::conj(tc) * w;
-      b[c] = b[c] - cw;
-      for (int r = c + 1; r < m; ++r) b[r] = b[r] - W[r + c * ld] * cw;
+      team_sync<NT>();
+      if (t == 0) b[c] = b[c] - cw;
+      for (int r = c + 1 + t; r < m; r += NT) b[r] = b[r] - W[r + c * ld] * cw;
+      team_sync<NT>();
     }
     for (int i = n - 1; i >= 0; --i) {
-      T s = b[i];
-      for (int j = i + 1; j < n; ++j) s = s - W[i + j * ld] * b[j];
-      const T d = W[i + i * ld];
-      b[i] = s * recip_t(d);
+      if (t == 0) {
+        const T x = b[i] * recip_t(W[i + i * ld]);
+        b[i] = x;
+        *xs = x;
+      }
+      team_sync<NT>();
+      const T x = *xs;
+      for (int j = t; j < i; j += NT) b[j] = b[j] - W[j + i * ld] * x;
+      team_sync<NT>();
     }
   }
 }
@@ -179,6 +220,7 @@ __global__ __launch_bounds__(64) void batched_small_kernel(int count, const int*
                                                            unsigned* counter, int lds_elems) {
   extern __shared__ double smem[];
   __shared__ double sh[8];
+  __shared__ T xs;
   __shared__ int next;
   T* W = reinterpret_cast<T*>(smem);
   while (true) {
@@ -200,29 +242,211 @@ __global__ __launch_bounds__(64) void batched_small_kernel(int count, const int*
       const int i = e % m, j = e / m;
       a[(size_t)i + (size_t)j * d.lda] = W[i + j * ld];
     }
-    if (d.nrhs > 0 && B && m >= n) team_solve<T, 64>(m, n, W, ld, tau + d.tau_off, d.nrhs, B + d.b_off, d.ldb);
+    if (d.nrhs > 0 && B && m >= n) team_solve<T, 64>(m, n, W, ld, tau + d.tau_off, d.nrhs, B + d.b_off, d.ldb, sh, &xs);
     team_sync<64>();
   }
 }
 
+// Tier L: one 512-thread workgroup per problem.  Problems of more than one
+// panel's columns take the blocked path below (panels in LDS, block
+// reflector updates on MFMA); narrower ones that fit are staged in LDS whole
+// (column-major, ld = m: the waves read down columns); the rest (taller than
+// the blocked path's panel buffer) are factored in place in global memory.
+// The Householder application gives each wave one column at a time with the
+// lanes splitting the rows (team_factor's WAVECOL mode).
+constexpr int kLargeLds = 20352;  // doubles of LDS per workgroup (159 KiB)
+constexpr int kLargeThreads = 512;
+constexpr int kBlockedMinCols = 33;  // more than one panel: blocked (measured faster than whole-LDS)
+
+// Panel width and the most panel rows the LDS carve-up holds, per type.
+template <typename T> struct Blk;
+template <> struct Blk<double> { static constexpr int PW = 32, ROWS = 512; };
+template <> struct Blk<zd> { static constexpr int PW = 16, ROWS = 256; };
+static_assert(Blk<double>::ROWS * Blk<double>::PW + 2 * 32 * 33 <= kLargeLds, "LDS carve-up (real)");
+static_assert(2 * (Blk<zd>::ROWS * Blk<zd>::PW + 2 * 16 * 17) <= kLargeLds, "LDS carve-up (complex)");
+
+// acc += a * b on FP64 MFMA for real or complex (4 real products) operands.
+struct CAcc {
+  f64x4 re, im;
+};
+__device__ __forceinline__ void cmma(f64x4& acc, double a, double b) { acc = mfma16(a, b, acc); }
+__device__ __forceinline__ void cmma(CAcc& acc, zd a, zd b) {
+  acc.re = mfma16(a.x, b.x, acc.re);
+  acc.re = mfma16(-a.y, b.y, acc.re);
+  acc.im = mfma16(a.x, b.y, acc.im);
+  acc.im = mfma16(a.y, b.x, acc.im);
+}
+template <typename T> struct AccOf;
+template <> struct AccOf<double> {
+  using type = f64x4;
+  __device__ static inline type zero() { return f64x4{0.0, 0.0, 0.0, 0.0}; }
+  __device__ static inline double get(const type& x, int q) { return x[q]; }
+  __device__ static inline void set(type& x, int q, double v) { x[q] = v; }
+};
+template <> struct AccOf<zd> {
+  using type = CAcc;
+  __device__ static inline type zero() { return CAcc{f64x4{0.0, 0.0, 0.0, 0.0}, f64x4{0.0, 0.0, 0.0, 0.0}}; }
+  __device__ static inline zd get(const type& x, int q) { return mkz(x.re[q], x.im[q]); }
+  __device__ static inline void set(type& x, int q, zd v) {
+    x.re[q] = v.x;
+    x.im[q] = v.y;
+  }
+};
+
+// A problem too large for LDS (m <= Blk<T>::ROWS): blocked by panels of PW
+// columns.  Each panel is staged in LDS and factored there (team_factor, the
+// QR::Factor_unblocked column loop); T comes from the panel's V as in
+// GenerateBlockTr("F","C") (Reflector.hpp:580-626): T(c,c) = tau_c,
+// T(0:c,c) = -tau_c T(0:c,0:c) V(:,0:c)^H v_c; the trailing columns then take
+// the block reflector at once (ApplyBlock, Reflector.hpp:739-777:
+// W = V^H C, Z = T^H W, C -= V Z), each wave 16 columns at a time on FP64
+// MFMA with W and Z kept in its accumulators (their layout is the next
+// product's operand layout, slice for slice).  The update is issued as
+// C^T -= Z^T V^T so that the C tile's loads and stores run down columns.
 template <typename T>
-__global__ __launch_bounds__(256) void batched_large_kernel(int count, const int* __restrict__ order,
+__device__ void blocked_factor(int m, int n, T* a, long lda, T* tau, double* lds, double* sh) {
+  constexpr int NT = kLargeThreads, NW = NT / 64, PW = Blk<T>::PW, NB2 = PW / 16, TS = PW + 1;
+  using AO = AccOf<T>;
+  using Acc = typename AO::type;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  T* Pn = reinterpret_cast<T*>(lds);     // mp x PW, ld mp
+  T* Tt = Pn + Blk<T>::ROWS * PW;        // T, row-major, stride TS
+  T* Gs = Tt + PW * TS;                  // V's Gram, row-major, stride TS
+  const T zero = S<T>::zero();
+  const int k = min(m, n);
+  for (int p0 = 0; p0 < k; p0 += PW) {
+    const int pw = min(PW, k - p0), mp = m - p0;
+    T* ap = a + (size_t)p0 + (size_t)p0 * lda;
+    for (int e = t; e < mp * pw; e += NT) Pn[e] = ap[(size_t)(e % mp) + (size_t)(e / mp) * lda];
+    __syncthreads();
+    team_factor<T, NT, true>(mp, pw, Pn, mp, tau + p0, sh);
+    __syncthreads();
+    for (int e = t; e < mp * pw; e += NT) ap[(size_t)(e % mp) + (size_t)(e / mp) * lda] = Pn[e];
+    const int nc = n - p0 - pw;
+    __syncthreads();
+    if (nc <= 0) continue;
+    // Clean V (unit diagonal, zeros above; zero columns beyond pw).
+    for (int e = t; e < mp * PW; e += NT) {
+      const int r = e % mp, c = e / mp;
+      if (c >= pw || r <= c) Pn[e] = (r == c && c < pw) ? S<T>::one() : zero;
+    }
+    __syncthreads();
+    for (int e = t; e < PW * PW; e += NT) {
+      const int l = e / PW, c = e % PW;
+      T g = zero;
+      if (l < c && c < pw)
+        for (int r = c; r < mp; ++r) g = g + S<T>::cmul(Pn[r + l * mp], Pn[r + c * mp]);
+      Gs[l * TS + c] = g;
+      Tt[l * TS + c] = zero;
+    }
+    __syncthreads();
+    for (int c = 0; c < pw; ++c) {
+      const T tc = tau[p0 + c];
+      if (t < c) {
+        T sacc = zero;
+        for (int l = t; l < c; ++l) sacc = sacc + Tt[t * TS + l] * Gs[l * TS + c];
+        Tt[t * TS + c] = zero - tc * sacc;
+      }
+      if (t == c) Tt[c * TS + c] = tc;
+      __syncthreads();
+    }
+    // Trailing columns [p0 + pw, n), rows [p0, m): 16-column tiles per wave.
+    T* cp = a + (size_t)p0;  // row p0 of column 0
+    const int jbeg = p0 + pw;
+    const int ntiles = (nc + 15) / 16;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int jt = wave; jt < ntiles; jt += NW) {
+      const int j0 = jbeg + 16 * jt;
+      Acc w[NB2];
+#pragma unroll
+      for (int ib = 0; ib < NB2; ++ib) w[ib] = AO::zero();
+      {
+        const int j = j0 + li;
+        for (int ks = 0; ks < mp; ks += 4) {
+          const int r = ks + lk;
+          const T cv = (r < mp && j < n) ? cp[(size_t)r + (size_t)j * lda] : zero;
+#pragma unroll
+          for (int ib = 0; ib < NB2; ++ib) {
+            const T vv = (r < mp) ? S<T>::conj(Pn[r + (16 * ib + li) * mp]) : zero;
+            cmma(w[ib], vv, cv);  // W = V^H C: a = V^H[i][k], b = C[k][j]
+          }
+        }
+      }
+      Acc z[NB2];
+#pragma unroll
+      for (int ib = 0; ib < NB2; ++ib) {
+        z[ib] = AO::zero();
+#pragma unroll
+        for (int kb = 0; kb < NB2; ++kb)
+#pragma unroll
+          for (int sl = 0; sl < 4; ++sl) {
+            const T tv = S<T>::conj(Tt[(16 * kb + 4 * sl + lk) * TS + 16 * ib + li]);  // T^H[i][k]
+            cmma(z[ib], tv, AO::get(w[kb], sl));
+          }
+      }
+      for (int r0 = 0; r0 < mp; r0 += 16) {
+        const int rr = r0 + li;
+        Acc acc = AO::zero();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = j0 + lk + 4 * q;
+          AO::set(acc, q, (rr < mp && j < n) ? cp[(size_t)rr + (size_t)j * lda] : zero);
+        }
+#pragma unroll
+        for (int kb = 0; kb < NB2; ++kb)
+#pragma unroll
+          for (int sl = 0; sl < 4; ++sl) {
+            const T vv = (rr < mp) ? Pn[rr + (16 * kb + 4 * sl + lk) * mp] : zero;
+            cmma(acc, zero - AO::get(z[kb], sl), vv);  // C^T -= Z^T V^T
+          }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = j0 + lk + 4 * q;
+          if (rr < mp && j < n) cp[(size_t)rr + (size_t)j * lda] = AO::get(acc, q);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kLargeThreads) void batched_large_kernel(int count, const int* __restrict__ order,
                                                             const rnpx_qr_desc* __restrict__ desc, T* A, T* tau,
-                                                            T* B, unsigned* counter) {
+                                                            T* B, unsigned* counter, int bmin) {
+  __shared__ double lds[kLargeLds];
   __shared__ double sh[8];
+  __shared__ T xs;
   __shared__ int next;
+  constexpr int cap = kLargeLds / (int)(sizeof(T) / sizeof(double));
+  T* Ws = reinterpret_cast<T*>(lds);
+  const int t = threadIdx.x;
   while (true) {
-    if (threadIdx.x == 0) next = (int)atomicAdd(counter, 1u);
+    if (t == 0) next = (int)atomicAdd(counter, 1u);
     __syncthreads();
     const int idx = next;
     __syncthreads();
     if (idx >= count) break;
     const rnpx_qr_desc d = desc[order[idx]];
+    const int m = d.m, n = d.n;
     T* a = A + d.a_off;
-    team_factor<T, 256>(d.m, d.n, a, d.lda, tau + d.tau_off, sh);
-    __threadfence_block();
-    __syncthreads();
-    if (d.nrhs > 0 && B && d.m >= d.n) team_solve<T, 256>(d.m, d.n, a, d.lda, tau + d.tau_off, d.nrhs, B + d.b_off, d.ldb);
+    const bool solve = d.nrhs > 0 && B && m >= n;
+    if (m * n <= cap && (n < bmin || m > Blk<T>::ROWS)) {
+      for (int e = t; e < m * n; e += kLargeThreads) Ws[e] = a[(size_t)(e % m) + (size_t)(e / m) * d.lda];
+      __syncthreads();
+      team_factor<T, kLargeThreads, true>(m, n, Ws, m, tau + d.tau_off, sh);
+      __syncthreads();
+      for (int e = t; e < m * n; e += kLargeThreads) a[(size_t)(e % m) + (size_t)(e / m) * d.lda] = Ws[e];
+      if (solve) team_solve<T, kLargeThreads>(m, n, Ws, m, tau + d.tau_off, d.nrhs, B + d.b_off, d.ldb, sh, &xs);
+    } else {
+      if (m <= Blk<T>::ROWS)
+        blocked_factor<T>(m, n, a, d.lda, tau + d.tau_off, lds, sh);
+      else
+        team_factor<T, kLargeThreads, true>(m, n, a, d.lda, tau + d.tau_off, sh);
+      __threadfence_block();
+      __syncthreads();
+      if (solve) team_solve<T, kLargeThreads>(m, n, a, d.lda, tau + d.tau_off, d.nrhs, B + d.b_off, d.ldb, sh, &xs);
+    }
     __syncthreads();
   }
 }
@@ -297,8 +521,14 @@ hipError_t launch_batched(int tier, int count, const int* order, const rnpx_qr_d
     hipLaunchKernelGGL((batched_small_kernel<T>), dim3(teams), dim3(64), lds_bytes, st, count, order, desc, A, tau, B,
                        counter, lds_bytes / (int)sizeof(T));
   } else {
-    hipLaunchKernelGGL((batched_large_kernel<T>), dim3(teams), dim3(256), 0, st, count, order, desc, A, tau, B,
-                       counter);
+    // Real problems of at least bmin columns take the blocked path even when
+    // they would fit in LDS whole (RNPX_BATCH_BLOCKED_MIN, read once).
+    static const int bmin = [] {
+      const char* e = getenv("RNPX_BATCH_BLOCKED_MIN");
+      return e ? atoi(e) : kBlockedMinCols;
+    }();
+    hipLaunchKernelGGL((batched_large_kernel<T>), dim3(teams), dim3(kLargeThreads), 0, st, count, order, desc, A, tau, B,
+                       counter, bmin);
   }
   return hipGetLastError();
 }
