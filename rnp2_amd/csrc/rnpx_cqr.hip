@@ -27,12 +27,12 @@
 // R2^-1 = I - E_U (both exact to O(|E|^2) <= 2^-68); otherwise it is
 // factored.
 //
-// The kernel gives up -- without writing anything but the fail flag --
-// when the first Cholesky meets a pivot below 2^-40 of its diagonal entry, a
-// column whose squared norm is outside [2^-600, 2^600] (so the Gram entries
-// cannot under/overflow), a non-finite value, or the second pass's test
-// above fails.  The column-by-column kernel then runs on the panel.  Every
-// workgroup takes the same decision from the same reduced Gram.
+// The panel is refactored column by column instead (by workgroup 0, inside
+// the same launch; the fail flag records it) when the first Cholesky meets a
+// pivot below 2^-40 of its diagonal entry, a column whose squared norm is
+// outside [2^-600, 2^600] (so the Gram entries cannot under/overflow), a
+// non-finite value, or the second pass's test above fails.  Every workgroup
+// takes the same decision from the same reduced Gram.
 //
 // Layout: G workgroups of 256 threads; workgroup g stages rows
 // [g*RW, (g+1)*RW) of the panel in LDS (row-major, padded), so the panel is
@@ -47,6 +47,7 @@
 // forms R, one more workgroup forms T.
 #include "rnpx_kernels.hpp"
 #include "rnpx_small64.hpp"
+#include "rnpx_team.hpp"
 
 #include <cstdlib>
 
@@ -275,6 +276,49 @@ __device__ __forceinline__ double dev_from_identity(const double (*M)[LDA_S], do
   return r;
 }
 
+// The column-by-column refactorization of a rejected panel, by one
+// workgroup after every other has left the kernel: team_factor
+// (Reflector::Generate's rules, RNP/LA/Reflector.hpp:117-158, and the
+// Householder application of QR::Factor_unblocked, QR.hpp:78-102) in place,
+// the clean V, then T from V's Gram (GenerateBlockTr("F","C"),
+// Reflector.hpp:580-626): T(c,c) = tau_c, T(0:c,c) = -tau_c T(0:c,0:c)
+// V(:,0:c)^T v_c.  Column-serial, so slow, but rejections are rare.  No
+// workgroup reads A after staging its rows, so rewriting A here is safe.
+__device__ void refactor_columnwise(const CqrArgs& a, double (*Gs)[LDA_S], double (*Ts)[LDA_S], double* sh, int tid) {
+  if (tid == 0) *a.fail = 1u;
+  const int mp = a.mp;
+  team_factor<double, 256, true>(mp, NB, a.A, (int)a.lda, a.tau, sh);
+  __syncthreads();
+  for (long e = tid; e < (long)mp * NB; e += 256) {
+    const int r = (int)(e % mp), c = (int)(e / mp);
+    a.V[(size_t)r + (size_t)c * a.ldv] = (r > c) ? a.A[(size_t)r + (size_t)c * a.lda] : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int l = e / NB, c = e % NB;
+    double gv = 0.0;
+    if (l < c)
+      for (int r = c; r < mp; ++r) gv = fma(a.V[(size_t)r + (size_t)l * a.ldv], a.V[(size_t)r + (size_t)c * a.ldv], gv);
+    Gs[l][c] = gv;
+    Ts[l][c] = 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < NB; ++c) {
+    const double tc = a.tau[c];
+    if (tid < c) {
+      double sacc = 0.0;
+      for (int l = tid; l < c; ++l) sacc = fma(Ts[tid][l], Gs[l][c], sacc);
+      Ts[tid][c] = -tc * sacc;
+    }
+    if (tid == c) Ts[c][c] = tc;
+    __syncthreads();
+  }
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    a.T[(size_t)i + (size_t)j * a.ldt] = Ts[i][j];
+  }
+}
+
 }  // namespace
 
 template <int RW>
@@ -322,9 +366,9 @@ __global__ __launch_bounds__(256) void cqr_panel_kernel(CqrArgs a) {
   __syncthreads();
   const bool ok1 = small64::chol64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, gdiag, &okf, tid);  // M1 = R1, M2 = R1^-1
   if (!ok1) {
-    // Every workgroup factored the same Gram and leaves here together;
-    // nothing of the panel has been written.
-    if (g == 0 && tid == 0) *a.fail = 1u;
+    // Every workgroup factored the same Gram and takes this branch; nothing
+    // of the panel has been written.
+    if (g == 0) refactor_columnwise(a, M1, M2, rowbuf, tid);
     return;
   }
   if (g == 1)
@@ -338,7 +382,7 @@ __global__ __launch_bounds__(256) void cqr_panel_kernel(CqrArgs a) {
   __syncthreads();
   const double dev = dev_from_identity(M1, red4, tid);
   if (!(dev <= kSecondPass)) {
-    if (g == 0 && tid == 0) *a.fail = 1u;
+    if (g == 0) refactor_columnwise(a, M1, M2, rowbuf, tid);
     return;
   }
   if (dev <= kFirstOrder) {

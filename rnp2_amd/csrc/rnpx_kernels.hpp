@@ -60,9 +60,9 @@ template <typename T>
 hipError_t launch_panel(const PanelArgs<T>& a, int nb, int rpt, hipStream_t st);
 
 // Real 64-column panel by CholeskyQR2 + Householder reconstruction
-// (rnpx_cqr.hip).  Outputs as the column-wise panel, plus T directly; on
-// rejection it writes only *fail = 1 and the caller runs the column-wise
-// panel (launched gated on fail) and the T kernel instead.
+// (rnpx_cqr.hip).  Outputs as the column-wise panel, plus T directly; a
+// panel it rejects is refactored column by column inside the same launch
+// (one workgroup), with the same outputs and *fail = 1.
 struct CqrArgs {
   int mp;               // panel rows (>= 128)
   double* A; long lda;  // panel top-left
@@ -84,12 +84,6 @@ constexpr int kCqrMaxGroups = 256;
 // panel is outside what the kernel takes (mp < 128 or too many rows).
 int cqr_panel_groups(int mp, int max_groups, int* rw);
 hipError_t launch_cqr_panel(const CqrArgs& a, int G, int rw, hipStream_t st);
-
-// One workgroup, gated on run_if: factors a real mp x ib (ib <= 64) panel the
-// CholeskyQR2 kernel rejected, column by column in place, and writes its
-// clean V and its T (64 x 64 block, zeros beyond ib).
-hipError_t launch_panel_fallback(int mp, int ib, double* A, long lda, double* tau, double* V, long ldv, double* Tm,
-                                 long ldt, const unsigned* run_if, hipStream_t st);
 
 // T factor of a forward, columnwise block reflector from its Gram matrix
 // (the larft analogue, RNP/LA/Reflector.hpp:573-626): k <= NB reflectors,

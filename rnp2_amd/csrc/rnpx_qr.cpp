@@ -348,9 +348,8 @@ static bool use_cqr(const Handle* h, int mp, int ib, int nb, int* g, int* rw) {
 // holds, per inner panel p: [1 + p] the column-wise kernel's barrier
 // counter, [1 + npan + p] the CholeskyQR2 kernel's fail flag, and
 // kCqrSyncWords words at 1 + 2 npan + kCqrSyncWords p for its reductions.
-// Where the CholeskyQR2 kernel applies it writes V and T itself; a panel it
-// rejects (fail flag set) is redone behind it by one gated workgroup, which
-// returns at once otherwise and so takes no grid from a concurrent update.
+// Where the CholeskyQR2 kernel applies it writes V and T itself (a panel it
+// rejects is refactored column by column inside the same launch).
 template <typename T>
 static int panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, T* A, long lda, T* tau, T* V, long ldv,
                       T* Tm, long ldt, hipStream_t st) {
@@ -375,8 +374,7 @@ static int panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, 
       ca.sync = sync + 1 + 2 * npan + (size_t)kCqrSyncWords * p;
       ca.abortf = (unsigned*)h->flag.p;
       ca.fail = sync + 1 + npan + p;
-      RNPX_TRY(launch_cqr_panel(ca, G, rw, st));
-      return hip_status(launch_panel_fallback(mp, ib, ca.A, lda, ca.tau, V, ldv, Tm, ldt, ca.fail, st));
+      return hip_status(launch_cqr_panel(ca, G, rw, st));
     }
   }
   PanelArgs<T> pa;
