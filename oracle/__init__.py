@@ -75,6 +75,11 @@ def _declare(L):
                                                        ctypes.POINTER(Tuning)]
         getattr(L, f"orc_{pre}_generateq_unblocked").argtypes = [_sz, _sz, _sz, _p, _sz, _p, _p]
         getattr(L, f"orc_{pre}_ls_solve").argtypes = [_sz, _sz, _sz, _p, _sz, _p, _sz]
+        getattr(L, f"orc_{pre}_lq_factor").argtypes = [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]
+        getattr(L, f"orc_{pre}_lq_multq").argtypes = [ctypes.c_char_p, ctypes.c_char_p, _sz, _sz, _sz,
+                                                      _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]
+        getattr(L, f"orc_{pre}_lq_generateq").argtypes = [_sz, _sz, _sz, _p, _sz, _p,
+                                                          ctypes.POINTER(_sz), _p]
         getattr(L, f"orc_{pre}_gemm").argtypes = [ctypes.c_char_p, ctypes.c_char_p, _sz, _sz, _sz,
                                                   scal, _p, _sz, _p, _sz, scal, _p, _sz]
         getattr(L, f"orc_{pre}_generate_vector").argtypes = [ctypes.c_int, _sz, _p, _p]
@@ -305,6 +310,67 @@ def generateq(afac, tau, n=None, k=None, tune=None, lwork=None, unblocked=False)
         lw = _sz(lwork)
     work = np.zeros(max(lw.value, 1), dtype=a.dtype)
     f(m, n, k, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), _ptr(work), _tuning(tune))
+    return a
+
+
+def lq_factor(a):
+    """LQ::Factor (LQ.hpp:126-190): returns (afac, tau); L on and below the
+    diagonal, the conjugated reflectors in the rows above it."""
+    a = _fortran(a, np.result_type(a, np.float64))
+    pre = _pre(a.dtype)
+    m, n = a.shape
+    k = min(m, n)
+    tau = np.zeros(max(k, 1), dtype=a.dtype)
+    f = getattr(lib(), f"orc_{pre}_lq_factor")
+    lw = _sz(0)
+    f(m, n, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), None)
+    work = np.zeros(max(lw.value, m, 1), dtype=a.dtype)
+    f(m, n, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), _ptr(work))
+    return a, tau[:k]
+
+
+def lq_factor_query(m, n, dtype=np.float64):
+    lw = _sz(0)
+    getattr(lib(), f"orc_{_pre(dtype)}_lq_factor")(m, n, None, max(m, 1), None, ctypes.byref(lw), None)
+    return lw.value
+
+
+def lq_multq(side, trans, afac, tau, c, k=None):
+    """LQ::MultQ (LQ.hpp:309-405): op(Q) C or C op(Q)."""
+    c = _fortran(c, np.result_type(c, afac, np.float64))
+    pre = _pre(c.dtype)
+    afac = _fortran(afac, c.dtype)
+    tau = np.ascontiguousarray(tau, dtype=c.dtype)
+    m, n = c.shape
+    if k is None:
+        k = tau.size
+    lda = afac.shape[0]
+    f = getattr(lib(), f"orc_{pre}_lq_multq")
+    lw = _sz(0)
+    f(side.encode(), trans.encode(), m, n, k, _ptr(afac), lda, _ptr(tau), _ptr(c), max(m, 1),
+      ctypes.byref(lw), None)
+    work = np.zeros(max(lw.value, m, n, 1), dtype=c.dtype)
+    f(side.encode(), trans.encode(), m, n, k, _ptr(afac), lda, _ptr(tau), _ptr(c), max(m, 1),
+      ctypes.byref(lw), _ptr(work))
+    return c
+
+
+def lq_generateq(afac, tau, m=None, k=None):
+    """LQ::GenerateQ (LQ.hpp:491-559): the first m rows of Q (m <= n)."""
+    a = _fortran(afac, np.result_type(afac, np.float64))
+    pre = _pre(a.dtype)
+    n = a.shape[1]
+    if m is None:
+        m = a.shape[0]
+    a = np.asfortranarray(a[:m, :])
+    tau = np.ascontiguousarray(tau, dtype=a.dtype)
+    if k is None:
+        k = min(tau.size, m)
+    f = getattr(lib(), f"orc_{pre}_lq_generateq")
+    lw = _sz(0)
+    f(m, n, k, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), None)
+    work = np.zeros(max(lw.value, m, 1), dtype=a.dtype)
+    f(m, n, k, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), _ptr(work))
     return a
 
 

@@ -117,6 +117,24 @@ void orc_d_generateq(size_t m, size_t n, size_t k, double *a, size_t lda, const 
 void orc_z_generateq(size_t m, size_t n, size_t k, orc_z *a, size_t lda, const orc_z *tau,
                      size_t *lwork, orc_z *work, const orc_tuning *tune);
 
+/* ---- LQ (RNP/LA/LQ.hpp) ----
+ *   LQ::Factor_unblocked  LQ.hpp:78-102,   LQ::Factor    LQ.hpp:126-190
+ *   LQ::MultQ_unblocked   LQ.hpp:220-270,  LQ::MultQ     LQ.hpp:309-405
+ *   LQ::GenerateQ_unblocked LQ.hpp:429-469, LQ::GenerateQ LQ.hpp:491-559
+ *   Reflector::Apply vconj branches, Reflector.hpp:238-291 */
+void orc_d_lq_factor_unblocked(size_t m, size_t n, double *a, size_t lda, double *tau, double *work);
+void orc_z_lq_factor_unblocked(size_t m, size_t n, orc_z *a, size_t lda, orc_z *tau, orc_z *work);
+void orc_d_lq_factor(size_t m, size_t n, double *a, size_t lda, double *tau, size_t *lwork, double *work);
+void orc_z_lq_factor(size_t m, size_t n, orc_z *a, size_t lda, orc_z *tau, size_t *lwork, orc_z *work);
+void orc_d_lq_multq(const char *side, const char *trans, size_t m, size_t n, size_t k, const double *a,
+                    size_t lda, const double *tau, double *c, size_t ldc, size_t *lwork, double *work);
+void orc_z_lq_multq(const char *side, const char *trans, size_t m, size_t n, size_t k, const orc_z *a,
+                    size_t lda, const orc_z *tau, orc_z *c, size_t ldc, size_t *lwork, orc_z *work);
+void orc_d_lq_generateq(size_t m, size_t n, size_t k, double *a, size_t lda, const double *tau,
+                        size_t *lwork, double *work);
+void orc_z_lq_generateq(size_t m, size_t n, size_t k, orc_z *a, size_t lda, const orc_z *tau,
+                        size_t *lwork, orc_z *work);
+
 /* Least squares min ||A x - B|| for m >= n by Factor, MultQ("L","C") and an
  * upper-triangular back substitution with R (BLAS dtrsm "L","U","N","N";
  * the composition of QR.hpp:129, QR.hpp:337 and BLAS_mix.hpp:2096).

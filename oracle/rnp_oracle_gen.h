@@ -586,6 +586,182 @@ void FN(generateq)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *tau,
 	}
 }
 
+/* ---------------------------------------------------------------------
+ * LQ (RNP/LA/LQ.hpp): reflectors stored in rows, conjugated.
+ * ------------------------------------------------------------------- */
+
+/* BLAS::Conjugate, RNP/BLAS/BLAS_mix.hpp:233-239. */
+static void FN(conjugate)(size_t n, T *x, size_t incx){
+	for(size_t i = 0; i < n; ++i){ x[i*incx] = CONJ(x[i*incx]); }
+}
+
+/* Reflector::Apply, Reflector.hpp:238-291, for vone == 1 and vconj == true:
+ * v = (1, conj(stored[1..])), H = I - tau v v^H.  LQ::MultQ_unblocked
+ * (LQ.hpp:255, 268) reaches these branches.  Side "L" follows the
+ * operations at Reflector.hpp:240-264: w = conj(C(0,:) + C(1:,:)^T v_s),
+ * C(0,:) -= tau conj(w), C(i,j) -= tau conj(v_s[i]) conj(w_j).  The
+ * reference's loop there writes c[i+0*ldc] for every j (Reflector.hpp:261),
+ * i.e. only column 0; this restatement uses column j, the update the lines
+ * compute (LQ.hpp:195-196 defines the result as Q C). */
+static void FN(apply_vconj)(const char *side, size_t m, size_t n, const T *v, size_t incv, T tau, T *c,
+                            size_t ldc, T *work){
+	size_t lenv = 0, lenc = 0;
+	const int left = ('L' == side[0]);
+	if(0.0 != tau){
+		lenv = left ? m : n;
+		if(0 == lenv){ return; }
+		size_t i = (lenv-1)*incv;
+		while(lenv > 0 && 0.0 == v[i]){
+			--lenv;
+			i -= incv;
+		}
+		if(left){
+			lenc = FN(last_nonzero_column_limit)(lenv, n, c, ldc);
+		}else{
+			lenc = FN(last_nonzero_row_limit)(m, lenv, c, ldc);
+		}
+		if(0 == lenc){ return; }
+	}
+	if(left){
+		for(size_t j = 0; j < lenc; ++j){
+			T w = 0;
+			for(size_t i = 1; i < lenv; ++i){ w += c[i+j*ldc] * v[i*incv]; }
+			w += c[0+j*ldc];
+			work[j] = CONJ(w);
+		}
+		for(size_t j = 0; j < lenc; ++j){ c[0+j*ldc] += -tau * CONJ(work[j]); }
+		for(size_t j = 0; j < lenc; ++j){
+			for(size_t i = 1; i < lenv; ++i){
+				c[i+j*ldc] -= tau * CONJ(v[i*incv]) * CONJ(work[j]);
+			}
+		}
+	}else{
+		for(size_t i = 0; i < lenc; ++i){ work[i] = c[i]; }
+		for(size_t j = 1; j < lenv; ++j){
+			const T cvj = CONJ(v[j*incv]);
+			for(size_t i = 0; i < lenc; ++i){ work[i] += c[i+j*ldc] * cvj; }
+		}
+		for(size_t i = 0; i < lenc; ++i){ c[i] += -tau * work[i]; }
+		/* Rank1Update (geru): C(:,1:) += -tau w v_s^T */
+		for(size_t j = 1; j < lenv; ++j){
+			for(size_t i = 0; i < lenc; ++i){ c[i+j*ldc] += -tau * work[i] * v[j*incv]; }
+		}
+	}
+}
+
+/* LQ::Factor_unblocked, LQ.hpp:78-102. */
+void FN(lq_factor_unblocked)(size_t m, size_t n, T *a, size_t lda, T *tau, T *work){
+	const size_t k = m < n ? m : n;
+	for(size_t i = 0; i < k; ++i){
+		FN(conjugate)(n-i, &a[i+i*lda], lda);
+		T alpha = a[i+i*lda];
+		T *x = (i+1 < n) ? &a[i+(i+1)*lda] : &a[i+i*lda];  /* n-i-1 == 0: not read */
+		FN(generate)(n-i, &alpha, x, lda, &tau[i]);
+		if(i+1 < m){
+			a[i+i*lda] = 1;
+			FN(apply)("R", 0, m-i-1, n-i, &a[i+i*lda], lda, tau[i], &a[i+1+i*lda], lda, work);
+		}
+		a[i+i*lda] = alpha;
+		FN(conjugate)(n-i, &a[i+i*lda], lda);
+	}
+}
+
+/* LQ::Factor, LQ.hpp:126-190: argument checks and the workspace query
+ * (nb n, nb = LQ::Tuning default 32) as there.  The blocked loop
+ * (LQ.hpp:163-183: Factor_unblocked on row blocks, GenerateBlockTr("F","R"),
+ * ApplyBlock("R","N","F","R")) computes the same factors as the unblocked
+ * loop in exact arithmetic; the restatement runs the unblocked loop for all
+ * sizes, so its rounding is that of the reference's unblocked path. */
+void FN(lq_factor)(size_t m, size_t n, T *a, size_t lda, T *tau, size_t *lwork, T *work){
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(NULL == lwork){ orc_assert_fail("NULL != lwork", __LINE__); return; }
+	if(!(0 == *lwork || *lwork >= m)){ orc_assert_fail("0 == *lwork || *lwork >= m", __LINE__); }
+	const size_t k = m < n ? m : n;
+	if(0 == k){ return; }
+	if(NULL == work || 0 == *lwork){
+		*lwork = 32*n;
+		return;
+	}
+	FN(lq_factor_unblocked)(m, n, a, lda, tau, work);
+}
+
+/* LQ::MultQ_unblocked, LQ.hpp:220-270: forward loop for (L,N) and (R,C),
+ * backward otherwise; tau conjugated for "N". */
+void FN(lq_multq_unblocked)(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                            const T *a, size_t lda, const T *tau, T *c, size_t ldc, T *work){
+	if(!(ldc >= m)){ orc_assert_fail("ldc >= m", __LINE__); }
+	if(0 == m || 0 == n || 0 == k){ return; }
+	const int left = ('L' == side[0]);
+	const int notran = ('N' == trans[0]);
+	if(!((left && k <= m) || (!left && k <= n))){ orc_assert_fail("k range", __LINE__); }
+	if(!(lda >= k)){ orc_assert_fail("lda >= k", __LINE__); }
+	size_t mi = m, ni = n, ic = 0, jc = 0;
+	const int forward = ((left && notran) || (!left && !notran));
+	for(size_t s = 0; s < k; ++s){
+		const size_t i = forward ? s : k-1-s;
+		if(left){ mi = m-i; ic = i; }else{ ni = n-i; jc = i; }
+		const T taui = notran ? CONJ(tau[i]) : tau[i];
+		FN(apply_vconj)(side, mi, ni, &a[i+i*lda], lda, taui, &c[ic+jc*ldc], ldc, work);
+	}
+}
+
+/* LQ::MultQ, LQ.hpp:309-405: checks and the workspace query (nb nw + nb^2);
+ * the product itself by the unblocked loop (the blocked one applies the
+ * same reflectors in the same order). */
+void FN(lq_multq)(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                  const T *a, size_t lda, const T *tau, T *c, size_t ldc, size_t *lwork, T *work){
+	if(!(ldc >= m)){ orc_assert_fail("ldc >= m", __LINE__); }
+	if(0 == m || 0 == n || 0 == k){ return; }
+	const int left = ('L' == side[0]);
+	const size_t nw = left ? n : m;
+	if(0 == *lwork || NULL == work){
+		*lwork = 32*nw + 32*32;
+		return;
+	}
+	FN(lq_multq_unblocked)(side, trans, m, n, k, a, lda, tau, c, ldc, work);
+}
+
+/* LQ::GenerateQ_unblocked, LQ.hpp:429-469. */
+void FN(lq_generateq_unblocked)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *tau, T *work){
+	if(!(n >= m)){ orc_assert_fail("n >= m", __LINE__); }
+	if(!(k <= m)){ orc_assert_fail("k <= m", __LINE__); }
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(0 == m){ return; }
+	if(k < m){
+		for(size_t j = 0; j < n; ++j){
+			for(size_t i = k; i < m; ++i){ a[i+j*lda] = 0; }
+			if(j >= k && j < m){ a[j+j*lda] = 1; }
+		}
+	}
+	size_t i = k;
+	while(i-- > 0){
+		if(i+1 < n){
+			FN(conjugate)(n-i-1, &a[i+(i+1)*lda], lda);
+			if(i+1 < m){
+				a[i+i*lda] = 1;
+				FN(apply)("R", 0, m-i-1, n-i, &a[i+i*lda], lda, CONJ(tau[i]), &a[i+1+i*lda], lda, work);
+			}
+			for(size_t j = i+1; j < n; ++j){ a[i+j*lda] *= -tau[i]; }
+			FN(conjugate)(n-i-1, &a[i+(i+1)*lda], lda);
+		}
+		a[i+i*lda] = 1.0 - CONJ(tau[i]);
+		for(size_t l = 0; l < i; ++l){ a[i+l*lda] = 0; }
+	}
+}
+
+/* LQ::GenerateQ, LQ.hpp:491-559: checks and the workspace query (n nb); Q by
+ * the unblocked loop. */
+void FN(lq_generateq)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *tau, size_t *lwork, T *work){
+	if(!(k <= m)){ orc_assert_fail("k <= m", __LINE__); }
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(0 == m){ return; }
+	if(0 == *lwork || NULL == work){
+		*lwork = n*32;
+		return;
+	}
+	FN(lq_generateq_unblocked)(m, n, k, a, lda, tau, work);
+}
+
 /* Least squares by QR: Factor (QR.hpp:129), MultQ("L","C") on B
  * (QR.hpp:337), then R x = (Q^H B)(0:n,:) by back substitution
  * (SolveTrM "L","U","N","N", BLAS_mix.hpp:2096). */

@@ -507,3 +507,100 @@ def test_gemm(ta, tb):
     B = b if tb == "N" else (b.conj().T.copy() if tb == "C" else b.T.copy())
     got = O.gemm(ta, tb, 2.0 - 1j, A, B)
     np.testing.assert_allclose(got, (2.0 - 1j) * (a @ b), atol=1e-12 * np.abs(a @ b).max())
+
+
+# ---------------------------------------------------------------------------
+# LQ (RNP/LA/LQ.hpp).  LAPACK's gelq2 is conj(row) + larfg + larf from the
+# right, i.e. QR of A^H: the oracle's row-wise restatement must reproduce
+# LAPACK geqrf on A^H (R^H = L, conj of V in the rows, same tau).
+# ---------------------------------------------------------------------------
+
+LQ_SHAPES = [(5, 9), (9, 5), (40, 70), (70, 40), (33, 33), (1, 1), (1, 6), (6, 1), (130, 200)]
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("shape", LQ_SHAPES)
+def test_lq_factor_matches_lapack_qr_of_adjoint(shape, cplx):
+    m, n = shape
+    rng = np.random.default_rng(m * 1000 + n)
+    a = _rand(rng, m, n, cplx)
+    af, tau = O.lq_factor(a)
+    k = min(m, n)
+    qr, tl, _, info = (lapack.zgeqrf if cplx else lapack.dgeqrf)(a.conj().T.copy())
+    assert info == 0
+    L = np.tril(af)[:, :k]
+    s = max(np.abs(a).max(), 1e-300)
+    assert np.abs(L - np.triu(qr)[:k].conj().T).max() <= 1e-13 * s * max(m, n)
+    assert np.abs(np.triu(af, 1)[:k] - np.tril(qr, -1)[:, :k].conj().T).max() <= 1e-12
+    np.testing.assert_allclose(tau, tl[:k], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_lq_brute_force_reflectors(cplx):
+    """Q from GenerateQ equals (H_0 H_1 ... H_{k-1})^H built from the stored
+    rows (v_i = conj of row i, unit at i) and tau; A = L Q, Q Q^H = I."""
+    rng = np.random.default_rng(8)
+    m, n = 6, 9
+    a = _rand(rng, m, n, cplx)
+    af, tau = O.lq_factor(a)
+    H = np.eye(n, dtype=af.dtype)
+    for i in range(m):
+        v = np.zeros(n, dtype=af.dtype)
+        v[i] = 1.0
+        v[i + 1:] = np.conj(af[i, i + 1:])
+        H = H @ (np.eye(n) - tau[i] * np.outer(v, v.conj()))
+    Q = O.lq_generateq(af, tau)
+    np.testing.assert_allclose(Q, H.conj().T[:m], atol=1e-14)
+    np.testing.assert_allclose(np.tril(af)[:, :m] @ Q, a, atol=1e-13)
+    np.testing.assert_allclose(Q @ Q.conj().T, np.eye(m), atol=1e-14)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("side,trans", [("L", "N"), ("L", "C"), ("R", "N"), ("R", "C")])
+def test_lq_multq_against_explicit_q(side, trans, cplx):
+    """MultQ (LQ.hpp:195-196: Q C, Q^H C, C Q, C Q^H) against the explicit Q
+    of GenerateQ.  k <= 32 takes the reference's unblocked loop; its side "L"
+    branch of Reflector::Apply (Reflector.hpp:258-263) updates column 0 only,
+    so the restatement follows the documented product instead."""
+    rng = np.random.default_rng(12)
+    for k, nq in ((5, 11), (20, 40)):
+        a = _rand(rng, k, nq, cplx)
+        af, tau = O.lq_factor(a)
+        full = np.zeros((nq, nq), dtype=af.dtype, order="F")
+        full[:k] = af
+        Q = O.lq_generateq(full, tau, m=nq, k=k)
+        c = _rand(rng, nq, 7, cplx) if side == "L" else _rand(rng, 7, nq, cplx)
+        got = O.lq_multq(side, trans, af, tau, c, k=k)
+        op = Q if trans == "N" else Q.conj().T
+        want = op @ c if side == "L" else c @ op
+        np.testing.assert_allclose(got, want, atol=1e-13 * max(1.0, np.abs(want).max()))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_lq_sign_rule_and_edges(cplx):
+    """Re(alpha) == 0 gives beta > 0 (Reflector.hpp:131); a zero row takes
+    tau = 0; tiny and huge rows take the scaled paths."""
+    rng = np.random.default_rng(14)
+    a = _rand(rng, 8, 12, cplx)
+    a[0, 0] = 0.0
+    a[2, :] = 0.0
+    a[3, :] *= 1e-300
+    a[4, :] *= 1e300
+    af, tau = O.lq_factor(a)
+    assert af[0, 0].real > 0
+    assert tau[2] == 0
+    # LAPACK's sign at alpha == 0 differs, and with the zero row that changes
+    # the later pivots; the oracle's own column-wise QR of A^H (RNP's rule,
+    # pinned above) is the comparison: same factors, conjugate-transposed.
+    qf, qt = O.factor(a.conj().T)
+    k = 8
+    np.testing.assert_allclose(np.diag(af).real, np.diag(qf).real, rtol=1e-13)
+    np.testing.assert_allclose(tau, qt, rtol=0, atol=1e-13)
+    sc = np.abs(np.tril(af)[:, :k]).max(axis=1, keepdims=True)
+    sc[sc == 0] = 1.0
+    assert np.max(np.abs(np.tril(af)[:, :k] - np.triu(qf)[:k].conj().T) / sc) <= 1e-13
+
+
+def test_lq_workspace_queries():
+    assert O.lq_factor_query(50, 70) == 32 * 70
+    assert O.lq_factor_query(0, 70) == 0
