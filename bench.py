@@ -114,6 +114,35 @@ def leg_factor(R, torch, m, n, cplx, reps=3):
             "gflops": round(fl / t / 1e9, 1)}
 
 
+def leg_lq(R, torch, m=4096, n=8192, reps=3):
+    """LQ::Factor of a wide m x n matrix (RNP/LA/LQ.hpp:126), beside QR::Factor
+    of its n x m transpose: the difference is the two conjugate transposes."""
+    h = R.Handle()
+    a0 = R.colmajor_empty(m, n, torch.float64)
+    h.larnv(2, [2, 7, 1, 8], a0)
+    a = R.colmajor_empty(m, n, torch.float64)
+    at = R.colmajor_empty(n, m, torch.float64)
+    tau = torch.empty(min(m, n), dtype=torch.float64, device="cuda")
+    h.synchronize()
+    stream = torch.cuda.ExternalStream(h.stream)
+
+    def run_lq():
+        with torch.cuda.stream(stream):
+            a.copy_(a0)
+        h.gelqf(a, tau)
+
+    def run_qr():
+        with torch.cuda.stream(stream):
+            at.copy_(a0.t())
+        h.geqrf(at, tau)
+
+    t_lq = _ev_time(run_lq, reps, h.synchronize)
+    t_qr = _ev_time(run_qr, reps, h.synchronize)
+    fl = qr_flops(n, m)
+    return {"shape": [m, n], "dtype": "f64", "lq_ms": round(t_lq * 1e3, 3), "lq_gflops": round(fl / t_lq / 1e9, 1),
+            "qr_of_transpose_ms": round(t_qr * 1e3, 3)}
+
+
 def c3_shapes(count, seed):
     rng = np.random.default_rng(seed)
     m = np.rint(np.exp(rng.uniform(np.log(16), np.log(256), count))).astype(int)
@@ -279,7 +308,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--legs", default="all",
-                    help="comma list of extra legs: complex,tall,batched,mixed,serving; 'none' or 'all'")
+                    help="comma list of extra legs: complex,tall,batched,mixed,serving,lq; 'none' or 'all'")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -367,7 +396,7 @@ def main():
         "panel_hbm_gbs": round(pan["bytes"] / (pan["ms"] * 1e-3) / 1e9, 1) if pan["ms"] > 0 else None,
         "tau_sane": ok,
     }
-    legs = [] if args.legs == "none" else (["complex", "tall", "batched", "mixed", "serving"]
+    legs = [] if args.legs == "none" else (["complex", "tall", "batched", "mixed", "serving", "lq"]
                                            if args.legs == "all" else args.legs.split(","))
     extra = {}
     for leg in legs:
@@ -383,6 +412,8 @@ def main():
                 extra["mixed_ls_16384x2048"] = leg_mixed_ls(R, torch)
             elif leg == "serving":
                 extra["serving"] = leg_serving(R, torch)
+            elif leg == "lq":
+                extra["lq_4096x8192"] = leg_lq(R, torch)
         except Exception as e:  # a failing leg is reported, the headline line still prints
             extra[leg] = {"error": repr(e)}
     if extra:

@@ -95,6 +95,32 @@ int rnpx_dorgqr(size_t m, size_t n, size_t k, double *a, size_t lda, const doubl
 int rnpx_zungqr(size_t m, size_t n, size_t k, rnpx_zcomplex *a, size_t lda,
                 const rnpx_zcomplex *tau, size_t *lwork, rnpx_zcomplex *work);
 
+/* LQ (RNP/LA/LQ.hpp): A = L Q with the reflectors conjugated in the rows
+ * above L.  Same argument lists, conventions and workspace queries as the
+ * reference: LQ::Factor nb*n (LQ.hpp:135-138), LQ::MultQ nb*nw + nb*nb
+ * (LQ.hpp:329-333), LQ::GenerateQ n*nb (LQ.hpp:500-504), nb = 32. */
+/* LQ::Factor<double> -- RNP/LA/LQ.hpp:126-190 (dgelqf). */
+int rnpx_dgelqf(size_t m, size_t n, double *a, size_t lda, double *tau,
+                size_t *lwork, double *work);
+/* LQ::Factor<std::complex<double>> -- RNP/LA/LQ.hpp:126-190 (zgelqf). */
+int rnpx_zgelqf(size_t m, size_t n, rnpx_zcomplex *a, size_t lda, rnpx_zcomplex *tau,
+                size_t *lwork, rnpx_zcomplex *work);
+/* LQ::MultQ<double> -- RNP/LA/LQ.hpp:309-405 (dormlq): c <- op(Q) c or c op(Q),
+ * the product LQ.hpp:195-196 documents. */
+int rnpx_dormlq(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                const double *a, size_t lda, const double *tau, double *c, size_t ldc,
+                size_t *lwork, double *work);
+/* LQ::MultQ<std::complex<double>> -- RNP/LA/LQ.hpp:309-405 (zunmlq). */
+int rnpx_zunmlq(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                const rnpx_zcomplex *a, size_t lda, const rnpx_zcomplex *tau,
+                rnpx_zcomplex *c, size_t ldc, size_t *lwork, rnpx_zcomplex *work);
+/* LQ::GenerateQ<double> -- RNP/LA/LQ.hpp:491-559 (dorglq): the first m rows of Q. */
+int rnpx_dorglq(size_t m, size_t n, size_t k, double *a, size_t lda, const double *tau,
+                size_t *lwork, double *work);
+/* LQ::GenerateQ<std::complex<double>> -- RNP/LA/LQ.hpp:491-559 (zunglq). */
+int rnpx_zunglq(size_t m, size_t n, size_t k, rnpx_zcomplex *a, size_t lda,
+                const rnpx_zcomplex *tau, size_t *lwork, rnpx_zcomplex *work);
+
 /* ---------------------------------------------------------------------
  * Handles (device-resident work on one HIP stream).
  * ------------------------------------------------------------------- */
@@ -125,6 +151,10 @@ int rnpx_profile_read(rnpx_handle h, int kernel, long *launches, double *ms, dou
 /* Device-pointer forms of Factor / MultQ / GenerateQ (same semantics). */
 int rnpx_dgeqrf_dev(rnpx_handle h, size_t m, size_t n, double *a, size_t lda, double *tau);
 int rnpx_zgeqrf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex *a, size_t lda,
+                    rnpx_zcomplex *tau);
+/* LQ::Factor on device pointers, asynchronous (RNP/LA/LQ.hpp:126-190). */
+int rnpx_dgelqf_dev(rnpx_handle h, size_t m, size_t n, double *a, size_t lda, double *tau);
+int rnpx_zgelqf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex *a, size_t lda,
                     rnpx_zcomplex *tau);
 int rnpx_dormqr_dev(rnpx_handle h, const char *side, const char *trans, size_t m, size_t n,
                     size_t k, const double *a, size_t lda, const double *tau, double *c,

@@ -71,6 +71,12 @@ SIGNATURES = {
     "rnpx_zunmqr": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
     "rnpx_dorgqr": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
     "rnpx_zungqr": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_dgelqf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_zgelqf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_dormlq": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "rnpx_zunmlq": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "rnpx_dorglq": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_zunglq": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
     "rnpx_create": (_i, [ctypes.POINTER(_p), _sz]),
     "rnpx_destroy": (_i, [_p]),
     "rnpx_set_stream": (_i, [_p, _p]),
@@ -84,6 +90,8 @@ SIGNATURES = {
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "rnpx_dgeqrf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_zgeqrf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_dgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_zgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_dormqr_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
     "rnpx_zunmqr_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
     "rnpx_dorgqr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p]),
@@ -265,6 +273,57 @@ def generateq(afac, tau, n=None, k=None):
     return a
 
 
+def lq_factor(a):
+    """LQ-factor a numpy matrix (LQ::Factor): returns (L and conjugated
+    reflector rows packed, tau)."""
+    a = np.array(a, dtype=np.result_type(a, np.float64), order="F", copy=True)
+    m, n = a.shape
+    k = min(m, n)
+    tau = np.zeros(max(k, 1), dtype=a.dtype)
+    pre = _pre(a.dtype)
+    lw = _sz(0)
+    fn = getattr(lib(), f"rnpx_{pre}gelqf")
+    _check(fn(m, n, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), None), "LQ::Factor query")
+    work = np.zeros(max(lw.value, 1), dtype=a.dtype)
+    _check(fn(m, n, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), _ptr(work)), "LQ::Factor")
+    return a, tau[:k]
+
+
+def lq_multq(side, trans, afac, tau, c, k=None):
+    """LQ::MultQ: op(Q) C or C op(Q) for the LQ factor (afac, tau)."""
+    c = np.array(c, dtype=np.result_type(c, afac, np.float64), order="F", copy=True)
+    afac = np.asfortranarray(afac, dtype=c.dtype)
+    tau = np.ascontiguousarray(tau, dtype=c.dtype)
+    m, n = c.shape
+    k = tau.size if k is None else k
+    pre = _pre(c.dtype)
+    fn = getattr(lib(), "rnpx_dormlq" if pre == "d" else "rnpx_zunmlq")
+    lw = _sz(0)
+    _check(fn(side.encode(), trans.encode(), m, n, k, _ptr(afac), afac.shape[0], _ptr(tau), _ptr(c), max(m, 1),
+              ctypes.byref(lw), None), "LQ::MultQ query")
+    work = np.zeros(max(lw.value, 1), dtype=c.dtype)
+    _check(fn(side.encode(), trans.encode(), m, n, k, _ptr(afac), afac.shape[0], _ptr(tau), _ptr(c), max(m, 1),
+              ctypes.byref(lw), _ptr(work)), "LQ::MultQ")
+    return c
+
+
+def lq_generateq(afac, tau, m=None, k=None):
+    """LQ::GenerateQ: the first m rows of Q (m <= n)."""
+    a = np.array(afac, dtype=np.result_type(afac, np.float64), order="F", copy=True)
+    n = a.shape[1]
+    m = a.shape[0] if m is None else m
+    a = np.asfortranarray(a[:m, :])
+    tau = np.ascontiguousarray(tau, dtype=a.dtype)
+    k = min(tau.size, m) if k is None else k
+    pre = _pre(a.dtype)
+    fn = getattr(lib(), "rnpx_dorglq" if pre == "d" else "rnpx_zunglq")
+    lw = _sz(0)
+    _check(fn(m, n, k, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), None), "LQ::GenerateQ query")
+    work = np.zeros(max(lw.value, 1), dtype=a.dtype)
+    _check(fn(m, n, k, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), _ptr(work)), "LQ::GenerateQ")
+    return a
+
+
 def version():
     return lib().rnpx_version().decode()
 
@@ -362,6 +421,12 @@ class Handle:
         pre = _tpre(a)
         _check(getattr(lib(), f"rnpx_{pre}geqrf_dev")(self.h, m, n, _dptr(a), _ld(a), _dptr(tau)),
                "geqrf_dev")
+
+    def gelqf(self, a, tau):
+        m, n = a.shape
+        pre = _tpre(a)
+        _check(getattr(lib(), f"rnpx_{pre}gelqf_dev")(self.h, m, n, _dptr(a), _ld(a), _dptr(tau)),
+               "gelqf_dev")
 
     def ormqr(self, side, trans, a, tau, c, k=None):
         m, n = c.shape

@@ -27,6 +27,7 @@ SOURCES = [
     "rnpx_batched.hip",
     "rnpx_mixed.hip",
     "rnpx_qr.cpp",
+    "rnpx_lq.cpp",
     "rnpx_ls.cpp",
     "rnpx_serve.cpp",
     "rnpx_api.cpp",

@@ -148,6 +148,85 @@ int dropin_orgqr(size_t m, size_t n, size_t k, T* a, size_t lda, const T* tau, s
   return sync_handle(h);
 }
 
+// LQ drop-ins (RNP/LA/LQ.hpp): argument checks and workspace queries as the
+// reference's LQ::Factor (LQ.hpp:126-138: nb n), LQ::MultQ (LQ.hpp:318-333:
+// nb nw + nb^2) and LQ::GenerateQ (LQ.hpp:496-504: n nb).
+template <typename T>
+int dropin_gelqf(size_t m, size_t n, T* a, size_t lda, T* tau, size_t* lwork) {
+  if (lda < m || lda == 0) return -4;
+  if (!lwork) return -6;
+  if (!(*lwork == 0 || *lwork >= m)) return -6;
+  const size_t k = m < n ? m : n;
+  if (k == 0) return RNPX_OK;
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU;
+  if ((st = A.in(h, h->stage1, a, m, n, lda, true))) return st;
+  if ((st = TAU.in(h, h->stage2, tau, k, 1, k, false))) return st;
+  st = gelqf_dev<T>(h, (int)m, (int)n, A.dev, A.ld, TAU.dev);
+  if (st) return st;
+  if ((st = A.out(h, lda))) return st;
+  if ((st = TAU.out(h, k))) return st;
+  return sync_handle(h);
+}
+
+template <typename T>
+int dropin_ormlq(const char* side, const char* trans, size_t m, size_t n, size_t k, const T* a, size_t lda,
+                 const T* tau, T* c, size_t ldc, size_t* lwork, bool query) {
+  if (!side || !trans) return !side ? -1 : -2;
+  if (ldc < m || ldc == 0) return -10;
+  if (m == 0 || n == 0 || k == 0) return RNPX_OK;
+  const bool left = ('L' == side[0]);
+  if (side[0] != 'L' && side[0] != 'R') return -1;
+  if (trans[0] != 'N' && trans[0] != 'C' && trans[0] != 'T') return -2;
+  if ((left && k > m) || (!left && k > n)) return -5;
+  const size_t nq = left ? m : n, nw = left ? n : m;
+  if (lda < k) return -7;
+  if (!lwork) return -11;
+  if (query) {
+    *lwork = kRefNb * nw + kRefNb * kRefNb;
+    return RNPX_OK;
+  }
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU, C;
+  if ((st = A.in(h, h->stage1, const_cast<T*>(a), k, nq, lda, true))) return st;
+  if ((st = TAU.in(h, h->stage2, const_cast<T*>(tau), k, 1, k, true))) return st;
+  if ((st = C.in(h, h->stage3, c, m, n, ldc, true))) return st;
+  st = ormlq_dev<T>(h, side[0], trans[0], (int)m, (int)n, (int)k, A.dev, A.ld, TAU.dev, C.dev, C.ld);
+  if (st) return st;
+  if ((st = C.out(h, ldc))) return st;
+  return sync_handle(h);
+}
+
+template <typename T>
+int dropin_orglq(size_t m, size_t n, size_t k, T* a, size_t lda, const T* tau, size_t* lwork, bool query) {
+  if (k > m) return -3;
+  if (lda < m || lda == 0) return -5;
+  if (m == 0) return RNPX_OK;
+  if (!lwork) return -7;
+  if (query) {
+    *lwork = n * kRefNb;
+    return RNPX_OK;
+  }
+  if (n < m) return -2;
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU;
+  if ((st = A.in(h, h->stage1, a, m, n, lda, true))) return st;
+  if ((st = TAU.in(h, h->stage2, const_cast<T*>(tau), k ? k : 1, 1, k ? k : 1, k > 0))) return st;
+  st = orglq_dev<T>(h, (int)m, (int)n, (int)k, A.dev, A.ld, TAU.dev);
+  if (st) return st;
+  if ((st = A.out(h, lda))) return st;
+  return sync_handle(h);
+}
+
 Handle* H(rnpx_handle h) { return reinterpret_cast<Handle*>(h); }
 
 }  // namespace
@@ -214,6 +293,46 @@ int rnpx_zungqr(size_t m, size_t n, size_t k, rnpx_zcomplex* a, size_t lda, cons
                           lwork && (work == nullptr || *lwork == 0));
 }
 
+int rnpx_dgelqf(size_t m, size_t n, double* a, size_t lda, double* tau, size_t* lwork, double* work) {
+  if (lwork && (work == nullptr || *lwork == 0)) {
+    if (lda < m || lda == 0) return -4;
+    if ((m < n ? m : n) == 0) return RNPX_OK;
+    *lwork = kRefNb * n;
+    return RNPX_OK;
+  }
+  return dropin_gelqf<double>(m, n, a, lda, tau, lwork);
+}
+int rnpx_zgelqf(size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau, size_t* lwork,
+                rnpx_zcomplex* work) {
+  if (lwork && (work == nullptr || *lwork == 0)) {
+    if (lda < m || lda == 0) return -4;
+    if ((m < n ? m : n) == 0) return RNPX_OK;
+    *lwork = kRefNb * n;
+    return RNPX_OK;
+  }
+  return dropin_gelqf<zd>(m, n, reinterpret_cast<zd*>(a), lda, reinterpret_cast<zd*>(tau), lwork);
+}
+int rnpx_dormlq(const char* side, const char* trans, size_t m, size_t n, size_t k, const double* a, size_t lda,
+                const double* tau, double* c, size_t ldc, size_t* lwork, double* work) {
+  return dropin_ormlq<double>(side, trans, m, n, k, a, lda, tau, c, ldc, lwork,
+                              lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zunmlq(const char* side, const char* trans, size_t m, size_t n, size_t k, const rnpx_zcomplex* a, size_t lda,
+                const rnpx_zcomplex* tau, rnpx_zcomplex* c, size_t ldc, size_t* lwork, rnpx_zcomplex* work) {
+  return dropin_ormlq<zd>(side, trans, m, n, k, reinterpret_cast<const zd*>(a), lda,
+                          reinterpret_cast<const zd*>(tau), reinterpret_cast<zd*>(c), ldc, lwork,
+                          lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_dorglq(size_t m, size_t n, size_t k, double* a, size_t lda, const double* tau, size_t* lwork,
+                double* work) {
+  return dropin_orglq<double>(m, n, k, a, lda, tau, lwork, lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zunglq(size_t m, size_t n, size_t k, rnpx_zcomplex* a, size_t lda, const rnpx_zcomplex* tau, size_t* lwork,
+                rnpx_zcomplex* work) {
+  return dropin_orglq<zd>(m, n, k, reinterpret_cast<zd*>(a), lda, reinterpret_cast<const zd*>(tau), lwork,
+                          lwork && (work == nullptr || *lwork == 0));
+}
+
 int rnpx_create(rnpx_handle* h, size_t hbm_budget_bytes) {
   if (!h) return -1;
   Handle* p = nullptr;
@@ -273,6 +392,14 @@ int rnpx_dgeqrf_dev(rnpx_handle h, size_t m, size_t n, double* a, size_t lda, do
 int rnpx_zgeqrf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau) {
   RNPX_DEV_CHECK(h, m, n);
   return geqrf_dev<zd>(H(h), (int)m, (int)n, reinterpret_cast<zd*>(a), (long)lda, reinterpret_cast<zd*>(tau));
+}
+int rnpx_dgelqf_dev(rnpx_handle h, size_t m, size_t n, double* a, size_t lda, double* tau) {
+  RNPX_DEV_CHECK(h, m, n);
+  return gelqf_dev<double>(H(h), (int)m, (int)n, a, (long)lda, tau);
+}
+int rnpx_zgelqf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau) {
+  RNPX_DEV_CHECK(h, m, n);
+  return gelqf_dev<zd>(H(h), (int)m, (int)n, reinterpret_cast<zd*>(a), (long)lda, reinterpret_cast<zd*>(tau));
 }
 int rnpx_dormqr_dev(rnpx_handle h, const char* side, const char* trans, size_t m, size_t n, size_t k, const double* a,
                     size_t lda, const double* tau, double* c, size_t ldc) {

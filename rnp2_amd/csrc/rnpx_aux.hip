@@ -2,6 +2,8 @@
 // extraction from a packed factor, zeroing, and RNP's random generator.
 #include "rnpx_kernels.hpp"
 
+#include <algorithm>
+
 namespace rnpx {
 
 template <typename T>
@@ -128,9 +130,41 @@ hipError_t launch_larnv(int dist, const int iseed[4], long long count, T* x, hip
   return hipGetLastError();
 }
 
+// B (cols x rows) = conj(A)^T for A (rows x cols): 32x32 tiles through LDS,
+// both sides read and written down their columns.
+template <typename T>
+__global__ __launch_bounds__(256) void conj_transpose_kernel(int rows, int cols, const T* __restrict__ A, long lda,
+                                                             T* __restrict__ B, long ldb) {
+  __shared__ T tile[32][33];
+  const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+#pragma unroll
+  for (int q = 0; q < 32; q += 8) {
+    const int r = r0 + tx, c = c0 + ty + q;
+    if (r < rows && c < cols) tile[ty + q][tx] = A[(size_t)r + (size_t)c * lda];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 32; q += 8) {
+    const int c = c0 + tx, r = r0 + ty + q;  // B(c, r) = conj(A(r, c))
+    if (r < rows && c < cols) B[(size_t)c + (size_t)r * ldb] = S<T>::conj(tile[tx][ty + q]);
+  }
+}
+
+template <typename T>
+hipError_t launch_conj_transpose(int rows, int cols, const T* A, long lda, T* B, long ldb, hipStream_t st) {
+  if (rows < 0 || cols < 0 || lda < std::max(1, rows) || ldb < std::max(1, cols)) return hipErrorInvalidValue;
+  if ((long)rows * cols == 0) return hipSuccess;
+  if ((rows + 31) / 32 > 2147483647 || (cols + 31) / 32 > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((conj_transpose_kernel<T>), dim3((rows + 31) / 32, (cols + 31) / 32), dim3(256), 0, st, rows, cols,
+                     A, lda, B, ldb);
+  return hipGetLastError();
+}
+
 #define RNPX_INST(T)                                                                  \
   template hipError_t launch_set<T>(int, int, T, T, T*, long, hipStream_t);           \
   template hipError_t launch_copy<T>(int, int, const T*, long, T*, long, hipStream_t); \
+  template hipError_t launch_conj_transpose<T>(int, int, const T*, long, T*, long, hipStream_t); \
   template hipError_t launch_extract_v<T>(int, int, const T*, long, T*, long, hipStream_t); \
   template hipError_t launch_zero_rows<T>(int, int, T*, long, hipStream_t);           \
   template hipError_t launch_larnv<T>(int, const int*, long long, T*, hipStream_t);

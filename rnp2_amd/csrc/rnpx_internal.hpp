@@ -54,12 +54,13 @@ struct Handle {
   Buffer cqr_part, cqr_gsum, cqr_scratch;  // CholeskyQR2 panel scratch
   Buffer wbuf2, zbuf2, part2, misc2;       // GEMM scratch of the panel stream
   Buffer trsm_flags;                       // per-block flags of the blocked triangular solve
+  Buffer lqbuf;                            // A^H for the LQ routines
   unsigned trsm_epoch = 0;                 // value the flags take in the current solve
   // Every buffer above, for accounting and release.
   std::vector<Buffer*> buffers() {
     return {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc,
             &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
-            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags};
+            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf};
   }
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
@@ -104,6 +105,12 @@ template <typename T>
 int ormqr_dev(Handle* h, char side, char trans, int m, int n, int k, const T* A, long lda, const T* tau, T* C,
               long ldc);
 template <typename T> int orgqr_dev(Handle* h, int m, int n, int k, T* A, long lda, const T* tau);
+// LQ (rnpx_lq.cpp): LQ::Factor, LQ::MultQ, LQ::GenerateQ on device memory.
+template <typename T> int gelqf_dev(Handle* h, int m, int n, T* A, long lda, T* tau);
+template <typename T>
+int ormlq_dev(Handle* h, char side, char trans, int m, int n, int k, const T* A, long lda, const T* tau, T* C,
+              long ldc);
+template <typename T> int orglq_dev(Handle* h, int m, int n, int k, T* A, long lda, const T* tau);
 template <typename T>
 int larft_dev(Handle* h, int n, int k, const T* V, long ldv, const T* tau, T* Tout, long ldt);
 template <typename T>

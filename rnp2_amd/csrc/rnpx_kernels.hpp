@@ -115,6 +115,9 @@ hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long
 // Small helpers.
 template <typename T>
 hipError_t launch_set(int m, int n, T offdiag, T diag, T* A, long lda, hipStream_t st);
+// B (cols x rows, leading dimension ldb) = conj(A)^T, A rows x cols.
+template <typename T>
+hipError_t launch_conj_transpose(int rows, int cols, const T* A, long lda, T* B, long ldb, hipStream_t st);
 template <typename T>
 hipError_t launch_copy(int m, int n, const T* A, long lda, T* B, long ldb, hipStream_t st);
 // Clean V (unit lower trapezoidal, n x k) out of a packed factor.

@@ -202,3 +202,40 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_cpp_lq_surface_compiles_and_queries():
+    """include/rnpx_lq.hpp: LQ's template signatures and workspace queries
+    (LQ.hpp:135-138 nb*n, 329-333 nb*nw + nb^2, 500-504 n*nb)."""
+    src = r'''
+    #include <complex>
+    #include <cstdio>
+    #include "rnpx_lq.hpp"
+    using namespace RNP::LA;
+    int main() {
+      size_t lw = 0;
+      LQ::Factor<double>(50, 70, NULL, 50, NULL, &lw, NULL);
+      std::printf("factor %zu\n", lw);
+      lw = 0;
+      LQ::Factor<std::complex<double> >(0, 70, NULL, 1, NULL, &lw, NULL);
+      std::printf("empty %zu\n", lw);
+      lw = 0;
+      LQ::MultQ<double>("R", "N", 40, 100, 30, NULL, 30, NULL, NULL, 40, &lw, NULL);
+      std::printf("multq %zu\n", lw);
+      lw = 0;
+      LQ::GenerateQ<float>(30, 100, 30, NULL, 30, NULL, &lw, NULL);
+      std::printf("genq %zu\n", lw);
+      LQ::Factor<double>(10, 5, NULL, 9, NULL, &lw, NULL);        /* lda < m */
+      double a[50] = {1}, tau[5], work[400];
+      lw = 400;
+      LQ::Factor(5, 10, a, 5, tau, &lw, work);
+      std::printf("status %d\n", LQ::LastStatus());
+      return 0;
+    }'''
+    out = _run_c(src, "g++", "cpp", ("-std=c++11", "-L", os.path.dirname(R.LIB_PATH), "-lrnpx",
+                                     f"-Wl,-rpath,{os.path.dirname(R.LIB_PATH)}"))
+    assert "factor 2240" in out and "empty 0" in out
+    assert "multq 2304" in out and "genq 3200" in out
+    assert "Assertion failed in" in out and "lda >= m" in out
+    if not has_gpu():
+        assert f"status {R.RNPX_ERR_NOGPU}" in out and "rnpx: LQ::Factor: status" in out
