@@ -137,8 +137,9 @@ __device__ __forceinline__ void gram_partial(const double (*Ab)[LDA_S], double* 
 // Sum of partials src[w * kGramElems + e], w = 0..n-1 (n <= 16), in order.
 // Thread tid owns the entry pairs 2 tid + 512 q; eight partials' loads are
 // in flight at once.
+template <int ELEMS = kGramElems, int STRIDE = ELEMS>
 __device__ __forceinline__ void sum_partials(const double* src, int n, double* dst, int tid) {
-  constexpr int Q = kGramElems / 512;  // pairs per thread
+  constexpr int Q = ELEMS / 512;  // pairs per thread
   constexpr int W = 8;                 // partials per batch
   double2 acc[Q];
 #pragma unroll
@@ -147,7 +148,7 @@ __device__ __forceinline__ void sum_partials(const double* src, int n, double* d
     double2 v[W][Q];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      const double2* p = reinterpret_cast<const double2*>(src + (size_t)(w0 + w) * kGramElems) + tid;
+      const double2* p = reinterpret_cast<const double2*>(src + (size_t)(w0 + w) * STRIDE) + tid;
 #pragma unroll
       for (int q = 0; q < Q; ++q) v[w][q] = (w0 + w < n) ? p[256 * q] : make_double2(0.0, 0.0);
     }
@@ -236,6 +237,7 @@ __device__ __forceinline__ void times_upper(double (*Ab)[LDA_S], const double (*
 // LDS; entries of A below its diagonal are treated as zero when mask_a, and
 // column k of A is scaled by dcol[k] when dcol is given):
 // acc[tj] item r = C[16 w + lane/16 + 4r][16 tj + lane%16].
+template <bool TB = false>  // TB: B is given transposed (B[k][j] read as B[j][k])
 __device__ __forceinline__ void upper_product_row(const double (*A)[LDA_S], const double (*B)[LDA_S], bool mask_a, const double* dcol,
                                   f64x4 (&acc)[4], int lane, int wave) {
   const int ti = wave;
@@ -248,7 +250,7 @@ __device__ __forceinline__ void upper_product_row(const double (*A)[LDA_S], cons
       double x = A[i][k];
       if (mask_a && k < i) x = 0.0;
       if (dcol) x *= dcol[k];
-      const double y = B[k][16 * tj + (lane & 15)];
+      const double y = TB ? B[16 * tj + (lane & 15)][k] : B[k][16 * tj + (lane & 15)];
       acc[tj] = mfma16(x, y, acc[tj]);
     }
   }
@@ -488,6 +490,487 @@ __global__ __launch_bounds__(256) void cqr_panel_kernel(CqrArgs a) {
         a.T[(size_t)i + (size_t)j * a.ldt] = (i <= j) ? -tacc[tj][r] : 0.0;
       }
   }
+}
+
+// ---------------------------------------------------------------------------
+// One outer block of 128 columns in one launch: panel 1 (columns 0..63) by
+// CholeskyQR2 + reconstruction as above, its block reflector applied to
+// panel 2's columns (W = V1^T A2 by a grid reduction, Z = T1^T W,
+// A2 -= V1 Z, all in LDS), panel 2 (rows 64.., columns 64..127) the same
+// way over workgroups 1.., and the merged compact-WY form
+//   V = [V1 | (0; V2)],  T = [T1, -T1 (V1^T V2) T2; 0, T2]
+// with V1^T V2 by a third grid reduction.  This replaces, per outer block,
+// two panel launches and the six small products around them.  Workgroup g
+// holds rows [64 g, 64 g + 64) of both panels in LDS for the whole launch;
+// nothing is written to A before panel 2 has passed its tests, so a
+// rejection of either panel leaves A untouched and workgroup 0 refactors
+// all 128 columns column by column instead.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kBlkPassWords = 32;                 // per reduction pass
+constexpr int kBlkPub1 = 6 * kBlkPassWords;        // panel 1's LU published
+constexpr int kBlkPubT1 = kBlkPub1 + 1;            // T1 published
+constexpr int kBlkPub2 = kBlkPub1 + 2;             // panel 2's LU published
+static_assert(kBlkPub2 < kCqrBlockSyncWords, "block sync words");
+constexpr int kCross = 16 * 256;                   // 64x64 product partial (16 tiles)
+// scratch (doubles)
+constexpr int kBR1a = 0, kBU1i = 4096, kBULP1 = 8192, kBD1 = 12288, kBTau1 = 12352, kBRF1 = 12416, kBT1 = 16512,
+              kBR1b = 20608, kBU2i = 24704, kBULP2 = 28800, kBD2 = 32896, kBG = 32960, kBTF = 49344;
+static_assert(kBTF + 16384 <= kCqrBlockScratch, "block scratch");
+
+// Reduction over the participating workgroups (index gi of Gp) of partials
+// of ELEMS doubles at part[gi]; the sum lands in gs and every workgroup
+// (participant or not) waits for it.
+template <int ELEMS>
+__device__ __forceinline__ void blk_allreduce(const CqrBlockArgs& a, int pass, bool participate, int gi, int Gp,
+                                              double* gs, int tid, unsigned* slot) {
+  unsigned* sw = a.sync + pass * kBlkPassWords;
+  if (participate) {
+    const int grp = gi / kGroup, ngroups = (Gp + kGroup - 1) / kGroup;
+    const int gbeg = grp * kGroup, gsize = min(kGroup, Gp - gbeg);
+    if (arrive(&sw[grp], slot) == (unsigned)(gsize - 1)) {
+      // partials and group sums both sit kCross doubles apart
+      if (ngroups == 1) {
+        sum_partials<ELEMS, kCross>(a.part, gsize, gs, tid);
+        publish(&sw[kSyncDone]);
+      } else {
+        sum_partials<ELEMS, kCross>(a.part + (size_t)gbeg * kCross, gsize, a.gpart + (size_t)grp * kCross, tid);
+        if (arrive(&sw[kSyncTop], slot) == (unsigned)(ngroups - 1)) {
+          sum_partials<ELEMS, kCross>(a.gpart, ngroups, gs, tid);
+          publish(&sw[kSyncDone]);
+        }
+      }
+    }
+  }
+  wait_flag(&sw[kSyncDone], a.abortf);
+}
+
+// P = X^T Y over 64 rows (16 tiles of 16x16, tile t = 4 ti + tj, row-major).
+__device__ __forceinline__ void cross_partial(const double (*X)[LDA_S], const double (*Y)[LDA_S], double* out, int lane,
+                                              int wave) {
+  const int ti = wave;
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < 64; k0 += 4) {
+      const double x = X[k0 + (lane >> 4)][16 * ti + (lane & 15)];
+      const double y = Y[k0 + (lane >> 4)][16 * tj + (lane & 15)];
+      acc = mfma16(x, y, acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[(4 * ti + tj) * 256 + ((lane >> 4) + 4 * r) * 16 + (lane & 15)] = acc[r];
+  }
+}
+
+__device__ __forceinline__ void load_tiles16(const double* gs, double (*M)[LDA_S], int tid) {
+  for (int e = tid; e < kCross; e += 256) {
+    const int t = e >> 8, r = (e >> 4) & 15, c = e & 15;
+    M[16 * (t >> 2) + r][16 * (t & 3) + c] = gs[e];
+  }
+}
+
+// acc[tj] = row block `wave` of op(A) B, 64x64 full operands in LDS;
+// item r = C[16 wave + lane/16 + 4 r][16 tj + lane%16].
+template <bool TA>
+__device__ __forceinline__ void prod64(const double (*A)[LDA_S], const double (*B)[LDA_S], f64x4 (&acc)[4], int lane,
+                                       int wave) {
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    acc[tj] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < 64; k0 += 4) {
+      const int i = 16 * wave + (lane & 15), k = k0 + (lane >> 4);
+      const double x = TA ? A[k][i] : A[i][k];
+      const double y = B[k][16 * tj + (lane & 15)];
+      acc[tj] = mfma16(x, y, acc[tj]);
+    }
+  }
+}
+
+__device__ __forceinline__ void store_acc(const f64x4 (&acc)[4], double (*C)[LDA_S], double scale, int lane, int wave) {
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) C[16 * wave + (lane >> 4) + 4 * r][16 * tj + (lane & 15)] = scale * acc[tj][r];
+}
+
+// Rejected block: all 128 columns column by column in place (team_factor),
+// the clean V, and T (128 x 128) from V's Gram by the recursion of
+// GenerateBlockTr("F","C") (Reflector.hpp:580-626), the Gram and T held in
+// scratch.  One workgroup; the others have left.
+__device__ void blk_refactor(const CqrBlockArgs& a, double* sh, int tid) {
+  constexpr int K = 2 * NB;
+  if (tid == 0) *a.fail = 1u;
+  const int mp = a.mp;
+  double* Gm = a.scratch + kBG;
+  double* Tm = a.scratch + kBTF;
+  team_factor<double, 256, true>(mp, K, a.A, (int)a.lda, a.tau, sh);
+  __syncthreads();
+  for (long e = tid; e < (long)mp * K; e += 256) {
+    const int r = (int)(e % mp), c = (int)(e / mp);
+    a.V[(size_t)r + (size_t)c * a.ldv] = (r > c) ? a.A[(size_t)r + (size_t)c * a.lda] : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int e = tid; e < K * K; e += 256) {
+    const int l = e / K, c = e % K;
+    double gv = 0.0;
+    if (l < c)
+      for (int r = c; r < mp; ++r) gv = fma(a.V[(size_t)r + (size_t)l * a.ldv], a.V[(size_t)r + (size_t)c * a.ldv], gv);
+    Gm[e] = gv;
+    Tm[e] = 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < K; ++c) {
+    const double tc = a.tau[c];
+    if (tid < c) {
+      double sacc = 0.0;
+      for (int l = tid; l < c; ++l) sacc = fma(Tm[tid * K + l], Gm[l * K + c], sacc);
+      Tm[tid * K + c] = -tc * sacc;
+    }
+    if (tid == c) Tm[c * K + c] = tc;
+    __syncthreads();
+  }
+  for (int e = tid; e < K * K; e += 256) {
+    const int i = e % K, j = e / K;
+    a.T[(size_t)i + (size_t)j * a.ldt] = Tm[i * K + j];
+  }
+}
+
+// The reconstruction's R = R2 R1 product (R1 from scratch at r1off) into acc;
+// M1 holds R2.
+__device__ __forceinline__ void blk_r_product(const double (*M1)[LDA_S], double (*M2)[LDA_S], const double* r1,
+                                              f64x4 (&acc)[4], int tid, int lane, int wave) {
+  for (int e = tid; e < NB * NB; e += 256) M2[e / NB][e % NB] = r1[e];
+  __syncthreads();
+  upper_product_row(M1, M2, false, nullptr, acc, lane, wave);
+}
+
+// T = -U D Z^T, Z = L^-1, from U/L packed (ulp) and D; acc holds it (upper).
+__device__ __forceinline__ void blk_t_factor(double (*M1)[LDA_S], double (*M2)[LDA_S], double* rowbuf, double* tmpb,
+                                             const double* ulp, const double* dd, f64x4 (&acc)[4], int tid, int lane,
+                                             int wave) {
+  for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = ulp[e];
+  if (tid < NB) rowbuf[tid] = dd[tid];
+  __syncthreads();
+  small64::inv_unit_lower64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, tid);  // M2 = L^-1 (lower blocks)
+  upper_product_row<true>(M1, M2, true, rowbuf, acc, lane, wave);
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
+      acc[tj][r] = (i <= j) ? -acc[tj][r] : 0.0;
+    }
+}
+
+// R2 and R2^-1 from the second pass's Gram in M1 (dev <= kSecondPass):
+// first order when |E| <= 2^-40, factored otherwise.
+__device__ __forceinline__ void blk_second_r(double (*M1)[LDA_S], double (*M2)[LDA_S], double dev, double* tmpb,
+                                             double* gdiag, int* okf, int tid) {
+  if (dev <= kFirstOrder) {
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int i = e / NB, j = e % NB;
+      const double E = M1[i][j] - (i == j ? 1.0 : 0.0);
+      const double eu = (i < j) ? E : ((i == j) ? 0.5 * E : 0.0);
+      M1[i][j] = (i == j ? 1.0 : 0.0) + eu;
+      M2[i][j] = (i == j ? 1.0 : 0.0) - eu;
+    }
+    __syncthreads();
+  } else {
+    small64::chol64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, gdiag, okf, tid);
+  }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
+  __shared__ double Ab1[64][LDA_S];
+  __shared__ double Ab2[64][LDA_S];
+  __shared__ double M1[NB][LDA_S];
+  __shared__ double M2[NB][LDA_S];
+  __shared__ double rowbuf[NB];
+  __shared__ double red4[4];
+  __shared__ double gdiag[NB];
+  __shared__ double tmpb[small64::kTmpDoubles];
+  __shared__ unsigned slot;
+  __shared__ int okf;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = blockIdx.x, G = gridDim.x;
+  const int r0 = 64 * g;
+  const int mp = a.mp;
+  double* sc = a.scratch;
+  const int tgrp1 = 2;  // G >= 3
+  double* part = a.part + (size_t)g * kCross;
+
+  // Both panels' rows of this workgroup.
+  {
+    constexpr int PER = 64 * NB / 256;
+    double v[2][PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + 256 * q, r = e % 64, c = e / 64;
+      const bool in = r0 + r < mp;
+      v[0][q] = in ? a.A[(size_t)(r0 + r) + (size_t)c * a.lda] : 0.0;
+      v[1][q] = in ? a.A[(size_t)(r0 + r) + (size_t)(c + NB) * a.lda] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + 256 * q;
+      Ab1[e % 64][e / 64] = v[0][q];
+      Ab2[e % 64][e / 64] = v[1][q];
+    }
+  }
+  __syncthreads();
+
+  // ---- Panel 1 ----
+  gram_partial<64>(Ab1, part, lane, wave);
+  blk_allreduce<kGramElems>(a, 0, true, g, G, a.gsum, tid, &slot);
+  load_gram(a.gsum, M1, tid);
+  __syncthreads();
+  if (!small64::chol64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, gdiag, &okf, tid)) {
+    if (g == 0) blk_refactor(a, rowbuf, tid);
+    return;
+  }
+  if (g == 1)
+    for (int e = tid; e < NB * NB; e += 256) sc[kBR1a + e] = M1[e / NB][e % NB];
+  times_upper<64>(Ab1, M2, lane, wave);
+  gram_partial<64>(Ab1, part, lane, wave);
+  blk_allreduce<kGramElems>(a, 1, true, g, G, a.gsum + kCross, tid, &slot);
+  load_gram(a.gsum + kCross, M1, tid);
+  __syncthreads();
+  double dev = dev_from_identity(M1, red4, tid);
+  if (!(dev <= kSecondPass)) {
+    if (g == 0) blk_refactor(a, rowbuf, tid);
+    return;
+  }
+  blk_second_r(M1, M2, dev, tmpb, gdiag, &okf, tid);
+  times_upper<64>(Ab1, M2, lane, wave);  // Ab1 = Q, M1 = R2
+  if (g == 0) {
+    for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = Ab1[e / NB][e % NB];
+    __syncthreads();
+    small64::lu64<LDA_S>(&M1[0][0], rowbuf, &M2[0][0], tmpb, tid);  // M1 = U and L, M2 = U^-1
+    for (int e = tid; e < NB * NB; e += 256) {
+      sc[kBULP1 + e] = M1[e / NB][e % NB];
+      sc[kBU1i + e] = M2[e / NB][e % NB];
+    }
+    if (tid < NB) {
+      sc[kBD1 + tid] = rowbuf[tid];
+      sc[kBTau1 + tid] = -M1[tid][tid] * rowbuf[tid];
+    }
+    publish(&a.sync[kBlkPub1]);
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int i = e / NB, j = e % NB;
+      Ab1[i][j] = (i > j) ? M1[i][j] : (i == j ? 1.0 : 0.0);  // V1's top block: L1
+    }
+    __syncthreads();
+  } else {
+    f64x4 racc[4];
+    if (g == 1) blk_r_product(M1, M2, sc + kBR1a, racc, tid, lane, wave);
+    wait_flag(&a.sync[kBlkPub1], a.abortf);
+    if (g == 1) {
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
+          sc[kBRF1 + i * NB + j] = sc[kBD1 + i] * racc[tj][r];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < NB * NB; e += 256) M2[e / NB][e % NB] = sc[kBU1i + e];
+    __syncthreads();
+    times_upper<64>(Ab1, M2, lane, wave);  // Ab1 = V1 rows
+    if (g == tgrp1) {
+      f64x4 tacc[4];
+      blk_t_factor(M1, M2, rowbuf, tmpb, sc + kBULP1, sc + kBD1, tacc, tid, lane, wave);
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
+          sc[kBT1 + i * NB + j] = tacc[tj][r];
+          a.T[(size_t)i + (size_t)j * a.ldt] = tacc[tj][r];
+        }
+      publish(&a.sync[kBlkPubT1]);
+    }
+  }
+
+  // ---- Panel 1's reflectors applied to panel 2: A2 -= V1 (T1^T (V1^T A2)) ----
+  cross_partial(Ab1, Ab2, part, lane, wave);
+  blk_allreduce<kCross>(a, 2, true, g, G, a.gsum + 2 * kCross, tid, &slot);
+  wait_flag(&a.sync[kBlkPubT1], a.abortf);
+  load_tiles16(a.gsum + 2 * kCross, M2, tid);                                     // W
+  for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = sc[kBT1 + e];   // T1
+  __syncthreads();
+  {
+    f64x4 acc[4];
+    prod64<true>(M1, M2, acc, lane, wave);  // Z = T1^T W
+    __syncthreads();
+    store_acc(acc, M1, 1.0, lane, wave);
+    __syncthreads();
+    prod64<false>(Ab1, M1, acc, lane, wave);  // V1 Z
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Ab2[16 * wave + (lane >> 4) + 4 * r][16 * tj + (lane & 15)] -= acc[tj][r];
+    __syncthreads();
+  }
+
+  // ---- Panel 2: rows 64.. (workgroups 1..), LU by workgroup 1, R by 2, T by 0 ----
+  const bool p2 = (g >= 1);
+  const int gi = g - 1, G2 = G - 1;
+  if (p2) gram_partial<64>(Ab2, a.part + (size_t)gi * kCross, lane, wave);
+  blk_allreduce<kGramElems>(a, 3, p2, gi, G2, a.gsum + 3 * kCross, tid, &slot);
+  load_gram(a.gsum + 3 * kCross, M1, tid);
+  __syncthreads();
+  if (!small64::chol64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, gdiag, &okf, tid)) {
+    if (g == 0) blk_refactor(a, rowbuf, tid);
+    return;
+  }
+  if (g == 2)
+    for (int e = tid; e < NB * NB; e += 256) sc[kBR1b + e] = M1[e / NB][e % NB];
+  if (p2) times_upper<64>(Ab2, M2, lane, wave);
+  else __syncthreads();
+  if (p2) gram_partial<64>(Ab2, a.part + (size_t)gi * kCross, lane, wave);
+  blk_allreduce<kGramElems>(a, 4, p2, gi, G2, a.gsum + 4 * kCross, tid, &slot);
+  load_gram(a.gsum + 4 * kCross, M1, tid);
+  __syncthreads();
+  dev = dev_from_identity(M1, red4, tid);
+  if (!(dev <= kSecondPass)) {
+    if (g == 0) blk_refactor(a, rowbuf, tid);
+    return;
+  }
+  blk_second_r(M1, M2, dev, tmpb, gdiag, &okf, tid);
+
+  // Both panels are accepted: panel 1's results go out now.
+  if (g == 0) {
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int i = e % NB, j = e / NB;
+      const double v = Ab1[i][j];
+      if (i > j) a.A[(size_t)i + (size_t)j * a.lda] = v;
+      a.V[(size_t)i + (size_t)j * a.ldv] = v;
+      a.A[(size_t)i + (size_t)(j + NB) * a.lda] = Ab2[i][j];  // R12
+      a.V[(size_t)i + (size_t)(j + NB) * a.ldv] = 0.0;
+      a.T[(size_t)(i + NB) + (size_t)j * a.ldt] = 0.0;
+    }
+    if (tid < NB) a.tau[tid] = sc[kBTau1 + tid];
+  } else {
+    for (int e = tid; e < 64 * NB; e += 256) {
+      const int r = e % 64, c = e / 64, gr = r0 + r;
+      if (gr < mp) {
+        const double v = Ab1[r][c];
+        a.A[(size_t)gr + (size_t)c * a.lda] = v;
+        a.V[(size_t)gr + (size_t)c * a.ldv] = v;
+      }
+    }
+    if (g == 1)
+      for (int e = tid; e < NB * NB; e += 256) {
+        const int i = e % NB, j = e / NB;
+        if (i <= j) a.A[(size_t)i + (size_t)j * a.lda] = sc[kBRF1 + i * NB + j];
+      }
+  }
+
+  if (p2) times_upper<64>(Ab2, M2, lane, wave);  // Ab2 = Q2 (g >= 1), M1 = R2'
+  else __syncthreads();
+  if (g == 1) {
+    for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = Ab2[e / NB][e % NB];
+    __syncthreads();
+    small64::lu64<LDA_S>(&M1[0][0], rowbuf, &M2[0][0], tmpb, tid);
+    for (int e = tid; e < NB * NB; e += 256) {
+      sc[kBULP2 + e] = M1[e / NB][e % NB];
+      sc[kBU2i + e] = M2[e / NB][e % NB];
+    }
+    if (tid < NB) sc[kBD2 + tid] = rowbuf[tid];
+    publish(&a.sync[kBlkPub2]);
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int i = e % NB, j = e / NB;
+      const double l = M1[i][j];
+      const double v = (i > j) ? l : (i == j ? 1.0 : 0.0);
+      Ab2[i][j] = v;
+      if (i > j) a.A[(size_t)(NB + i) + (size_t)(NB + j) * a.lda] = l;
+      a.V[(size_t)(NB + i) + (size_t)(NB + j) * a.ldv] = v;
+    }
+    if (tid < NB) a.tau[NB + tid] = -M1[tid][tid] * rowbuf[tid];
+    __syncthreads();
+  } else if (g >= 2) {
+    f64x4 racc[4];
+    if (g == 2) blk_r_product(M1, M2, sc + kBR1b, racc, tid, lane, wave);
+    wait_flag(&a.sync[kBlkPub2], a.abortf);
+    if (g == 2) {
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
+          if (i <= j) a.A[(size_t)(NB + i) + (size_t)(NB + j) * a.lda] = sc[kBD2 + i] * racc[tj][r];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < NB * NB; e += 256) M2[e / NB][e % NB] = sc[kBU2i + e];
+    __syncthreads();
+    times_upper<64>(Ab2, M2, lane, wave);  // V2 rows
+    for (int e = tid; e < 64 * NB; e += 256) {
+      const int r = e % 64, c = e / 64, gr = r0 + r;
+      if (gr < mp) {
+        const double v = Ab2[r][c];
+        a.A[(size_t)gr + (size_t)(c + NB) * a.lda] = v;
+        a.V[(size_t)gr + (size_t)(c + NB) * a.ldv] = v;
+      }
+    }
+  } else {
+    // g == 0: T2, kept in Ab2 (workgroup 0's panel-2 rows are R12, already out).
+    wait_flag(&a.sync[kBlkPub2], a.abortf);
+    f64x4 tacc[4];
+    blk_t_factor(M1, M2, rowbuf, tmpb, sc + kBULP2, sc + kBD2, tacc, tid, lane, wave);
+    __syncthreads();
+    store_acc(tacc, Ab2, 1.0, lane, wave);
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
+        a.T[(size_t)(NB + i) + (size_t)(NB + j) * a.ldt] = tacc[tj][r];
+      }
+    __syncthreads();
+  }
+
+  // ---- Merge: T12 = -T1 (V1^T V2) T2 ----
+  if (p2) cross_partial(Ab1, Ab2, a.part + (size_t)gi * kCross, lane, wave);
+  blk_allreduce<kCross>(a, 5, p2, gi, G2, a.gsum + 5 * kCross, tid, &slot);
+  if (g == 0) {
+    load_tiles16(a.gsum + 5 * kCross, M2, tid);                                     // S
+    for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = sc[kBT1 + e];   // T1
+    __syncthreads();
+    f64x4 acc[4];
+    prod64<false>(M1, M2, acc, lane, wave);  // X = T1 S
+    __syncthreads();
+    store_acc(acc, M1, 1.0, lane, wave);
+    __syncthreads();
+    prod64<false>(M1, Ab2, acc, lane, wave);  // X T2
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
+        a.T[(size_t)i + (size_t)(NB + j) * a.ldt] = -acc[tj][r];
+      }
+  }
+}
+
+int cqr_block_groups(int mp, int max_groups) {
+  if (mp < 3 * NB) return 0;
+  const int G = (mp + 63) / 64;
+  return (G >= 3 && G <= max_groups && G <= 16 * kGroup) ? G : 0;
+}
+
+hipError_t launch_cqr_block(const CqrBlockArgs& a, int G, hipStream_t st) {
+  if (G < 3 || (long)G * 64 < a.mp || (long)(G - 1) * 64 >= a.mp || a.lda < a.mp || a.ldv < a.mp || a.ldt < 2 * NB ||
+      !a.fail || !a.sync || !a.abortf)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(cqr_block_kernel, dim3(G), dim3(256), 0, st, a);
+  return hipGetLastError();
 }
 
 int cqr_panel_groups(int mp, int max_groups, int* rw) {

@@ -85,6 +85,34 @@ constexpr int kCqrMaxGroups = 256;
 int cqr_panel_groups(int mp, int max_groups, int* rw);
 hipError_t launch_cqr_panel(const CqrArgs& a, int G, int rw, hipStream_t st);
 
+// A real outer block of 128 columns (two 64-column panels) in one launch
+// (rnpx_cqr.hip): both panels by CholeskyQR2 + reconstruction, panel 1
+// applied to panel 2 and the compact-WY forms merged in LDS.  Outputs: R and
+// V packed in A (128 columns), tau (128), the block's clean V (ldv >= mp,
+// zeros above the second panel) and T (128 x 128, ldt >= 128).  If either
+// panel is rejected, A is untouched until workgroup 0 refactors the block
+// column by column in the same launch (*fail = 1).  G = ceil(mp/64)
+// workgroups, all resident at once (they meet at grid-wide reductions).
+struct CqrBlockArgs {
+  int mp;               // block rows (>= 192)
+  double* A; long lda;  // block top-left
+  double* tau;          // out: 128 scalars
+  double* V; long ldv;  // out: clean V, mp x 128
+  double* T; long ldt;  // out: 128 x 128 T
+  double* part;         // scratch: G * 4096
+  double* gpart;        // scratch: 16 * 4096 (group sums)
+  double* gsum;         // scratch: 6 * 4096 (reduced products of each pass)
+  double* scratch;      // scratch: kCqrBlockScratch doubles
+  unsigned* sync;       // kCqrBlockSyncWords words, zero at launch
+  unsigned* abortf;     // sticky error flag
+  unsigned* fail;       // zero at launch; 1 = refactored column by column
+};
+constexpr int kCqrBlockSyncWords = 256;
+constexpr int kCqrBlockScratch = 65728;
+// Workgroups for an mp-row block, 0 when the block kernel does not apply.
+int cqr_block_groups(int mp, int max_groups);
+hipError_t launch_cqr_block(const CqrBlockArgs& a, int G, hipStream_t st);
+
 // T factor of a forward, columnwise block reflector from its Gram matrix
 // (the larft analogue, RNP/LA/Reflector.hpp:573-626): k <= NB reflectors,
 // gram(j,c) = V(:,j)^H V(:,c) for j < c and tau_c on the diagonal.

@@ -442,6 +442,16 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   RNPX_TRY(h->rowbuf.ensure(sizeof(T) * 2 * (size_t)nb, h));
   const int sync_words = 1 + 2 * npan + kCqrSyncWords * npan;
   RNPX_TRY(h->sync.ensure(sizeof(unsigned) * (size_t)sync_words, h));
+  // Outer blocks in one launch (real, 64-column panels, 128-column blocks,
+  // tuning panel 0): buffers for the block kernel.
+  const bool use_block = !S<T>::cplx && h->tune.panel == 0 && nb == 64 && nbo == 2 * nb &&
+                         cqr_block_groups(m, std::min(256, h->num_cus)) > 0;
+  static_assert(2 * kCqrSyncWords >= kCqrBlockSyncWords, "a block uses its two panels' sync words");
+  if (use_block) {
+    RNPX_TRY(h->cqr_part.ensure(sizeof(double) * (size_t)cqr_block_groups(m, std::min(256, h->num_cus)) * 4096, h));
+    RNPX_TRY(h->cqr_gsum.ensure(sizeof(double) * (6 + 16) * 4096, h));
+    RNPX_TRY(h->cqr_scratch.ensure(sizeof(double) * kCqrBlockScratch, h));
+  }
   {
     int g0 = 0, rw0 = 0;
     if (use_cqr<T>(h, m, std::min(nb, k), nb, &g0, &rw0)) {
@@ -470,6 +480,34 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
     T* To = (T*)tb[q]->p;
     hipEvent_t ev;
     h->prof_begin(&ev, ps);
+    if constexpr (!S<T>::cplx) {
+      const int Gb = use_block ? cqr_block_groups(m - i, std::min(256, h->num_cus)) : 0;
+      if (wo == 2 * nb && Gb > 0) {
+        // Both panels, the apply between them and the merge in one launch.
+        const int p1 = i / nb;
+        CqrBlockArgs ba;
+        ba.mp = m - i;
+        ba.A = A + (size_t)i + (size_t)i * lda;
+        ba.lda = lda;
+        ba.tau = tau + i;
+        ba.V = Vo;
+        ba.ldv = ldv;
+        ba.T = To;
+        ba.ldt = ldt;
+        ba.part = (double*)h->cqr_part.p;
+        ba.gsum = (double*)h->cqr_gsum.p;
+        ba.gpart = ba.gsum + 6 * 4096;
+        ba.scratch = (double*)h->cqr_scratch.p;
+        ba.sync = (unsigned*)h->sync.p + 1 + 2 * npan + (size_t)kCqrSyncWords * p1;  // panels p1 and p1+1
+        ba.abortf = (unsigned*)h->flag.p;
+        ba.fail = (unsigned*)h->sync.p + 1 + npan + p1;
+        RNPX_TRY(launch_cqr_block(ba, Gb, ps));
+        const double mpd = m - i;
+        const double fl = 2.0 * mpd * wo * wo - 2.0 / 3.0 * wo * wo * wo;
+        h->prof_end(K_PANEL, ev, fl, sizeof(T) * 3.0 * mpd * wo, ps);
+        return RNPX_OK;
+      }
+    }
     const int ib1 = std::min(nb, wo);
     int s = panel_step<T>(h, i / nb, npan, i, ib1, m, nb, A, lda, tau, Vo, ldv, To, ldt, ps);
     if (s) return s;

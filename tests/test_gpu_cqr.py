@@ -20,10 +20,12 @@ TOL_ELEM = 1e-11
 
 
 def _factor_dev(a, panel):
+    """panel 1: column-wise; 2: CholeskyQR2 per 64-column panel (64-column
+    blocks); 0: the default, 128-column outer blocks in one launch each."""
     import torch
     m, n = a.shape
     h = R.Handle()
-    h.set_tuning(nb=64, panel=panel)
+    h.set_tuning(nb=0 if panel == 0 else 64, panel=panel)
     d = R.colmajor_empty(m, n, torch.float64)
     d.copy_(torch.from_numpy(np.ascontiguousarray(a.T)).to("cuda").t())
     tau = torch.zeros(min(m, n), dtype=torch.float64, device="cuda")
@@ -65,25 +67,32 @@ def test_cqr_panel_matches_oracle(shape):
     assert rel_err_colscaled(np.triu(af2[:k]), np.triu(af1[:k])) <= TOL_ELEM
     af3, tau3 = _factor_dev(a, panel=2)
     assert np.array_equal(af3, af2) and np.array_equal(tau3, tau2), "not deterministic"
+    af0, tau0 = _factor_dev(a, panel=0)
+    _compare(a, af0, tau0, "outer-block kernel")
+    assert rel_err_colscaled(np.triu(af0[:k]), np.triu(af1[:k])) <= TOL_ELEM
+    af4, tau4 = _factor_dev(a, panel=0)
+    assert np.array_equal(af4, af0) and np.array_equal(tau4, tau0), "outer-block kernel not deterministic"
 
 
-def test_cqr_sign_rule_zero_pivot():
+@pytest.mark.parametrize("panel", [2, 0])
+def test_cqr_sign_rule_zero_pivot(panel):
     """A panel whose running pivot is exactly zero (alpha == 0): D = +1, the
     reference's beta > 0."""
-    print("[case] cqr zero pivot", flush=True)
+    print(f"[case] cqr zero pivot panel={panel}", flush=True)
     rng = np.random.default_rng(3)
-    a = np.asfortranarray(rng.standard_normal((256, 64)))
+    a = np.asfortranarray(rng.standard_normal((256, 128)))
     a[0, 0] = 0.0
-    af, tau = _factor_dev(a, panel=2)
+    af, tau = _factor_dev(a, panel=panel)
     _compare(a, af, tau, "zero pivot")
     assert af[0, 0] > 0
 
 
-@pytest.mark.parametrize("kind", ["ill", "dependent", "scaled"])
-def test_cqr_rejected_panels_fall_back(kind):
+@pytest.mark.parametrize("panel", [2, 0])
+@pytest.mark.parametrize("kind", ["ill", "dependent", "scaled", "second"])
+def test_cqr_rejected_panels_fall_back(kind, panel):
     """Badly conditioned, rank-deficient and extremely scaled panels go to the
     column-wise kernel and still match the oracle."""
-    print(f"[case] cqr fallback {kind}", flush=True)
+    print(f"[case] cqr fallback {kind} panel={panel}", flush=True)
     rng = np.random.default_rng(11)
     m, n = 600, 192
     a = rng.standard_normal((m, n))
@@ -91,14 +100,16 @@ def test_cqr_rejected_panels_fall_back(kind):
         a[:, 70] = a[:, 69] + 1e-9 * a[:, 70]          # second panel: kappa ~ 1e9
     elif kind == "dependent":
         a[:, 5] = 2.0 * a[:, 2]
+    elif kind == "second":
+        a[:, 100] = a[:, 99] - 3.0 * a[:, 98]           # second panel of the first block
     else:
         a[:, 130] *= 1e200                              # third panel: Gram would overflow
     a = np.asfortranarray(a)
-    af, tau = _factor_dev(a, panel=2)
-    if kind in ("dependent", "ill"):
+    af, tau = _factor_dev(a, panel=panel)
+    if kind in ("dependent", "ill", "second"):
         # the reflectors after a (near) dependence are fixed by rounding noise
         # in any implementation: elementwise before it, normwise overall
-        c = 5 if kind == "dependent" else 69
+        c = {"dependent": 5, "ill": 69, "second": 99}[kind]
         af_o, tau_o = O.factor(a)
         assert rel_err_colscaled(np.triu(af[:n])[:, :c], np.triu(af_o[:n])[:, :c]) <= TOL_ELEM
         q = R.generateq(af, tau, n=n)
