@@ -504,7 +504,9 @@ __global__ __launch_bounds__(256) void cqr_panel_kernel(CqrArgs a) {
 // holds rows [64 g, 64 g + 64) of both panels in LDS for the whole launch;
 // nothing is written to A before panel 2 has passed its tests, so a
 // rejection of either panel leaves A untouched and workgroup 0 refactors
-// all 128 columns column by column instead.
+// all 128 columns column by column instead.  Each panel's T is formed by the
+// workgroup that factored its LU (still in LDS) while the next grid
+// reduction completes: T1 by workgroup 0, T2 and the merge by workgroup 1.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -515,35 +517,41 @@ constexpr int kBlkPub2 = kBlkPub1 + 2;             // panel 2's LU published
 static_assert(kBlkPub2 < kCqrBlockSyncWords, "block sync words");
 constexpr int kCross = 16 * 256;                   // 64x64 product partial (16 tiles)
 // scratch (doubles)
-constexpr int kBR1a = 0, kBU1i = 4096, kBULP1 = 8192, kBD1 = 12288, kBTau1 = 12352, kBRF1 = 12416, kBT1 = 16512,
-              kBR1b = 20608, kBU2i = 24704, kBULP2 = 28800, kBD2 = 32896, kBG = 32960, kBTF = 49344;
+constexpr int kBR1a = 0, kBU1i = 4096, kBD1 = 12288, kBTau1 = 12352, kBRF1 = 12416, kBT1 = 16512,
+              kBR1b = 20608, kBU2i = 24704, kBD2 = 32896, kBG = 32960, kBTF = 49344;
 static_assert(kBTF + 16384 <= kCqrBlockScratch, "block scratch");
 
 // Reduction over the participating workgroups (index gi of Gp) of partials
 // of ELEMS doubles at part[gi]; the sum lands in gs and every workgroup
 // (participant or not) waits for it.
 template <int ELEMS>
-__device__ __forceinline__ void blk_allreduce(const CqrBlockArgs& a, int pass, bool participate, int gi, int Gp,
-                                              double* gs, int tid, unsigned* slot) {
+__device__ __forceinline__ void blk_contribute(const CqrBlockArgs& a, int pass, int gi, int Gp, double* gs, int tid,
+                                               unsigned* slot) {
   unsigned* sw = a.sync + pass * kBlkPassWords;
-  if (participate) {
-    const int grp = gi / kGroup, ngroups = (Gp + kGroup - 1) / kGroup;
-    const int gbeg = grp * kGroup, gsize = min(kGroup, Gp - gbeg);
-    if (arrive(&sw[grp], slot) == (unsigned)(gsize - 1)) {
-      // partials and group sums both sit kCross doubles apart
-      if (ngroups == 1) {
-        sum_partials<ELEMS, kCross>(a.part, gsize, gs, tid);
+  const int grp = gi / kGroup, ngroups = (Gp + kGroup - 1) / kGroup;
+  const int gbeg = grp * kGroup, gsize = min(kGroup, Gp - gbeg);
+  if (arrive(&sw[grp], slot) == (unsigned)(gsize - 1)) {
+    // partials and group sums both sit kCross doubles apart
+    if (ngroups == 1) {
+      sum_partials<ELEMS, kCross>(a.part, gsize, gs, tid);
+      publish(&sw[kSyncDone]);
+    } else {
+      sum_partials<ELEMS, kCross>(a.part + (size_t)gbeg * kCross, gsize, a.gpart + (size_t)grp * kCross, tid);
+      if (arrive(&sw[kSyncTop], slot) == (unsigned)(ngroups - 1)) {
+        sum_partials<ELEMS, kCross>(a.gpart, ngroups, gs, tid);
         publish(&sw[kSyncDone]);
-      } else {
-        sum_partials<ELEMS, kCross>(a.part + (size_t)gbeg * kCross, gsize, a.gpart + (size_t)grp * kCross, tid);
-        if (arrive(&sw[kSyncTop], slot) == (unsigned)(ngroups - 1)) {
-          sum_partials<ELEMS, kCross>(a.gpart, ngroups, gs, tid);
-          publish(&sw[kSyncDone]);
-        }
       }
     }
   }
-  wait_flag(&sw[kSyncDone], a.abortf);
+}
+__device__ __forceinline__ void blk_wait(const CqrBlockArgs& a, int pass) {
+  wait_flag(&a.sync[pass * kBlkPassWords + kSyncDone], a.abortf);
+}
+template <int ELEMS>
+__device__ __forceinline__ void blk_allreduce(const CqrBlockArgs& a, int pass, bool participate, int gi, int Gp,
+                                              double* gs, int tid, unsigned* slot) {
+  if (participate) blk_contribute<ELEMS>(a, pass, gi, Gp, gs, tid, slot);
+  blk_wait(a, pass);
 }
 
 // P = X^T Y over 64 rows (16 tiles of 16x16, tile t = 4 ti + tj, row-major).
@@ -600,7 +608,7 @@ __device__ __forceinline__ void store_acc(const f64x4 (&acc)[4], double (*C)[LDA
 // the clean V, and T (128 x 128) from V's Gram by the recursion of
 // GenerateBlockTr("F","C") (Reflector.hpp:580-626), the Gram and T held in
 // scratch.  One workgroup; the others have left.
-__device__ void blk_refactor(const CqrBlockArgs& a, double* sh, int tid) {
+__device__ __noinline__ void blk_refactor(CqrBlockArgs a, double* sh, int tid) {
   constexpr int K = 2 * NB;
   if (tid == 0) *a.fail = 1u;
   const int mp = a.mp;
@@ -647,13 +655,10 @@ __device__ __forceinline__ void blk_r_product(const double (*M1)[LDA_S], double 
   upper_product_row(M1, M2, false, nullptr, acc, lane, wave);
 }
 
-// T = -U D Z^T, Z = L^-1, from U/L packed (ulp) and D; acc holds it (upper).
-__device__ __forceinline__ void blk_t_factor(double (*M1)[LDA_S], double (*M2)[LDA_S], double* rowbuf, double* tmpb,
-                                             const double* ulp, const double* dd, f64x4 (&acc)[4], int tid, int lane,
-                                             int wave) {
-  for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = ulp[e];
-  if (tid < NB) rowbuf[tid] = dd[tid];
-  __syncthreads();
+// T = -U D Z^T, Z = L^-1, from U/L packed in M1 and D in rowbuf (left there
+// by lu64); acc holds it (upper).  M2 is overwritten.
+__device__ __forceinline__ void blk_t_factor(const double (*M1)[LDA_S], double (*M2)[LDA_S], const double* rowbuf,
+                                             double* tmpb, f64x4 (&acc)[4], int tid, int lane, int wave) {
   small64::inv_unit_lower64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, tid);  // M2 = L^-1 (lower blocks)
   upper_product_row<true>(M1, M2, true, rowbuf, acc, lane, wave);
 #pragma unroll
@@ -686,6 +691,7 @@ __device__ __forceinline__ void blk_second_r(double (*M1)[LDA_S], double (*M2)[L
 }  // namespace
 
 __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
+  RNPX_CQR_GLOBAL(a);
   __shared__ double Ab1[64][LDA_S];
   __shared__ double Ab2[64][LDA_S];
   __shared__ double M1[NB][LDA_S];
@@ -701,7 +707,6 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
   const int r0 = 64 * g;
   const int mp = a.mp;
   double* sc = a.scratch;
-  const int tgrp1 = 2;  // G >= 3
   double* part = a.part + (size_t)g * kCross;
 
   // Both panels' rows of this workgroup.
@@ -751,10 +756,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
     for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = Ab1[e / NB][e % NB];
     __syncthreads();
     small64::lu64<LDA_S>(&M1[0][0], rowbuf, &M2[0][0], tmpb, tid);  // M1 = U and L, M2 = U^-1
-    for (int e = tid; e < NB * NB; e += 256) {
-      sc[kBULP1 + e] = M1[e / NB][e % NB];
-      sc[kBU1i + e] = M2[e / NB][e % NB];
-    }
+    for (int e = tid; e < NB * NB; e += 256) sc[kBU1i + e] = M2[e / NB][e % NB];
     if (tid < NB) {
       sc[kBD1 + tid] = rowbuf[tid];
       sc[kBTau1 + tid] = -M1[tid][tid] * rowbuf[tid];
@@ -782,24 +784,26 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
     for (int e = tid; e < NB * NB; e += 256) M2[e / NB][e % NB] = sc[kBU1i + e];
     __syncthreads();
     times_upper<64>(Ab1, M2, lane, wave);  // Ab1 = V1 rows
-    if (g == tgrp1) {
-      f64x4 tacc[4];
-      blk_t_factor(M1, M2, rowbuf, tmpb, sc + kBULP1, sc + kBD1, tacc, tid, lane, wave);
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
-          sc[kBT1 + i * NB + j] = tacc[tj][r];
-          a.T[(size_t)i + (size_t)j * a.ldt] = tacc[tj][r];
-        }
-      publish(&a.sync[kBlkPubT1]);
-    }
   }
 
   // ---- Panel 1's reflectors applied to panel 2: A2 -= V1 (T1^T (V1^T A2)) ----
   cross_partial(Ab1, Ab2, part, lane, wave);
-  blk_allreduce<kCross>(a, 2, true, g, G, a.gsum + 2 * kCross, tid, &slot);
+  blk_contribute<kCross>(a, 2, g, G, a.gsum + 2 * kCross, tid, &slot);
+  if (g == 0) {
+    // T1 while the reduction completes; the LU is still in M1 and rowbuf.
+    f64x4 tacc[4];
+    blk_t_factor(M1, M2, rowbuf, tmpb, tacc, tid, lane, wave);
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
+        sc[kBT1 + i * NB + j] = tacc[tj][r];
+        a.T[(size_t)i + (size_t)j * a.ldt] = tacc[tj][r];
+      }
+    publish(&a.sync[kBlkPubT1]);
+  }
+  blk_wait(a, 2);
   wait_flag(&a.sync[kBlkPubT1], a.abortf);
   load_tiles16(a.gsum + 2 * kCross, M2, tid);                                     // W
   for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = sc[kBT1 + e];   // T1
@@ -818,7 +822,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
     __syncthreads();
   }
 
-  // ---- Panel 2: rows 64.. (workgroups 1..), LU by workgroup 1, R by 2, T by 0 ----
+  // ---- Panel 2: rows 64.. (workgroups 1..), LU and T by workgroup 1, R by 2 ----
   const bool p2 = (g >= 1);
   const int gi = g - 1, G2 = G - 1;
   if (p2) gram_partial<64>(Ab2, a.part + (size_t)gi * kCross, lane, wave);
@@ -878,10 +882,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
     for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = Ab2[e / NB][e % NB];
     __syncthreads();
     small64::lu64<LDA_S>(&M1[0][0], rowbuf, &M2[0][0], tmpb, tid);
-    for (int e = tid; e < NB * NB; e += 256) {
-      sc[kBULP2 + e] = M1[e / NB][e % NB];
-      sc[kBU2i + e] = M2[e / NB][e % NB];
-    }
+    for (int e = tid; e < NB * NB; e += 256) sc[kBU2i + e] = M2[e / NB][e % NB];
     if (tid < NB) sc[kBD2 + tid] = rowbuf[tid];
     publish(&a.sync[kBlkPub2]);
     for (int e = tid; e < NB * NB; e += 256) {
@@ -919,13 +920,16 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
         a.V[(size_t)gr + (size_t)(c + NB) * a.ldv] = v;
       }
     }
-  } else {
-    // g == 0: T2, kept in Ab2 (workgroup 0's panel-2 rows are R12, already out).
-    wait_flag(&a.sync[kBlkPub2], a.abortf);
+  }
+
+  // ---- Merge: T12 = -T1 (V1^T V2) T2, by workgroup 1 (which holds LU2) ----
+  if (!p2) return;
+  cross_partial(Ab1, Ab2, a.part + (size_t)gi * kCross, lane, wave);
+  blk_contribute<kCross>(a, 5, gi, G2, a.gsum + 5 * kCross, tid, &slot);
+  if (g != 1) return;
+  {
     f64x4 tacc[4];
-    blk_t_factor(M1, M2, rowbuf, tmpb, sc + kBULP2, sc + kBD2, tacc, tid, lane, wave);
-    __syncthreads();
-    store_acc(tacc, Ab2, 1.0, lane, wave);
+    blk_t_factor(M1, M2, rowbuf, tmpb, tacc, tid, lane, wave);  // T2
 #pragma unroll
     for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
@@ -933,30 +937,25 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
         const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
         a.T[(size_t)(NB + i) + (size_t)(NB + j) * a.ldt] = tacc[tj][r];
       }
-    __syncthreads();
+    store_acc(tacc, Ab2, 1.0, lane, wave);  // Ab2 was last read before the arrival's barrier
   }
-
-  // ---- Merge: T12 = -T1 (V1^T V2) T2 ----
-  if (p2) cross_partial(Ab1, Ab2, a.part + (size_t)gi * kCross, lane, wave);
-  blk_allreduce<kCross>(a, 5, p2, gi, G2, a.gsum + 5 * kCross, tid, &slot);
-  if (g == 0) {
-    load_tiles16(a.gsum + 5 * kCross, M2, tid);                                     // S
-    for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = sc[kBT1 + e];   // T1
-    __syncthreads();
-    f64x4 acc[4];
-    prod64<false>(M1, M2, acc, lane, wave);  // X = T1 S
-    __syncthreads();
-    store_acc(acc, M1, 1.0, lane, wave);
-    __syncthreads();
-    prod64<false>(M1, Ab2, acc, lane, wave);  // X T2
+  blk_wait(a, 5);
+  load_tiles16(a.gsum + 5 * kCross, M2, tid);                                     // S
+  for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = sc[kBT1 + e];   // T1
+  __syncthreads();
+  f64x4 acc[4];
+  prod64<false>(M1, M2, acc, lane, wave);  // X = T1 S
+  __syncthreads();
+  store_acc(acc, M1, 1.0, lane, wave);
+  __syncthreads();
+  prod64<false>(M1, Ab2, acc, lane, wave);  // X T2
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
+  for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
-        a.T[(size_t)i + (size_t)(NB + j) * a.ldt] = -acc[tj][r];
-      }
-  }
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * wave + (lane >> 4) + 4 * r, j = 16 * tj + (lane & 15);
+      a.T[(size_t)i + (size_t)(NB + j) * a.ldt] = -acc[tj][r];
+    }
 }
 
 int cqr_block_groups(int mp, int max_groups) {

@@ -63,6 +63,22 @@ hipError_t launch_panel(const PanelArgs<T>& a, int nb, int rpt, hipStream_t st);
 // (rnpx_cqr.hip).  Outputs as the column-wise panel, plus T directly; a
 // panel it rejects is refactored column by column inside the same launch
 // (one workgroup), with the same outputs and *fail = 1.
+// A generic pointer known to address global memory, rewritten through the
+// global address space so that the compiler issues global instead of flat
+// loads and stores (flat ones also count against the LDS counter, so every
+// LDS wait would wait for them too).
+template <typename T>
+__device__ __forceinline__ T* as_global(T* p) {
+  return (T*)(__attribute__((address_space(1))) T*)p;
+}
+#define RNPX_CQR_GLOBAL(a)                                                                        \
+  do {                                                                                            \
+    (a).A = as_global((a).A); (a).tau = as_global((a).tau); (a).V = as_global((a).V);              \
+    (a).T = as_global((a).T); (a).part = as_global((a).part); (a).gpart = as_global((a).gpart);    \
+    (a).gsum = as_global((a).gsum); (a).scratch = as_global((a).scratch);                         \
+    (a).sync = as_global((a).sync); (a).abortf = as_global((a).abortf); (a).fail = as_global((a).fail); \
+  } while (0)
+
 struct CqrArgs {
   int mp;               // panel rows (>= 128)
   double* A; long lda;  // panel top-left

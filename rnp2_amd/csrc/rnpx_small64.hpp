@@ -117,17 +117,17 @@ __device__ __forceinline__ void inv_upper16(const double* M, int b, double* X, i
 // block (b, b) of M, written at block (zb, zb) of Z (stride SZ).
 template <int S, int SZ>
 __device__ __forceinline__ void inv_unit_lower16(const double* M, int b, double* Z, int zb, int lane) {
+  // Row l of Z per lane (z_l L = e_l, solved from the last column back);
+  // L is constant here, so its rows are read as LDS broadcasts.
   const int l = lane & 15;
-  double lc[16], acc[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) lc[k] = M[(16 * b + k) * S + 16 * b + l];  // column l
+  double acc[16];
 #pragma unroll
   for (int j = 0; j < 16; ++j) acc[j] = (j == l) ? 1.0 : 0.0;
 #pragma unroll
   for (int j = 15; j >= 0; --j) {
     const double zj = acc[j];
 #pragma unroll
-    for (int jp = 0; jp < j; ++jp) acc[jp] = fma(-readlane_t(lc[j], jp), zj, acc[jp]);
+    for (int jp = 0; jp < j; ++jp) acc[jp] = fma(-M[(16 * b + j) * S + 16 * b + jp], zj, acc[jp]);
   }
   if (lane < 16) {
 #pragma unroll
@@ -263,11 +263,11 @@ __device__ __forceinline__ void lu64(double* M, double* dsg, double* X, double* 
   double* rv = tmp + 4 * 16 * TS;  // 16 reciprocals
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    if (wave == 0) {
-      lu16<S>(M, b, dsg, lane);
-      inv_upper16<S, S>(M, b, X, b, rv, lane);
-      inv_unit_lower16<S, TS>(M, b, tmp, 0, lane);
-    }
+    if (wave == 0) lu16<S>(M, b, dsg, lane);
+    __syncthreads();
+    // the two inverses of the diagonal block side by side
+    if (wave == 1) inv_upper16<S, S>(M, b, X, b, rv, lane);
+    if (wave == 2) inv_unit_lower16<S, TS>(M, b, tmp, 0, lane);
     __syncthreads();
     // L_cb = W_cb U_bb^-1 and U_bc = L_bb^-1 W_bc for c > b
     const int n = 3 - b;
