@@ -385,7 +385,9 @@ def main():
                      "traffic": update_traffic_bytes() if (m, n) == (8192, 8192) else None,
                      "algorithmic_bytes_per_launch": round(upd["bytes"] / max(1, upd["launches"])),
                      "traffic_source": "profiles/qr_factor_8192_kernels.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE)",
-                     "kernel": "gemm_kernel<double,'N','N'> (trailing update C -= V Z)",
+                     "kernel": ("rocBLAS dgemm N,N (trailing update C -= V Z)"
+                                if os.environ.get("RNPX_UPDATE_GEMM", "vendor") != "own"
+                                else "gemm_kernel<double,'N','N'> (trailing update C -= V Z)"),
                      "launches": upd["launches"], "kernel_ms_per_step": round(upd["ms"] / args.steps, 3)},
         "breakdown_ms_per_step": {
             "update_gemm": round(upd["ms"] / args.steps, 3),

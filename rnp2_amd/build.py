@@ -26,7 +26,7 @@ SOURCES = [
     "rnpx_aux.hip",
     "rnpx_batched.hip",
     "rnpx_mixed.hip",
-    "rnpx_qr.cpp",
+    "rnpx_qr.cpp", "rnpx_blas.cpp",
     "rnpx_lq.cpp",
     "rnpx_ls.cpp",
     "rnpx_serve.cpp",
@@ -76,7 +76,7 @@ def build(verbose=False, jobs=None):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocblas"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

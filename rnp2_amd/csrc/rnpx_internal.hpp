@@ -65,6 +65,7 @@ struct Handle {
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
   int update_prec = 0;                 // 1: trailing-update products on FP32 MFMA
+  void* blas = nullptr;                // rocBLAS handle of the vendor update product (lazy)
   bool prof = false;
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> event_pool;
@@ -80,6 +81,10 @@ struct Handle {
 };
 
 int hip_status(hipError_t e);
+bool vendor_update_enabled();
+int vendor_dgemm_nn(Handle* h, hipStream_t st, int m, int n, int k, double alpha, const double* A, long lda,
+                    const double* B, long ldb, double beta, double* C, long ldc);
+void vendor_release(Handle* h);
 int create_handle(Handle** out, size_t budget);
 Handle* default_handle(int* status);
 int sync_handle(Handle* h);

@@ -125,6 +125,7 @@ int Handle::ensure_partition(int cus) {
 
 Handle::~Handle() {
   if (stream) hipStreamSynchronize(stream);
+  vendor_release(this);
   if (batch && batch_free) batch_free(batch);
   for (auto& r : recs) {
     hipEventDestroy(r.a);
@@ -297,7 +298,17 @@ static int apply_block_dev(Handle* h, char side, char trans, int mc, int nc, int
     if constexpr (!S<T>::cplx) {
       if (f32) RNPX_TRY(gemm32('N', 'N', mc, nc, k, -1.0, V, ldv, Z, k, 1.0, C, ldc, 1, nullptr, st));
     }
-    if (!f32) RNPX_TRY(gemm<T>('N', 'N', mc, nc, k, mone, V, ldv, Z, k, one, C, ldc, 1, nullptr, st));
+    if (!f32) {
+      bool done = false;
+      if constexpr (!S<T>::cplx) {
+        if (vendor_update_enabled()) {
+          const int vs = vendor_dgemm_nn(h, st, mc, nc, k, -1.0, V, ldv, Z, k, 1.0, C, ldc);
+          if (vs) return vs;
+          done = true;
+        }
+      }
+      if (!done) RNPX_TRY(gemm<T>('N', 'N', mc, nc, k, mone, V, ldv, Z, k, one, C, ldc, 1, nullptr, st));
+    }
     if (gw->prof) h->prof_end(K_UPDATE, ev, fl, esz * (2.0 * mc * nc + (double)mc * k + (double)k * nc), st);
   } else {
     // W = C V (mc x k), Z = W op(T), C -= Z V^H
