@@ -121,6 +121,57 @@ int rnpx_dorglq(size_t m, size_t n, size_t k, double *a, size_t lda, const doubl
 int rnpx_zunglq(size_t m, size_t n, size_t k, rnpx_zcomplex *a, size_t lda,
                 const rnpx_zcomplex *tau, size_t *lwork, rnpx_zcomplex *work);
 
+/* QL (RNP/LA/QL.hpp): A = Q L, L ending at the bottom-right corner, the
+ * reflectors in the columns above it, Q = H[k-1] ... H[0].  RQ
+ * (RNP/LA/RQ.hpp): A = R Q, R ending at the bottom-right corner, the
+ * reflectors conjugated in the rows left of it.  Same argument lists,
+ * conventions and workspace queries as the reference: QL::Factor nb*n
+ * (QL.hpp:139-141), RQ::Factor nb*m (RQ.hpp:142-144; lwork 0 or >= m),
+ * MultQ nb*nw + nb*nb (QL.hpp:351-354, RQ.hpp:359-362), QL::GenerateQ n*nb
+ * (QL.hpp:509-512), RQ::GenerateQ m*nb (RQ.hpp:528-531), nb = 32. */
+/* QL::Factor<double> -- RNP/LA/QL.hpp:126-205 (dgeqlf). */
+int rnpx_dgeqlf(size_t m, size_t n, double *a, size_t lda, double *tau,
+                size_t *lwork, double *work);
+/* QL::Factor<std::complex<double>> -- RNP/LA/QL.hpp:126-205 (zgeqlf). */
+int rnpx_zgeqlf(size_t m, size_t n, rnpx_zcomplex *a, size_t lda, rnpx_zcomplex *tau,
+                size_t *lwork, rnpx_zcomplex *work);
+/* QL::MultQ<double> -- RNP/LA/QL.hpp:335-430 (dormql): c <- op(Q) c or c op(Q);
+ * a holds the k reflector columns (lda >= the order of Q). */
+int rnpx_dormql(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                const double *a, size_t lda, const double *tau, double *c, size_t ldc,
+                size_t *lwork, double *work);
+/* QL::MultQ<std::complex<double>> -- RNP/LA/QL.hpp:335-430 (zunmql). */
+int rnpx_zunmql(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                const rnpx_zcomplex *a, size_t lda, const rnpx_zcomplex *tau,
+                rnpx_zcomplex *c, size_t ldc, size_t *lwork, rnpx_zcomplex *work);
+/* QL::GenerateQ<double> -- RNP/LA/QL.hpp:501-565 (dorgql): the last n columns of Q. */
+int rnpx_dorgql(size_t m, size_t n, size_t k, double *a, size_t lda, const double *tau,
+                size_t *lwork, double *work);
+/* QL::GenerateQ<std::complex<double>> -- RNP/LA/QL.hpp:501-565 (zungql). */
+int rnpx_zungql(size_t m, size_t n, size_t k, rnpx_zcomplex *a, size_t lda,
+                const rnpx_zcomplex *tau, size_t *lwork, rnpx_zcomplex *work);
+/* RQ::Factor<double> -- RNP/LA/RQ.hpp:131-205 (dgerqf). */
+int rnpx_dgerqf(size_t m, size_t n, double *a, size_t lda, double *tau,
+                size_t *lwork, double *work);
+/* RQ::Factor<std::complex<double>> -- RNP/LA/RQ.hpp:131-205 (zgerqf). */
+int rnpx_zgerqf(size_t m, size_t n, rnpx_zcomplex *a, size_t lda, rnpx_zcomplex *tau,
+                size_t *lwork, rnpx_zcomplex *work);
+/* RQ::MultQ<double> -- RNP/LA/RQ.hpp:344-440 (dormrq): c <- op(Q) c or c op(Q);
+ * a holds the k reflector rows. */
+int rnpx_dormrq(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                const double *a, size_t lda, const double *tau, double *c, size_t ldc,
+                size_t *lwork, double *work);
+/* RQ::MultQ<std::complex<double>> -- RNP/LA/RQ.hpp:344-440 (zunmrq). */
+int rnpx_zunmrq(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                const rnpx_zcomplex *a, size_t lda, const rnpx_zcomplex *tau,
+                rnpx_zcomplex *c, size_t ldc, size_t *lwork, rnpx_zcomplex *work);
+/* RQ::GenerateQ<double> -- RNP/LA/RQ.hpp:519-580 (dorgrq): the last m rows of Q. */
+int rnpx_dorgrq(size_t m, size_t n, size_t k, double *a, size_t lda, const double *tau,
+                size_t *lwork, double *work);
+/* RQ::GenerateQ<std::complex<double>> -- RNP/LA/RQ.hpp:519-580 (zungrq). */
+int rnpx_zungrq(size_t m, size_t n, size_t k, rnpx_zcomplex *a, size_t lda,
+                const rnpx_zcomplex *tau, size_t *lwork, rnpx_zcomplex *work);
+
 /* ---------------------------------------------------------------------
  * Handles (device-resident work on one HIP stream).
  * ------------------------------------------------------------------- */
@@ -155,6 +206,14 @@ int rnpx_zgeqrf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex *a, size_t 
 /* LQ::Factor on device pointers, asynchronous (RNP/LA/LQ.hpp:126-190). */
 int rnpx_dgelqf_dev(rnpx_handle h, size_t m, size_t n, double *a, size_t lda, double *tau);
 int rnpx_zgelqf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex *a, size_t lda,
+                    rnpx_zcomplex *tau);
+/* QL::Factor and RQ::Factor on device pointers, asynchronous
+ * (RNP/LA/QL.hpp:126-205, RNP/LA/RQ.hpp:131-205). */
+int rnpx_dgeqlf_dev(rnpx_handle h, size_t m, size_t n, double *a, size_t lda, double *tau);
+int rnpx_zgeqlf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex *a, size_t lda,
+                    rnpx_zcomplex *tau);
+int rnpx_dgerqf_dev(rnpx_handle h, size_t m, size_t n, double *a, size_t lda, double *tau);
+int rnpx_zgerqf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex *a, size_t lda,
                     rnpx_zcomplex *tau);
 int rnpx_dormqr_dev(rnpx_handle h, const char *side, const char *trans, size_t m, size_t n,
                     size_t k, const double *a, size_t lda, const double *tau, double *c,

@@ -80,6 +80,12 @@ def _declare(L):
                                                       _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]
         getattr(L, f"orc_{pre}_lq_generateq").argtypes = [_sz, _sz, _sz, _p, _sz, _p,
                                                           ctypes.POINTER(_sz), _p]
+        for fam in ("ql", "rq"):
+            getattr(L, f"orc_{pre}_{fam}_factor").argtypes = [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]
+            getattr(L, f"orc_{pre}_{fam}_multq").argtypes = [ctypes.c_char_p, ctypes.c_char_p, _sz, _sz, _sz,
+                                                             _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]
+            getattr(L, f"orc_{pre}_{fam}_generateq").argtypes = [_sz, _sz, _sz, _p, _sz, _p,
+                                                                 ctypes.POINTER(_sz), _p]
         getattr(L, f"orc_{pre}_gemm").argtypes = [ctypes.c_char_p, ctypes.c_char_p, _sz, _sz, _sz,
                                                   scal, _p, _sz, _p, _sz, scal, _p, _sz]
         getattr(L, f"orc_{pre}_generate_vector").argtypes = [ctypes.c_int, _sz, _p, _p]
@@ -372,6 +378,114 @@ def lq_generateq(afac, tau, m=None, k=None):
     work = np.zeros(max(lw.value, m, 1), dtype=a.dtype)
     f(m, n, k, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), _ptr(work))
     return a
+
+
+def _fam_factor(fam, a):
+    a = _fortran(a, np.result_type(a, np.float64))
+    pre = _pre(a.dtype)
+    m, n = a.shape
+    k = min(m, n)
+    tau = np.zeros(max(k, 1), dtype=a.dtype)
+    f = getattr(lib(), f"orc_{pre}_{fam}_factor")
+    lw = _sz(0)
+    f(m, n, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), None)
+    work = np.zeros(max(lw.value, m, n, 1), dtype=a.dtype)
+    f(m, n, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), _ptr(work))
+    return a, tau[:k]
+
+
+def _fam_query(fam, what, *args, dtype=np.float64):
+    lw = _sz(0)
+    f = getattr(lib(), f"orc_{_pre(dtype)}_{fam}_{what}")
+    if what == "factor":
+        m, n = args
+        f(m, n, None, max(m, 1), None, ctypes.byref(lw), None)
+    elif what == "multq":
+        side, trans, m, n, k = args
+        f(side.encode(), trans.encode(), m, n, k, None, max(k, 1), None, None, max(m, 1), ctypes.byref(lw), None)
+    else:
+        m, n, k = args
+        f(m, n, k, None, max(m, 1), None, ctypes.byref(lw), None)
+    return lw.value
+
+
+def _fam_multq(fam, side, trans, afac, tau, c, k=None):
+    c = _fortran(c, np.result_type(c, afac, np.float64))
+    pre = _pre(c.dtype)
+    afac = _fortran(afac, c.dtype)
+    tau = np.ascontiguousarray(tau, dtype=c.dtype)
+    m, n = c.shape
+    if k is None:
+        k = tau.size
+    lda = afac.shape[0]
+    f = getattr(lib(), f"orc_{pre}_{fam}_multq")
+    lw = _sz(0)
+    f(side.encode(), trans.encode(), m, n, k, _ptr(afac), lda, _ptr(tau), _ptr(c), max(m, 1),
+      ctypes.byref(lw), None)
+    work = np.zeros(max(lw.value, m, n, 1), dtype=c.dtype)
+    f(side.encode(), trans.encode(), m, n, k, _ptr(afac), lda, _ptr(tau), _ptr(c), max(m, 1),
+      ctypes.byref(lw), _ptr(work))
+    return c
+
+
+def _fam_generateq(fam, a, tau, k=None):
+    a = _fortran(a, np.result_type(a, np.float64))
+    pre = _pre(a.dtype)
+    m, n = a.shape
+    tau = np.ascontiguousarray(tau, dtype=a.dtype)
+    if k is None:
+        k = tau.size
+    f = getattr(lib(), f"orc_{pre}_{fam}_generateq")
+    lw = _sz(0)
+    f(m, n, k, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), None)
+    work = np.zeros(max(lw.value, m, n, 1), dtype=a.dtype)
+    f(m, n, k, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), _ptr(work))
+    return a
+
+
+def ql_factor(a):
+    """QL::Factor (QL.hpp:126-205): returns (afac, tau); L in the lower
+    triangle ending at the bottom-right corner, the reflectors above it."""
+    return _fam_factor("ql", a)
+
+
+def ql_multq(side, trans, afac, tau, c, k=None):
+    """QL::MultQ (QL.hpp:335-430): op(Q) C or C op(Q); afac holds the k
+    reflector columns (the last k columns of the factored matrix)."""
+    return _fam_multq("ql", side, trans, afac, tau, c, k)
+
+
+def ql_generateq(a, tau, k=None):
+    """QL::GenerateQ (QL.hpp:501-565): a (m x n, m >= n, reflectors in its
+    last k columns) becomes the last n columns of Q."""
+    return _fam_generateq("ql", a, tau, k)
+
+
+def rq_factor(a):
+    """RQ::Factor (RQ.hpp:131-205): returns (afac, tau); R in the upper
+    triangle ending at the bottom-right corner, the conjugated reflectors
+    in the rows left of it."""
+    return _fam_factor("rq", a)
+
+
+def rq_multq(side, trans, afac, tau, c, k=None):
+    """RQ::MultQ (RQ.hpp:344-440): op(Q) C or C op(Q); afac holds the k
+    reflector rows (the last k rows of the factored matrix)."""
+    return _fam_multq("rq", side, trans, afac, tau, c, k)
+
+
+def rq_generateq(a, tau, k=None):
+    """RQ::GenerateQ (RQ.hpp:519-580): a (m x n, m <= n, reflectors in its
+    last k rows) becomes the last m rows of Q."""
+    return _fam_generateq("rq", a, tau, k)
+
+
+def ql_query(what, *args, dtype=np.float64):
+    return _fam_query("ql", what, *args, dtype=dtype)
+
+
+def rq_query(what, *args, dtype=np.float64):
+    return _fam_query("rq", what, *args, dtype=dtype)
 
 
 def ls_solve(a, b):

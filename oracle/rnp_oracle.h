@@ -135,6 +135,35 @@ void orc_d_lq_generateq(size_t m, size_t n, size_t k, double *a, size_t lda, con
 void orc_z_lq_generateq(size_t m, size_t n, size_t k, orc_z *a, size_t lda, const orc_z *tau,
                         size_t *lwork, orc_z *work);
 
+/* ---- QL (RNP/LA/QL.hpp) and RQ (RNP/LA/RQ.hpp) ----
+ *   QL::Factor_unblocked  QL.hpp:79-100,   QL::Factor    QL.hpp:126-205
+ *   QL::MultQ_unblocked   QL.hpp:245-300,  QL::MultQ     QL.hpp:335-430
+ *   QL::GenerateQ_unblocked QL.hpp:445-471, QL::GenerateQ QL.hpp:501-565
+ *   RQ::Factor_unblocked  RQ.hpp:80-102,   RQ::Factor    RQ.hpp:131-205
+ *   RQ::MultQ_unblocked   RQ.hpp:245-303,  RQ::MultQ     RQ.hpp:344-440
+ *   RQ::GenerateQ_unblocked RQ.hpp:457-491, RQ::GenerateQ RQ.hpp:519-580
+ *   Reflector::Apply vone == -1 branches, Reflector.hpp:297-357 */
+void orc_d_ql_factor(size_t m, size_t n, double *a, size_t lda, double *tau, size_t *lwork, double *work);
+void orc_d_ql_multq(const char *side, const char *trans, size_t m, size_t n, size_t k, const double *a,
+                    size_t lda, const double *tau, double *c, size_t ldc, size_t *lwork, double *work);
+void orc_d_ql_generateq(size_t m, size_t n, size_t k, double *a, size_t lda, const double *tau,
+                        size_t *lwork, double *work);
+void orc_d_rq_factor(size_t m, size_t n, double *a, size_t lda, double *tau, size_t *lwork, double *work);
+void orc_d_rq_multq(const char *side, const char *trans, size_t m, size_t n, size_t k, const double *a,
+                    size_t lda, const double *tau, double *c, size_t ldc, size_t *lwork, double *work);
+void orc_d_rq_generateq(size_t m, size_t n, size_t k, double *a, size_t lda, const double *tau,
+                        size_t *lwork, double *work);
+void orc_z_ql_factor(size_t m, size_t n, orc_z *a, size_t lda, orc_z *tau, size_t *lwork, orc_z *work);
+void orc_z_ql_multq(const char *side, const char *trans, size_t m, size_t n, size_t k, const orc_z *a,
+                    size_t lda, const orc_z *tau, orc_z *c, size_t ldc, size_t *lwork, orc_z *work);
+void orc_z_ql_generateq(size_t m, size_t n, size_t k, orc_z *a, size_t lda, const orc_z *tau,
+                        size_t *lwork, orc_z *work);
+void orc_z_rq_factor(size_t m, size_t n, orc_z *a, size_t lda, orc_z *tau, size_t *lwork, orc_z *work);
+void orc_z_rq_multq(const char *side, const char *trans, size_t m, size_t n, size_t k, const orc_z *a,
+                    size_t lda, const orc_z *tau, orc_z *c, size_t ldc, size_t *lwork, orc_z *work);
+void orc_z_rq_generateq(size_t m, size_t n, size_t k, orc_z *a, size_t lda, const orc_z *tau,
+                        size_t *lwork, orc_z *work);
+
 /* Least squares min ||A x - B|| for m >= n by Factor, MultQ("L","C") and an
  * upper-triangular back substitution with R (BLAS dtrsm "L","U","N","N";
  * the composition of QR.hpp:129, QR.hpp:337 and BLAS_mix.hpp:2096).

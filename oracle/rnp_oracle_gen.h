@@ -762,6 +762,278 @@ void FN(lq_generateq)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *t
 	FN(lq_generateq_unblocked)(m, n, k, a, lda, tau, work);
 }
 
+/* ---------------------------------------------------------------------
+ * QL (RNP/LA/QL.hpp) and RQ (RNP/LA/RQ.hpp): reflectors generated from the
+ * last column (row) backwards, the implied 1 at the bottom (right) end of
+ * each vector.
+ * ------------------------------------------------------------------- */
+
+/* Reflector::Apply with vone == -1 (Reflector.hpp:297-357): v's last entry
+ * is an implied 1 (the stored value there is only read by the trailing-zero
+ * scan).  vconj == 0: H = I - tau v v^H with v as stored (QL::MultQ_unblocked,
+ * QL.hpp:272, 287).  vconj == 1: v = conj(stored) (RQ::MultQ_unblocked,
+ * RQ.hpp:279, 295). */
+static void FN(apply_bottom)(const char *side, int vconj, size_t m, size_t n, const T *v, size_t incv, T tau,
+                             T *c, size_t ldc, T *work){
+	size_t lenv = 0, lenc = 0;
+	const int left = ('L' == side[0]);
+	if(0.0 != tau){
+		lenv = left ? m : n;
+		if(0 == lenv){ return; }
+		size_t i = (lenv-1)*incv;
+		while(lenv > 0 && 0.0 == v[i]){
+			--lenv;
+			i -= incv;
+		}
+		if(left){
+			lenc = FN(last_nonzero_column_limit)(lenv, n, c, ldc);
+		}else{
+			lenc = FN(last_nonzero_row_limit)(m, lenv, c, ldc);
+		}
+		if(0 == lenc){ return; }
+	}
+	if(left){
+		const size_t r = lenv-1;  /* the implied-1 row */
+		for(size_t j = 0; j < lenc; ++j){
+			T w = 0;
+			if(vconj){
+				for(size_t i = 0; i < r; ++i){ w += c[i+j*ldc] * v[i*incv]; }
+				w = CONJ(w + c[r+j*ldc]);
+			}else{
+				w = CONJ(c[r+j*ldc]);
+				for(size_t i = 0; i < r; ++i){ w += CONJ(c[i+j*ldc]) * v[i*incv]; }
+			}
+			work[j] = w;
+		}
+		for(size_t j = 0; j < lenc; ++j){ c[r+j*ldc] += -tau * CONJ(work[j]); }
+		for(size_t j = 0; j < lenc; ++j){
+			for(size_t i = 0; i < r; ++i){
+				const T vi = vconj ? CONJ(v[i*incv]) : v[i*incv];
+				c[i+j*ldc] -= tau * vi * CONJ(work[j]);
+			}
+		}
+	}else{
+		const size_t r = lenv-1;  /* the implied-1 column */
+		for(size_t i = 0; i < lenc; ++i){ work[i] = c[i+r*ldc]; }
+		for(size_t j = 0; j < r; ++j){
+			const T vj = vconj ? CONJ(v[j*incv]) : v[j*incv];
+			for(size_t i = 0; i < lenc; ++i){ work[i] += c[i+j*ldc] * vj; }
+		}
+		for(size_t i = 0; i < lenc; ++i){ c[i+r*ldc] += -tau * work[i]; }
+		for(size_t j = 0; j < r; ++j){
+			const T vj = vconj ? v[j*incv] : CONJ(v[j*incv]);  /* v^H entries */
+			for(size_t i = 0; i < lenc; ++i){ c[i+j*ldc] += -tau * work[i] * vj; }
+		}
+	}
+}
+
+/* QL::Factor_unblocked, QL.hpp:79-100. */
+void FN(ql_factor_unblocked)(size_t m, size_t n, T *a, size_t lda, T *tau, T *work){
+	const size_t k = m < n ? m : n;
+	size_t i = k;
+	while(i-- > 0){
+		const size_t r = m-k+i, cc = n-k+i;
+		T alpha = a[r+cc*lda];
+		FN(generate)(r+1, &alpha, &a[0+cc*lda], 1, &tau[i]);
+		a[r+cc*lda] = 1;
+		FN(apply)("L", 0, r+1, cc, &a[0+cc*lda], 1, CONJ(tau[i]), a, lda, work);
+		a[r+cc*lda] = alpha;
+	}
+}
+
+/* QL::Factor, QL.hpp:126-205: the checks (lda >= m, lwork given) and the
+ * workspace query (nb n, nb = QL::Tuning default 32) as there; the factors
+ * by the unblocked loop for all sizes (the blocked loop, QL.hpp:175-196,
+ * computes the same factors in exact arithmetic). */
+void FN(ql_factor)(size_t m, size_t n, T *a, size_t lda, T *tau, size_t *lwork, T *work){
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(NULL == lwork){ orc_assert_fail("NULL != lwork", __LINE__); return; }
+	const size_t k = m < n ? m : n;
+	if(0 == k){ return; }
+	if(NULL == work || 0 == *lwork){
+		*lwork = 32*n;
+		return;
+	}
+	FN(ql_factor_unblocked)(m, n, a, lda, tau, work);
+}
+
+/* QL::MultQ_unblocked, QL.hpp:245-300: forward loop for (L,N) and (R,C),
+ * backward otherwise; tau conjugated for "C".  Reflector i acts on the
+ * leading nq-k+i+1 rows (columns) of C. */
+void FN(ql_multq_unblocked)(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                            const T *a, size_t lda, const T *tau, T *c, size_t ldc, T *work){
+	if(!(ldc >= m)){ orc_assert_fail("ldc >= m", __LINE__); }
+	const int left = ('L' == side[0]);
+	const int notran = ('N' == trans[0]);
+	if(!((left && k <= m) || (!left && k <= n))){ orc_assert_fail("k range", __LINE__); }
+	if(!(lda >= k)){ orc_assert_fail("lda >= k", __LINE__); }
+	if(m < 1 || n < 1 || k < 1){ return; }
+	const int forward = ((left && notran) || (!left && !notran));
+	for(size_t s = 0; s < k; ++s){
+		const size_t i = forward ? s : k-1-s;
+		const size_t mi = left ? m-k+i+1 : m, ni = left ? n : n-k+i+1;
+		const T taui = notran ? tau[i] : CONJ(tau[i]);
+		FN(apply_bottom)(side, 0, mi, ni, &a[0+i*lda], 1, taui, c, ldc, work);
+	}
+}
+
+/* QL::MultQ, QL.hpp:335-430: checks and the workspace query (nb nw + nb^2);
+ * the product by the unblocked loop. */
+void FN(ql_multq)(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                  const T *a, size_t lda, const T *tau, T *c, size_t ldc, size_t *lwork, T *work){
+	if(!(ldc >= m)){ orc_assert_fail("ldc >= m", __LINE__); }
+	if(0 == m || 0 == n || 0 == k){ return; }
+	const int left = ('L' == side[0]);
+	const size_t nw = left ? n : m;
+	if(0 == *lwork || NULL == work){
+		*lwork = 32*nw + 32*32;
+		return;
+	}
+	FN(ql_multq_unblocked)(side, trans, m, n, k, a, lda, tau, c, ldc, work);
+}
+
+/* QL::GenerateQ_unblocked, QL.hpp:445-471: the last n columns of
+ * H[k-1] ... H[0]. */
+void FN(ql_generateq_unblocked)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *tau, T *work){
+	if(!(m >= n)){ orc_assert_fail("m >= n", __LINE__); }
+	if(!(k <= n)){ orc_assert_fail("k <= n", __LINE__); }
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(n < 1){ return; }
+	for(size_t j = 0; j+k < n; ++j){
+		for(size_t i = 0; i < m; ++i){ a[i+j*lda] = 0; }
+		a[m-n+j+j*lda] = 1;
+	}
+	for(size_t i = 0; i < k; ++i){
+		const size_t ii = n-k+i, r = m-n+ii;
+		a[r+ii*lda] = 1;
+		FN(apply)("L", 0, r+1, ii, &a[0+ii*lda], 1, tau[i], a, lda, work);
+		for(size_t l = 0; l < r; ++l){ a[l+ii*lda] *= -tau[i]; }
+		a[r+ii*lda] = 1.0 - tau[i];
+		for(size_t l = r+1; l < m; ++l){ a[l+ii*lda] = 0; }
+	}
+}
+
+/* QL::GenerateQ, QL.hpp:501-565: checks and the workspace query (n nb, nb
+ * from QR::Tuning as there); Q by the unblocked loop. */
+void FN(ql_generateq)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *tau, size_t *lwork, T *work){
+	if(!(k <= n)){ orc_assert_fail("k <= n", __LINE__); }
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(0 == n){ return; }
+	if(0 == *lwork || NULL == work){
+		*lwork = n*32;
+		return;
+	}
+	FN(ql_generateq_unblocked)(m, n, k, a, lda, tau, work);
+}
+
+/* RQ::Factor_unblocked, RQ.hpp:80-102: row m-k+i conjugated, its reflector
+ * generated with the 1 at column n-k+i, applied from the right to the rows
+ * above, the stored part conjugated back. */
+void FN(rq_factor_unblocked)(size_t m, size_t n, T *a, size_t lda, T *tau, T *work){
+	const size_t k = m < n ? m : n;
+	size_t i = k;
+	while(i-- > 0){
+		const size_t r = m-k+i, cc = n-k+i;
+		FN(conjugate)(cc+1, &a[r+0*lda], lda);
+		T alpha = a[r+cc*lda];
+		FN(generate)(cc+1, &alpha, &a[r+0*lda], lda, &tau[i]);
+		a[r+cc*lda] = 1;
+		FN(apply)("R", 0, r, cc+1, &a[r+0*lda], lda, tau[i], a, lda, work);
+		a[r+cc*lda] = alpha;
+		FN(conjugate)(cc, &a[r+0*lda], lda);
+	}
+}
+
+/* RQ::Factor, RQ.hpp:131-205: checks (lda >= m, lwork given, 0 or >= m) and
+ * the workspace query (nb m, nb = RQ::Tuning default 32); the unblocked
+ * loop for all sizes. */
+void FN(rq_factor)(size_t m, size_t n, T *a, size_t lda, T *tau, size_t *lwork, T *work){
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(NULL == lwork){ orc_assert_fail("NULL != lwork", __LINE__); return; }
+	if(!(0 == *lwork || *lwork >= m)){ orc_assert_fail("0 == *lwork || *lwork >= m", __LINE__); }
+	const size_t k = m < n ? m : n;
+	if(0 == k){ return; }
+	if(NULL == work || 0 == *lwork){
+		*lwork = 32*m;
+		return;
+	}
+	FN(rq_factor_unblocked)(m, n, a, lda, tau, work);
+}
+
+/* RQ::MultQ_unblocked, RQ.hpp:245-303: forward loop for (L,C) and (R,N),
+ * backward otherwise; tau conjugated for "N"; the stored rows conjugated
+ * (vconj). */
+void FN(rq_multq_unblocked)(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                            const T *a, size_t lda, const T *tau, T *c, size_t ldc, T *work){
+	if(!(ldc >= m)){ orc_assert_fail("ldc >= m", __LINE__); }
+	const int left = ('L' == side[0]);
+	const int notran = ('N' == trans[0]);
+	if(!((left && k <= m) || (!left && k <= n))){ orc_assert_fail("k range", __LINE__); }
+	if(!(lda >= k)){ orc_assert_fail("lda >= k", __LINE__); }
+	if(m < 1 || n < 1 || k < 1){ return; }
+	const int forward = ((left && !notran) || (!left && notran));
+	for(size_t s = 0; s < k; ++s){
+		const size_t i = forward ? s : k-1-s;
+		const size_t mi = left ? m-k+i+1 : m, ni = left ? n : n-k+i+1;
+		const T taui = notran ? CONJ(tau[i]) : tau[i];
+		FN(apply_bottom)(side, 1, mi, ni, &a[i+0*lda], lda, taui, c, ldc, work);
+	}
+}
+
+/* RQ::MultQ, RQ.hpp:344-440: checks and the workspace query (nb nw + nb^2);
+ * the product by the unblocked loop. */
+void FN(rq_multq)(const char *side, const char *trans, size_t m, size_t n, size_t k,
+                  const T *a, size_t lda, const T *tau, T *c, size_t ldc, size_t *lwork, T *work){
+	if(!(ldc >= m)){ orc_assert_fail("ldc >= m", __LINE__); }
+	if(0 == m || 0 == n || 0 == k){ return; }
+	const int left = ('L' == side[0]);
+	const size_t nw = left ? n : m;
+	if(0 == *lwork || NULL == work){
+		*lwork = 32*nw + 32*32;
+		return;
+	}
+	FN(rq_multq_unblocked)(side, trans, m, n, k, a, lda, tau, c, ldc, work);
+}
+
+/* RQ::GenerateQ_unblocked, RQ.hpp:457-491: the last m rows of
+ * H[0]^H ... H[k-1]^H. */
+void FN(rq_generateq_unblocked)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *tau, T *work){
+	if(!(n >= m)){ orc_assert_fail("n >= m", __LINE__); }
+	if(!(k <= m)){ orc_assert_fail("k <= m", __LINE__); }
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(0 == m){ return; }
+	if(k < m){
+		for(size_t j = 0; j < n; ++j){
+			for(size_t i = 0; i+k < m; ++i){ a[i+j*lda] = 0; }
+			if(j >= n-m && j < n-k){ a[m-n+j+j*lda] = 1; }
+		}
+	}
+	for(size_t i = 0; i < k; ++i){
+		const size_t ii = m-k+i, cc = n-m+ii;
+		FN(conjugate)(cc, &a[ii+0*lda], lda);
+		a[ii+cc*lda] = 1;
+		FN(apply)("R", 0, ii, cc+1, &a[ii+0*lda], lda, CONJ(tau[i]), a, lda, work);
+		for(size_t l = 0; l < cc; ++l){ a[ii+l*lda] *= -tau[i]; }
+		FN(conjugate)(cc, &a[ii+0*lda], lda);
+		a[ii+cc*lda] = 1.0 - CONJ(tau[i]);
+		for(size_t l = cc+1; l < n; ++l){ a[ii+l*lda] = 0; }
+	}
+}
+
+/* RQ::GenerateQ, RQ.hpp:519-580: checks and the workspace query (m nb, nb
+ * from QR::Tuning as there); Q by the unblocked loop. */
+void FN(rq_generateq)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *tau, size_t *lwork, T *work){
+	if(!(n >= m)){ orc_assert_fail("n >= m", __LINE__); }
+	if(!(k <= m)){ orc_assert_fail("k <= m", __LINE__); }
+	if(!(lda >= m)){ orc_assert_fail("lda >= m", __LINE__); }
+	if(0 == m){ return; }
+	if(0 == *lwork || NULL == work){
+		*lwork = m*32;
+		return;
+	}
+	FN(rq_generateq_unblocked)(m, n, k, a, lda, tau, work);
+}
+
 /* Least squares by QR: Factor (QR.hpp:129), MultQ("L","C") on B
  * (QR.hpp:337), then R x = (Q^H B)(0:n,:) by back substitution
  * (SolveTrM "L","U","N","N", BLAS_mix.hpp:2096). */

@@ -77,6 +77,18 @@ SIGNATURES = {
     "rnpx_zunmlq": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
     "rnpx_dorglq": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
     "rnpx_zunglq": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_dgeqlf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_dormql": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "rnpx_dorgql": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_dgerqf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_dormrq": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "rnpx_dorgrq": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_zgeqlf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_zunmql": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "rnpx_zungql": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_zgerqf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_zunmrq": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "rnpx_zungrq": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
     "rnpx_create": (_i, [ctypes.POINTER(_p), _sz]),
     "rnpx_destroy": (_i, [_p]),
     "rnpx_set_stream": (_i, [_p, _p]),
@@ -92,6 +104,10 @@ SIGNATURES = {
     "rnpx_zgeqrf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_dgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_zgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_dgeqlf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_zgeqlf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_dgerqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_zgerqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_dormqr_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
     "rnpx_zunmqr_dev": (_i, [_p, _s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz]),
     "rnpx_dorgqr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p]),
@@ -324,6 +340,91 @@ def lq_generateq(afac, tau, m=None, k=None):
     return a
 
 
+_FAM = {"ql": ("QL", ("geqlf", "ormql", "orgql"), ("geqlf", "unmql", "ungql")),
+        "rq": ("RQ", ("gerqf", "ormrq", "orgrq"), ("gerqf", "unmrq", "ungrq"))}
+
+
+def _fam_fn(fam, which, pre):
+    name, dn, zn = _FAM[fam]
+    return getattr(lib(), "rnpx_" + pre + (dn if pre == "d" else zn)[which]), name
+
+
+def _fam_factor(fam, a):
+    a = np.array(a, dtype=np.result_type(a, np.float64), order="F", copy=True)
+    m, n = a.shape
+    k = min(m, n)
+    tau = np.zeros(max(k, 1), dtype=a.dtype)
+    fn, name = _fam_fn(fam, 0, _pre(a.dtype))
+    lw = _sz(0)
+    _check(fn(m, n, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), None), f"{name}::Factor query")
+    work = np.zeros(max(lw.value, m, 1), dtype=a.dtype)
+    _check(fn(m, n, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), _ptr(work)), f"{name}::Factor")
+    return a, tau[:k]
+
+
+def _fam_multq(fam, side, trans, afac, tau, c, k=None):
+    c = np.array(c, dtype=np.result_type(c, afac, np.float64), order="F", copy=True)
+    afac = np.asfortranarray(afac, dtype=c.dtype)
+    tau = np.ascontiguousarray(tau, dtype=c.dtype)
+    m, n = c.shape
+    k = tau.size if k is None else k
+    fn, name = _fam_fn(fam, 1, _pre(c.dtype))
+    lw = _sz(0)
+    _check(fn(side.encode(), trans.encode(), m, n, k, _ptr(afac), afac.shape[0], _ptr(tau), _ptr(c), max(m, 1),
+              ctypes.byref(lw), None), f"{name}::MultQ query")
+    work = np.zeros(max(lw.value, 1), dtype=c.dtype)
+    _check(fn(side.encode(), trans.encode(), m, n, k, _ptr(afac), afac.shape[0], _ptr(tau), _ptr(c), max(m, 1),
+              ctypes.byref(lw), _ptr(work)), f"{name}::MultQ")
+    return c
+
+
+def _fam_generateq(fam, a, tau, k=None):
+    a = np.array(a, dtype=np.result_type(a, np.float64), order="F", copy=True)
+    m, n = a.shape
+    tau = np.ascontiguousarray(tau, dtype=a.dtype)
+    k = tau.size if k is None else k
+    fn, name = _fam_fn(fam, 2, _pre(a.dtype))
+    lw = _sz(0)
+    _check(fn(m, n, k, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), None), f"{name}::GenerateQ query")
+    work = np.zeros(max(lw.value, 1), dtype=a.dtype)
+    _check(fn(m, n, k, _ptr(a), max(m, 1), _ptr(tau), ctypes.byref(lw), _ptr(work)), f"{name}::GenerateQ")
+    return a
+
+
+def ql_factor(a):
+    """QL-factor a numpy matrix (QL::Factor): returns (L and the reflector
+    columns packed, tau)."""
+    return _fam_factor("ql", a)
+
+
+def ql_multq(side, trans, afac, tau, c, k=None):
+    """QL::MultQ: op(Q) C or C op(Q); afac holds the k reflector columns."""
+    return _fam_multq("ql", side, trans, afac, tau, c, k)
+
+
+def ql_generateq(a, tau, k=None):
+    """QL::GenerateQ: a (m x n, m >= n, reflectors in its last k columns)
+    becomes the last n columns of Q."""
+    return _fam_generateq("ql", a, tau, k)
+
+
+def rq_factor(a):
+    """RQ-factor a numpy matrix (RQ::Factor): returns (R and the conjugated
+    reflector rows packed, tau)."""
+    return _fam_factor("rq", a)
+
+
+def rq_multq(side, trans, afac, tau, c, k=None):
+    """RQ::MultQ: op(Q) C or C op(Q); afac holds the k reflector rows."""
+    return _fam_multq("rq", side, trans, afac, tau, c, k)
+
+
+def rq_generateq(a, tau, k=None):
+    """RQ::GenerateQ: a (m x n, m <= n, reflectors in its last k rows)
+    becomes the last m rows of Q."""
+    return _fam_generateq("rq", a, tau, k)
+
+
 def version():
     return lib().rnpx_version().decode()
 
@@ -427,6 +528,18 @@ class Handle:
         pre = _tpre(a)
         _check(getattr(lib(), f"rnpx_{pre}gelqf_dev")(self.h, m, n, _dptr(a), _ld(a), _dptr(tau)),
                "gelqf_dev")
+
+    def geqlf(self, a, tau):
+        m, n = a.shape
+        pre = _tpre(a)
+        _check(getattr(lib(), f"rnpx_{pre}geqlf_dev")(self.h, m, n, _dptr(a), _ld(a), _dptr(tau)),
+               "geqlf_dev")
+
+    def gerqf(self, a, tau):
+        m, n = a.shape
+        pre = _tpre(a)
+        _check(getattr(lib(), f"rnpx_{pre}gerqf_dev")(self.h, m, n, _dptr(a), _ld(a), _dptr(tau)),
+               "gerqf_dev")
 
     def ormqr(self, side, trans, a, tau, c, k=None):
         m, n = c.shape

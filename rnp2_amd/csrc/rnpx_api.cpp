@@ -227,6 +227,106 @@ int dropin_orglq(size_t m, size_t n, size_t k, T* a, size_t lda, const T* tau, s
   return sync_handle(h);
 }
 
+// QL and RQ drop-ins (RNP/LA/QL.hpp, RNP/LA/RQ.hpp): argument checks and
+// workspace queries as the reference's QL::Factor (QL.hpp:132-141: nb n),
+// RQ::Factor (RQ.hpp:134-144: nb m, lwork 0 or >= m), MultQ (QL.hpp:342-354,
+// RQ.hpp:349-362: nb nw + nb^2) and GenerateQ (QL.hpp:505-512: n nb;
+// RQ.hpp:523-531: m nb).
+enum class Fam { QL, RQ };
+
+template <typename T>
+int dropin_gexxf(Fam f, size_t m, size_t n, T* a, size_t lda, T* tau, size_t* lwork) {
+  if (lda < m || lda == 0) return -4;
+  if (!lwork) return -6;
+  if (f == Fam::RQ && !(*lwork == 0 || *lwork >= m)) return -6;
+  const size_t k = m < n ? m : n;
+  if (k == 0) return RNPX_OK;
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU;
+  if ((st = A.in(h, h->stage1, a, m, n, lda, true))) return st;
+  if ((st = TAU.in(h, h->stage2, tau, k, 1, k, false))) return st;
+  st = f == Fam::QL ? geqlf_dev<T>(h, (int)m, (int)n, A.dev, A.ld, TAU.dev)
+                    : gerqf_dev<T>(h, (int)m, (int)n, A.dev, A.ld, TAU.dev);
+  if (st) return st;
+  if ((st = A.out(h, lda))) return st;
+  if ((st = TAU.out(h, k))) return st;
+  return sync_handle(h);
+}
+
+template <typename T>
+int dropin_ormxx(Fam f, const char* side, const char* trans, size_t m, size_t n, size_t k, const T* a, size_t lda,
+                 const T* tau, T* c, size_t ldc, size_t* lwork, bool query) {
+  if (!side || !trans) return !side ? -1 : -2;
+  if (ldc < m || ldc == 0) return -10;
+  if (m == 0 || n == 0 || k == 0) return RNPX_OK;
+  const bool left = ('L' == side[0]);
+  if (side[0] != 'L' && side[0] != 'R') return -1;
+  if (trans[0] != 'N' && trans[0] != 'C' && trans[0] != 'T') return -2;
+  if ((left && k > m) || (!left && k > n)) return -5;
+  const size_t nq = left ? m : n, nw = left ? n : m;
+  if (lda < k || (f == Fam::QL && lda < nq)) return -7;
+  if (!lwork) return -11;
+  if (query) {
+    *lwork = kRefNb * nw + kRefNb * kRefNb;
+    return RNPX_OK;
+  }
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU, C;
+  const size_t ar = f == Fam::QL ? nq : k, ac = f == Fam::QL ? k : nq;
+  if ((st = A.in(h, h->stage1, const_cast<T*>(a), ar, ac, lda, true))) return st;
+  if ((st = TAU.in(h, h->stage2, const_cast<T*>(tau), k, 1, k, true))) return st;
+  if ((st = C.in(h, h->stage3, c, m, n, ldc, true))) return st;
+  st = f == Fam::QL
+           ? ormql_dev<T>(h, side[0], trans[0], (int)m, (int)n, (int)k, A.dev, A.ld, TAU.dev, C.dev, C.ld)
+           : ormrq_dev<T>(h, side[0], trans[0], (int)m, (int)n, (int)k, A.dev, A.ld, TAU.dev, C.dev, C.ld);
+  if (st) return st;
+  if ((st = C.out(h, ldc))) return st;
+  return sync_handle(h);
+}
+
+template <typename T>
+int dropin_orgxx(Fam f, size_t m, size_t n, size_t k, T* a, size_t lda, const T* tau, size_t* lwork, bool query) {
+  const bool ql = f == Fam::QL;
+  if (ql ? k > n : k > m) return -3;
+  if (lda < m || lda == 0) return -5;
+  if (ql ? n == 0 : m == 0) return RNPX_OK;
+  if (!lwork) return -7;
+  if (query) {
+    *lwork = (ql ? n : m) * kRefNb;
+    return RNPX_OK;
+  }
+  if (ql ? n > m : n < m) return -2;
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU;
+  if ((st = A.in(h, h->stage1, a, m, n, lda, true))) return st;
+  if ((st = TAU.in(h, h->stage2, const_cast<T*>(tau), k ? k : 1, 1, k ? k : 1, k > 0))) return st;
+  st = ql ? orgql_dev<T>(h, (int)m, (int)n, (int)k, A.dev, A.ld, TAU.dev)
+          : orgrq_dev<T>(h, (int)m, (int)n, (int)k, A.dev, A.ld, TAU.dev);
+  if (st) return st;
+  if ((st = A.out(h, lda))) return st;
+  return sync_handle(h);
+}
+
+template <typename T>
+int gexxf_entry(Fam f, size_t m, size_t n, T* a, size_t lda, T* tau, size_t* lwork, T* work) {
+  if (lwork && (work == nullptr || *lwork == 0)) {
+    if (lda < m || lda == 0) return -4;
+    if ((m < n ? m : n) == 0) return RNPX_OK;
+    *lwork = kRefNb * (f == Fam::QL ? n : m);
+    return RNPX_OK;
+  }
+  return dropin_gexxf<T>(f, m, n, a, lda, tau, lwork);
+}
+
 Handle* H(rnpx_handle h) { return reinterpret_cast<Handle*>(h); }
 
 }  // namespace
@@ -293,6 +393,62 @@ int rnpx_zungqr(size_t m, size_t n, size_t k, rnpx_zcomplex* a, size_t lda, cons
                           lwork && (work == nullptr || *lwork == 0));
 }
 
+int rnpx_dgeqlf(size_t m, size_t n, double* a, size_t lda, double* tau, size_t* lwork, double* work) {
+  return gexxf_entry<double>(Fam::QL, m, n, a, lda, tau, lwork, work);
+}
+int rnpx_zgeqlf(size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau, size_t* lwork,
+                rnpx_zcomplex* work) {
+  return gexxf_entry<zd>(Fam::QL, m, n, reinterpret_cast<zd*>(a), lda, reinterpret_cast<zd*>(tau), lwork,
+                         reinterpret_cast<zd*>(work));
+}
+int rnpx_dormql(const char* side, const char* trans, size_t m, size_t n, size_t k, const double* a, size_t lda,
+                const double* tau, double* c, size_t ldc, size_t* lwork, double* work) {
+  return dropin_ormxx<double>(Fam::QL, side, trans, m, n, k, a, lda, tau, c, ldc, lwork,
+                              lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zunmql(const char* side, const char* trans, size_t m, size_t n, size_t k, const rnpx_zcomplex* a, size_t lda,
+                const rnpx_zcomplex* tau, rnpx_zcomplex* c, size_t ldc, size_t* lwork, rnpx_zcomplex* work) {
+  return dropin_ormxx<zd>(Fam::QL, side, trans, m, n, k, reinterpret_cast<const zd*>(a), lda,
+                          reinterpret_cast<const zd*>(tau), reinterpret_cast<zd*>(c), ldc, lwork,
+                          lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_dorgql(size_t m, size_t n, size_t k, double* a, size_t lda, const double* tau, size_t* lwork,
+                double* work) {
+  return dropin_orgxx<double>(Fam::QL, m, n, k, a, lda, tau, lwork, lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zungql(size_t m, size_t n, size_t k, rnpx_zcomplex* a, size_t lda, const rnpx_zcomplex* tau, size_t* lwork,
+                rnpx_zcomplex* work) {
+  return dropin_orgxx<zd>(Fam::QL, m, n, k, reinterpret_cast<zd*>(a), lda, reinterpret_cast<const zd*>(tau), lwork,
+                          lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_dgerqf(size_t m, size_t n, double* a, size_t lda, double* tau, size_t* lwork, double* work) {
+  return gexxf_entry<double>(Fam::RQ, m, n, a, lda, tau, lwork, work);
+}
+int rnpx_zgerqf(size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau, size_t* lwork,
+                rnpx_zcomplex* work) {
+  return gexxf_entry<zd>(Fam::RQ, m, n, reinterpret_cast<zd*>(a), lda, reinterpret_cast<zd*>(tau), lwork,
+                         reinterpret_cast<zd*>(work));
+}
+int rnpx_dormrq(const char* side, const char* trans, size_t m, size_t n, size_t k, const double* a, size_t lda,
+                const double* tau, double* c, size_t ldc, size_t* lwork, double* work) {
+  return dropin_ormxx<double>(Fam::RQ, side, trans, m, n, k, a, lda, tau, c, ldc, lwork,
+                              lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zunmrq(const char* side, const char* trans, size_t m, size_t n, size_t k, const rnpx_zcomplex* a, size_t lda,
+                const rnpx_zcomplex* tau, rnpx_zcomplex* c, size_t ldc, size_t* lwork, rnpx_zcomplex* work) {
+  return dropin_ormxx<zd>(Fam::RQ, side, trans, m, n, k, reinterpret_cast<const zd*>(a), lda,
+                          reinterpret_cast<const zd*>(tau), reinterpret_cast<zd*>(c), ldc, lwork,
+                          lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_dorgrq(size_t m, size_t n, size_t k, double* a, size_t lda, const double* tau, size_t* lwork,
+                double* work) {
+  return dropin_orgxx<double>(Fam::RQ, m, n, k, a, lda, tau, lwork, lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zungrq(size_t m, size_t n, size_t k, rnpx_zcomplex* a, size_t lda, const rnpx_zcomplex* tau, size_t* lwork,
+                rnpx_zcomplex* work) {
+  return dropin_orgxx<zd>(Fam::RQ, m, n, k, reinterpret_cast<zd*>(a), lda, reinterpret_cast<const zd*>(tau), lwork,
+                          lwork && (work == nullptr || *lwork == 0));
+}
 int rnpx_dgelqf(size_t m, size_t n, double* a, size_t lda, double* tau, size_t* lwork, double* work) {
   if (lwork && (work == nullptr || *lwork == 0)) {
     if (lda < m || lda == 0) return -4;
@@ -400,6 +556,22 @@ int rnpx_dgelqf_dev(rnpx_handle h, size_t m, size_t n, double* a, size_t lda, do
 int rnpx_zgelqf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau) {
   RNPX_DEV_CHECK(h, m, n);
   return gelqf_dev<zd>(H(h), (int)m, (int)n, reinterpret_cast<zd*>(a), (long)lda, reinterpret_cast<zd*>(tau));
+}
+int rnpx_dgeqlf_dev(rnpx_handle h, size_t m, size_t n, double* a, size_t lda, double* tau) {
+  RNPX_DEV_CHECK(h, m, n);
+  return geqlf_dev<double>(H(h), (int)m, (int)n, a, (long)lda, tau);
+}
+int rnpx_zgeqlf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau) {
+  RNPX_DEV_CHECK(h, m, n);
+  return geqlf_dev<zd>(H(h), (int)m, (int)n, reinterpret_cast<zd*>(a), (long)lda, reinterpret_cast<zd*>(tau));
+}
+int rnpx_dgerqf_dev(rnpx_handle h, size_t m, size_t n, double* a, size_t lda, double* tau) {
+  RNPX_DEV_CHECK(h, m, n);
+  return gerqf_dev<double>(H(h), (int)m, (int)n, a, (long)lda, tau);
+}
+int rnpx_zgerqf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau) {
+  RNPX_DEV_CHECK(h, m, n);
+  return gerqf_dev<zd>(H(h), (int)m, (int)n, reinterpret_cast<zd*>(a), (long)lda, reinterpret_cast<zd*>(tau));
 }
 int rnpx_dormqr_dev(rnpx_handle h, const char* side, const char* trans, size_t m, size_t n, size_t k, const double* a,
                     size_t lda, const double* tau, double* c, size_t ldc) {

@@ -161,10 +161,62 @@ hipError_t launch_conj_transpose(int rows, int cols, const T* A, long lda, T* B,
   return hipGetLastError();
 }
 
+// Reversal of the rows (fr) and/or columns (fc) of A (rows x cols).  The
+// element (r, c) trades places with (fr ? rows-1-r : r, fc ? cols-1-c : c);
+// in place, the thread whose element comes first in column-major order
+// swaps the pair (self-pairs stay).  Out of place (B != A), B(r, c) takes
+// A's partner element.  One thread per element, rows fastest.
+template <typename T>
+__global__ __launch_bounds__(256) void flip_kernel(int rows, int cols, const T* A, long lda, T* B, long ldb, int fr,
+                                                   int fc, int inplace) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long long)rows * cols) return;
+  const int r = (int)(e % rows), c = (int)(e / rows);
+  const int pr = fr ? rows - 1 - r : r, pc = fc ? cols - 1 - c : c;
+  if (inplace) {
+    const long long pe = (long long)pr + (long long)pc * rows;
+    if (pe <= e) return;
+    T* a = const_cast<T*>(A);
+    T* x = a + (size_t)r + (size_t)c * lda;
+    T* y = a + (size_t)pr + (size_t)pc * lda;
+    const T t = *x;
+    *x = *y;
+    *y = t;
+  } else {
+    B[(size_t)r + (size_t)c * ldb] = A[(size_t)pr + (size_t)pc * lda];
+  }
+}
+
+template <typename T>
+hipError_t launch_flip(int rows, int cols, T* A, long lda, bool fr, bool fc, hipStream_t st) {
+  if (rows < 0 || cols < 0 || lda < std::max(1, rows)) return hipErrorInvalidValue;
+  const long long total = (long long)rows * cols;
+  if (total == 0 || (!fr && !fc)) return hipSuccess;
+  const long long blocks = (total + 255) / 256;
+  if (blocks > 2147483647LL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((flip_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, rows, cols, (const T*)A, lda, A, lda,
+                     (int)fr, (int)fc, 1);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_flip_copy(int rows, int cols, const T* A, long lda, T* B, long ldb, bool fr, bool fc, hipStream_t st) {
+  if (rows < 0 || cols < 0 || lda < std::max(1, rows) || ldb < std::max(1, rows)) return hipErrorInvalidValue;
+  const long long total = (long long)rows * cols;
+  if (total == 0) return hipSuccess;
+  const long long blocks = (total + 255) / 256;
+  if (blocks > 2147483647LL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((flip_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, rows, cols, A, lda, B, ldb, (int)fr,
+                     (int)fc, 0);
+  return hipGetLastError();
+}
+
 #define RNPX_INST(T)                                                                  \
   template hipError_t launch_set<T>(int, int, T, T, T*, long, hipStream_t);           \
   template hipError_t launch_copy<T>(int, int, const T*, long, T*, long, hipStream_t); \
   template hipError_t launch_conj_transpose<T>(int, int, const T*, long, T*, long, hipStream_t); \
+  template hipError_t launch_flip<T>(int, int, T*, long, bool, bool, hipStream_t);    \
+  template hipError_t launch_flip_copy<T>(int, int, const T*, long, T*, long, bool, bool, hipStream_t); \
   template hipError_t launch_extract_v<T>(int, int, const T*, long, T*, long, hipStream_t); \
   template hipError_t launch_zero_rows<T>(int, int, T*, long, hipStream_t);           \
   template hipError_t launch_larnv<T>(int, const int*, long long, T*, hipStream_t);

@@ -55,12 +55,13 @@ struct Handle {
   Buffer wbuf2, zbuf2, part2, misc2;       // GEMM scratch of the panel stream
   Buffer trsm_flags;                       // per-block flags of the blocked triangular solve
   Buffer lqbuf;                            // A^H for the LQ routines
+  Buffer qlbuf;                            // reversed reflectors and tau (QL, RQ)
   unsigned trsm_epoch = 0;                 // value the flags take in the current solve
   // Every buffer above, for accounting and release.
   std::vector<Buffer*> buffers() {
     return {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc,
             &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
-            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf};
+            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf};
   }
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
@@ -116,6 +117,18 @@ template <typename T>
 int ormlq_dev(Handle* h, char side, char trans, int m, int n, int k, const T* A, long lda, const T* tau, T* C,
               long ldc);
 template <typename T> int orglq_dev(Handle* h, int m, int n, int k, T* A, long lda, const T* tau);
+// QL and RQ (rnpx_qlrq.cpp): the QR and LQ paths on the row- and
+// column-reversed matrix.
+template <typename T> int geqlf_dev(Handle* h, int m, int n, T* A, long lda, T* tau);
+template <typename T>
+int ormql_dev(Handle* h, char side, char trans, int m, int n, int k, const T* A, long lda, const T* tau, T* C,
+              long ldc);
+template <typename T> int orgql_dev(Handle* h, int m, int n, int k, T* A, long lda, const T* tau);
+template <typename T> int gerqf_dev(Handle* h, int m, int n, T* A, long lda, T* tau);
+template <typename T>
+int ormrq_dev(Handle* h, char side, char trans, int m, int n, int k, const T* A, long lda, const T* tau, T* C,
+              long ldc);
+template <typename T> int orgrq_dev(Handle* h, int m, int n, int k, T* A, long lda, const T* tau);
 template <typename T>
 int larft_dev(Handle* h, int n, int k, const T* V, long ldv, const T* tau, T* Tout, long ldt);
 template <typename T>

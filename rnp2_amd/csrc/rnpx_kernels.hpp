@@ -164,6 +164,13 @@ template <typename T>
 hipError_t launch_conj_transpose(int rows, int cols, const T* A, long lda, T* B, long ldb, hipStream_t st);
 template <typename T>
 hipError_t launch_copy(int m, int n, const T* A, long lda, T* B, long ldb, hipStream_t st);
+// Rows (fr) and/or columns (fc) of A (rows x cols) reversed, in place or
+// into B (rows x cols, leading dimension ldb).
+template <typename T>
+hipError_t launch_flip(int rows, int cols, T* A, long lda, bool fr, bool fc, hipStream_t st);
+template <typename T>
+hipError_t launch_flip_copy(int rows, int cols, const T* A, long lda, T* B, long ldb, bool fr, bool fc, hipStream_t st);
+
 // Clean V (unit lower trapezoidal, n x k) out of a packed factor.
 template <typename T>
 hipError_t launch_extract_v(int n, int k, const T* A, long lda, T* V, long ldv, hipStream_t st);

@@ -239,3 +239,77 @@ def test_cpp_lq_surface_compiles_and_queries():
     assert "Assertion failed in" in out and "lda >= m" in out
     if not has_gpu():
         assert f"status {R.RNPX_ERR_NOGPU}" in out and "rnpx: LQ::Factor: status" in out
+
+
+def test_ql_rq_workspace_queries_match_reference():
+    """QL::Factor nb*n (QL.hpp:139-141), RQ::Factor nb*m (RQ.hpp:142-144),
+    MultQ nb*nw + nb^2 (QL.hpp:351-354, RQ.hpp:359-362), QL::GenerateQ n*nb
+    (QL.hpp:509-512), RQ::GenerateQ m*nb (RQ.hpp:528-531)."""
+    L = R.lib()
+    sz = ctypes.c_size_t
+    for fn, m, n, want in (("rnpx_dgeqlf", 100, 300, 32 * 300), ("rnpx_zgeqlf", 300, 100, 32 * 100),
+                           ("rnpx_dgerqf", 100, 300, 32 * 100), ("rnpx_zgerqf", 300, 100, 32 * 300)):
+        lw = sz(0)
+        assert getattr(L, fn)(m, n, None, m, None, ctypes.byref(lw), None) == 0
+        assert lw.value == want, fn
+    for fn in ("rnpx_dormql", "rnpx_zunmql", "rnpx_dormrq", "rnpx_zunmrq"):
+        lw = sz(0)
+        assert getattr(L, fn)(b"L", b"C", 100, 40, 30, None, 100, None, None, 100, ctypes.byref(lw), None) == 0
+        assert lw.value == 32 * 40 + 32 * 32, fn
+    lw = sz(0)
+    assert L.rnpx_dorgql(100, 40, 30, None, 100, None, ctypes.byref(lw), None) == 0 and lw.value == 40 * 32
+    lw = sz(0)
+    assert L.rnpx_zungrq(40, 100, 30, None, 40, None, ctypes.byref(lw), None) == 0 and lw.value == 40 * 32
+    # argument errors, LAPACK style
+    assert L.rnpx_dgeqlf(10, 5, None, 9, None, ctypes.byref(sz(0)), None) == -4
+    a = np.zeros(50)
+    assert L.rnpx_dgerqf(10, 5, a.ctypes.data_as(ctypes.c_void_p), 10, None, ctypes.byref(sz(5)),
+                         a.ctypes.data_as(ctypes.c_void_p)) == -6                   # 0 < lwork < m
+    assert L.rnpx_dormql(b"L", b"N", 10, 5, 3, None, 2, None, None, 10, ctypes.byref(sz(0)), None) == -7
+    assert L.rnpx_dorgql(10, 5, 6, None, 10, None, ctypes.byref(sz(0)), None) == -3   # k > n
+    assert L.rnpx_dorgrq(5, 10, 6, None, 5, None, ctypes.byref(sz(0)), None) == -3   # k > m
+
+
+def test_cpp_ql_rq_surfaces_compile_and_query():
+    """include/rnpx_ql.hpp and include/rnpx_rq.hpp: the reference's template
+    signatures and workspace queries."""
+    src = r'''
+    #include <complex>
+    #include <cstdio>
+    #include "rnpx_ql.hpp"
+    #include "rnpx_rq.hpp"
+    using namespace RNP::LA;
+    int main() {
+      size_t lw = 0;
+      QL::Factor<double>(50, 70, NULL, 50, NULL, &lw, NULL);
+      std::printf("qlfactor %zu\n", lw);
+      lw = 0;
+      RQ::Factor<std::complex<double> >(50, 70, NULL, 50, NULL, &lw, NULL);
+      std::printf("rqfactor %zu\n", lw);
+      lw = 0;
+      QL::MultQ<double>("L", "N", 100, 40, 30, NULL, 100, NULL, NULL, 100, &lw, NULL);
+      std::printf("qlmultq %zu\n", lw);
+      lw = 0;
+      RQ::MultQ<float>("R", "C", 40, 100, 30, NULL, 30, NULL, NULL, 40, &lw, NULL);
+      std::printf("rqmultq %zu\n", lw);
+      lw = 0;
+      QL::GenerateQ<double>(100, 40, 30, NULL, 100, NULL, &lw, NULL);
+      std::printf("qlgenq %zu\n", lw);
+      lw = 0;
+      RQ::GenerateQ<double>(30, 100, 30, NULL, 30, NULL, &lw, NULL);
+      std::printf("rqgenq %zu\n", lw);
+      RQ::Factor<double>(10, 5, NULL, 9, NULL, &lw, NULL);        /* lda < m */
+      double a[50] = {1}, tau[5], work[400];
+      lw = 400;
+      QL::Factor(10, 5, a, 10, tau, &lw, work);
+      std::printf("status %d\n", QL::LastStatus());
+      return 0;
+    }'''
+    out = _run_c(src, "g++", "cpp", ("-std=c++11", "-L", os.path.dirname(R.LIB_PATH), "-lrnpx",
+                                     f"-Wl,-rpath,{os.path.dirname(R.LIB_PATH)}"))
+    assert "qlfactor 2240" in out and "rqfactor 1600" in out
+    assert "qlmultq 2304" in out and "rqmultq 2304" in out
+    assert "qlgenq 1280" in out and "rqgenq 960" in out
+    assert "Assertion failed in" in out and "lda >= m" in out
+    if not has_gpu():
+        assert f"status {R.RNPX_ERR_NOGPU}" in out and "rnpx: QL::Factor: status" in out

@@ -604,3 +604,210 @@ def test_lq_sign_rule_and_edges(cplx):
 def test_lq_workspace_queries():
     assert O.lq_factor_query(50, 70) == 32 * 70
     assert O.lq_factor_query(0, 70) == 0
+
+
+# ---------------------------------------------------------------------------
+# QL (QL.hpp) and RQ (RQ.hpp).  Pins: LAPACK's QR of the matrix with rows
+# and columns reversed (QL: the reversal of A = Q L is a QR factorization
+# with the same reflectors reversed, so LAPACK's independent geqrf pins every
+# stored entry), LAPACK gerqf/orgrq directly for RQ, the explicit reflector
+# products (brute force) for MultQ and GenerateQ, and A = Q L / A = R Q.
+# ---------------------------------------------------------------------------
+
+QLRQ_SHAPES = [(5, 9), (9, 5), (40, 70), (70, 40), (33, 33), (1, 1), (1, 6), (6, 1), (130, 200)]
+
+
+def _rot(x):
+    return np.asfortranarray(x[::-1, ::-1])
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("shape", QLRQ_SHAPES)
+def test_ql_factor_matches_lapack_qr_of_reversed(shape, cplx):
+    m, n = shape
+    rng = np.random.default_rng(m * 977 + n)
+    a = _rand(rng, m, n, cplx)
+    af, tau = O.ql_factor(a)
+    k = min(m, n)
+    qr, tq, _, info = (lapack.zgeqrf if cplx else lapack.dgeqrf)(_rot(a))
+    assert info == 0
+    want = _rot(qr)
+    s = max(np.abs(a).max(), 1e-300)
+    # L: rows m-k.., on and below the diagonal that ends at (m-1, n-1)
+    lmask = np.zeros((m, n), bool)
+    for i in range(m):
+        for j in range(n):
+            lmask[i, j] = (j - (n - k)) <= (i - (m - k)) and i >= m - k or (j < n - k)
+    vmask = ~lmask
+    assert np.abs((af - want)[lmask]).max() <= 1e-13 * s * max(m, n)
+    if vmask.any():
+        assert np.abs((af - want)[vmask]).max() <= 1e-12
+    np.testing.assert_allclose(tau, tq[:k][::-1], rtol=0, atol=1e-13)
+
+
+def _ql_explicit_q(af, tau, k):
+    """H[k-1] ... H[0] (QL.hpp:59-64; the order MultQ_unblocked applies),
+    H[i] = I - tau_i v v^H, v stored in column n-k+i above row nq-k+i, 1 there."""
+    nq, n = af.shape
+    Q = np.eye(nq, dtype=af.dtype)
+    for i in range(k):
+        v = np.zeros(nq, dtype=af.dtype)
+        r = nq - k + i
+        v[:r] = af[:r, n - k + i]
+        v[r] = 1.0
+        Q = (np.eye(nq) - tau[i] * np.outer(v, v.conj())) @ Q
+    return Q
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_ql_brute_force_reflectors(cplx):
+    """GenerateQ equals the last n columns of H[k-1] ... H[0] from the stored
+    vectors; A = Q L with L the bottom n x n lower triangle; Q^H Q = I."""
+    rng = np.random.default_rng(21)
+    m, n = 11, 6
+    a = _rand(rng, m, n, cplx)
+    af, tau = O.ql_factor(a)
+    H = _ql_explicit_q(af, tau, n)
+    Q = O.ql_generateq(af.copy(), tau)
+    np.testing.assert_allclose(Q, H[:, m - n:], atol=1e-14)
+    L = np.tril(af[m - n:, :])
+    np.testing.assert_allclose(Q @ L, a, atol=1e-13)
+    np.testing.assert_allclose(Q.conj().T @ Q, np.eye(n), atol=1e-14)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("side,trans", [("L", "N"), ("L", "C"), ("R", "N"), ("R", "C")])
+def test_ql_multq_against_explicit_q(side, trans, cplx):
+    rng = np.random.default_rng(22)
+    for nq, k in ((11, 5), (40, 20)):
+        a = _rand(rng, nq, k, cplx)
+        af, tau = O.ql_factor(a)
+        Q = _ql_explicit_q(af, tau, k)
+        c = _rand(rng, nq, 7, cplx) if side == "L" else _rand(rng, 7, nq, cplx)
+        got = O.ql_multq(side, trans, af, tau, c, k=k)
+        op = Q if trans == "N" else Q.conj().T
+        want = op @ c if side == "L" else c @ op
+        np.testing.assert_allclose(got, want, atol=1e-13 * max(1.0, np.abs(want).max()))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_ql_generateq_partial_k(cplx):
+    """k < n: the first n-k columns start as unit columns (QL.hpp:455-460)."""
+    rng = np.random.default_rng(23)
+    m, n, k = 12, 7, 4
+    a = _rand(rng, m, k, cplx)
+    af, tau = O.ql_factor(a)
+    full = np.zeros((m, n), dtype=af.dtype, order="F")
+    full[:, n - k:] = af
+    Q = O.ql_generateq(full, tau, k=k)
+    H = _ql_explicit_q(af, tau, k)
+    np.testing.assert_allclose(Q, H[:, m - n:], atol=1e-14)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("shape", QLRQ_SHAPES)
+def test_rq_factor_matches_lapack_gerqf(shape, cplx):
+    m, n = shape
+    rng = np.random.default_rng(m * 991 + n)
+    a = _rand(rng, m, n, cplx)
+    af, tau = O.rq_factor(a)
+    k = min(m, n)
+    rq, tl, _, info = (lapack.zgerqf if cplx else lapack.dgerqf)(a.copy())
+    assert info == 0
+    s = max(np.abs(a).max(), 1e-300)
+    rmask = np.zeros((m, n), bool)
+    for i in range(m):
+        for j in range(n):
+            rmask[i, j] = (i < m - k) or (j - (n - k) >= i - (m - k))
+    assert np.abs((af - rq)[rmask]).max() <= 1e-13 * s * max(m, n)
+    if (~rmask).any():
+        assert np.abs((af - rq)[~rmask]).max() <= 1e-12
+    np.testing.assert_allclose(tau, tl[:k], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_rq_generateq_matches_lapack_orgrq(cplx):
+    """GenerateQ against LAPACK orgrq/ungrq for k = m and for the last k < m
+    reflector rows (the leading rows start as unit rows, RQ.hpp:465-474)."""
+    rng = np.random.default_rng(24)
+    for m, n, k in ((6, 11, 6), (6, 11, 4), (30, 50, 30), (30, 50, 17)):
+        a = _rand(rng, m, n, cplx)
+        af, tau = O.rq_factor(a)
+        inp = np.zeros((m, n), dtype=af.dtype, order="F")
+        inp[m - k:] = af[m - k:]
+        got = O.rq_generateq(inp.copy(), tau[m - k:], k=k)
+        qw, _, info = (lapack.zungrq if cplx else lapack.dorgrq)(inp.copy(), tau[m - k:])
+        assert info == 0
+        np.testing.assert_allclose(got, qw, atol=1e-13)
+        np.testing.assert_allclose(got @ got.conj().T, np.eye(m), atol=1e-13)
+        if k == m:
+            np.testing.assert_allclose(np.triu(af[:, n - m:]) @ got, a, atol=1e-12)
+
+
+def _rq_explicit_q(af, tau, k):
+    """Q = H[0]^H ... H[k-1]^H with v_i = conj of the stored row m-k+i left of
+    column nq-k+i, 1 there (RQ.hpp:57-66; LAPACK's dgerqf convention)."""
+    m, nq = af.shape
+    Q = np.eye(nq, dtype=af.dtype)
+    for i in range(k):
+        v = np.zeros(nq, dtype=af.dtype)
+        c = nq - k + i
+        v[:c] = np.conj(af[m - k + i, :c])
+        v[c] = 1.0
+        Q = Q @ (np.eye(nq) - tau[i] * np.outer(v, v.conj())).conj().T
+    return Q
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("side,trans", [("L", "N"), ("L", "C"), ("R", "N"), ("R", "C")])
+def test_rq_multq_against_explicit_q(side, trans, cplx):
+    rng = np.random.default_rng(25)
+    for k, nq in ((5, 11), (20, 40)):
+        a = _rand(rng, k, nq, cplx)
+        af, tau = O.rq_factor(a)
+        Q = _rq_explicit_q(af, tau, k)
+        np.testing.assert_allclose(np.triu(af[:, nq - k:]) @ Q[nq - k:], a, atol=1e-12)
+        c = _rand(rng, nq, 7, cplx) if side == "L" else _rand(rng, 7, nq, cplx)
+        got = O.rq_multq(side, trans, af, tau, c, k=k)
+        op = Q if trans == "N" else Q.conj().T
+        want = op @ c if side == "L" else c @ op
+        np.testing.assert_allclose(got, want, atol=1e-13 * max(1.0, np.abs(want).max()))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_ql_rq_sign_rule_and_edges(cplx):
+    """Re(alpha) == 0 gives beta > 0 (Reflector.hpp:131); a zero column (row)
+    takes tau = 0; tiny and huge columns (rows) take the scaled paths."""
+    rng = np.random.default_rng(26)
+    a = _rand(rng, 12, 8, cplx)
+    a[11, 7] = 0.0          # QL's first alpha
+    a[:, 5] = 0.0           # a zero column: its reflector is H = I
+    a[:, 3] *= 1e-300
+    a[:, 2] *= 1e300
+    af, tau = O.ql_factor(a)
+    assert af[11, 7].real > 0
+    assert tau[5] == 0
+    # the oracle's own QR of the reversed matrix (RNP's sign rule) agrees
+    qf, qt = O.factor(_rot(a))
+    np.testing.assert_allclose(tau, qt[::-1], rtol=0, atol=1e-13)
+    L = np.tril(af[4:, :])
+    Lw = np.tril(_rot(qf)[4:, :])
+    sc = np.abs(Lw).max(axis=0, keepdims=True)
+    sc[sc == 0] = 1.0
+    assert np.max(np.abs(L - Lw) / sc) <= 1e-13
+    b = _rand(rng, 8, 12, cplx)
+    b[7, 11] = 0.0
+    b[5, :] = 0.0
+    af, tau = O.rq_factor(b)
+    assert af[7, 11].real > 0
+    assert tau[5] == 0
+
+
+def test_ql_rq_workspace_queries():
+    assert O.ql_query("factor", 50, 70) == 32 * 70
+    assert O.rq_query("factor", 50, 70) == 32 * 50
+    assert O.ql_query("factor", 0, 70) == 0
+    assert O.ql_query("multq", "L", "N", 40, 9, 5) == 32 * 9 + 32 * 32
+    assert O.rq_query("multq", "R", "C", 40, 9, 5) == 32 * 40 + 32 * 32
+    assert O.ql_query("generateq", 40, 30, 20) == 30 * 32
+    assert O.rq_query("generateq", 30, 40, 20) == 30 * 32
