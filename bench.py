@@ -143,6 +143,34 @@ def leg_lq(R, torch, m=4096, n=8192, reps=3):
             "qr_of_transpose_ms": round(t_qr * 1e3, 3)}
 
 
+def leg_qlrq(R, torch, m=8192, n=4096, reps=3):
+    """QL::Factor (RNP/LA/QL.hpp:126) of a tall m x n matrix and RQ::Factor
+    (RNP/LA/RQ.hpp:131) of its wide transpose, beside QR::Factor of the same
+    tall matrix: the difference is the row and column reversals."""
+    h = R.Handle()
+    a0 = R.colmajor_empty(m, n, torch.float64)
+    h.larnv(2, [3, 1, 4, 1], a0)
+    a = R.colmajor_empty(m, n, torch.float64)
+    at = R.colmajor_empty(n, m, torch.float64)
+    tau = torch.empty(min(m, n), dtype=torch.float64, device="cuda")
+    h.synchronize()
+    stream = torch.cuda.ExternalStream(h.stream)
+
+    def run(fn, dst, src):
+        def f():
+            with torch.cuda.stream(stream):
+                dst.copy_(src)
+            fn(dst, tau)
+        return f
+
+    t_ql = _ev_time(run(h.geqlf, a, a0), reps, h.synchronize)
+    t_rq = _ev_time(run(h.gerqf, at, a0.t()), reps, h.synchronize)
+    t_qr = _ev_time(run(h.geqrf, a, a0), reps, h.synchronize)
+    fl = qr_flops(m, n)
+    return {"shape": [m, n], "dtype": "f64", "ql_ms": round(t_ql * 1e3, 3), "ql_gflops": round(fl / t_ql / 1e9, 1),
+            "rq_of_transpose_ms": round(t_rq * 1e3, 3), "qr_ms": round(t_qr * 1e3, 3)}
+
+
 def c3_shapes(count, seed):
     rng = np.random.default_rng(seed)
     m = np.rint(np.exp(rng.uniform(np.log(16), np.log(256), count))).astype(int)
@@ -398,7 +426,7 @@ def main():
         "panel_hbm_gbs": round(pan["bytes"] / (pan["ms"] * 1e-3) / 1e9, 1) if pan["ms"] > 0 else None,
         "tau_sane": ok,
     }
-    legs = [] if args.legs == "none" else (["complex", "tall", "batched", "mixed", "serving", "lq"]
+    legs = [] if args.legs == "none" else (["complex", "tall", "batched", "mixed", "serving", "lq", "qlrq"]
                                            if args.legs == "all" else args.legs.split(","))
     extra = {}
     for leg in legs:
@@ -416,6 +444,8 @@ def main():
                 extra["serving"] = leg_serving(R, torch)
             elif leg == "lq":
                 extra["lq_4096x8192"] = leg_lq(R, torch)
+            elif leg == "qlrq":
+                extra["ql_rq_8192x4096"] = leg_qlrq(R, torch)
         except Exception as e:  # a failing leg is reported, the headline line still prints
             extra[leg] = {"error": repr(e)}
     if extra:
