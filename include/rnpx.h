@@ -172,6 +172,26 @@ int rnpx_dorgrq(size_t m, size_t n, size_t k, double *a, size_t lda, const doubl
 int rnpx_zungrq(size_t m, size_t n, size_t k, rnpx_zcomplex *a, size_t lda,
                 const rnpx_zcomplex *tau, size_t *lwork, rnpx_zcomplex *work);
 
+/* Hessenberg (RNP/LA/Hessenberg.hpp): the Q of a Hessenberg reduction
+ * (Reduce's reflectors one row below the diagonal, 0-based ilo, exclusive
+ * ihi) through the QR routines.  Checks and workspace queries as the
+ * reference: MultQ passes QR::MultQ's (nb*nw + nb*nb, none when
+ * ihi-ilo-1 = 0), GenerateQ nh*nb with nh = ihi-ilo-1 (Hessenberg.hpp:553-556). */
+/* Hessenberg::MultQ<double> -- RNP/LA/Hessenberg.hpp:478-514 (dormhr). */
+int rnpx_dormhr(const char *side, const char *trans, size_t m, size_t n, size_t ilo, size_t ihi,
+                const double *a, size_t lda, const double *tau, double *c, size_t ldc,
+                size_t *lwork, double *work);
+/* Hessenberg::MultQ<std::complex<double>> -- RNP/LA/Hessenberg.hpp:478-514 (zunmhr). */
+int rnpx_zunmhr(const char *side, const char *trans, size_t m, size_t n, size_t ilo, size_t ihi,
+                const rnpx_zcomplex *a, size_t lda, const rnpx_zcomplex *tau, rnpx_zcomplex *c,
+                size_t ldc, size_t *lwork, rnpx_zcomplex *work);
+/* Hessenberg::GenerateQ<double> -- RNP/LA/Hessenberg.hpp:540-588 (dorghr): the n x n Q. */
+int rnpx_dorghr(size_t n, size_t ilo, size_t ihi, double *a, size_t lda, const double *tau,
+                size_t *lwork, double *work);
+/* Hessenberg::GenerateQ<std::complex<double>> -- RNP/LA/Hessenberg.hpp:540-588 (zunghr). */
+int rnpx_zunghr(size_t n, size_t ilo, size_t ihi, rnpx_zcomplex *a, size_t lda,
+                const rnpx_zcomplex *tau, size_t *lwork, rnpx_zcomplex *work);
+
 /* ---------------------------------------------------------------------
  * Handles (device-resident work on one HIP stream).
  * ------------------------------------------------------------------- */
@@ -215,6 +235,11 @@ int rnpx_zgeqlf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex *a, size_t 
 int rnpx_dgerqf_dev(rnpx_handle h, size_t m, size_t n, double *a, size_t lda, double *tau);
 int rnpx_zgerqf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex *a, size_t lda,
                     rnpx_zcomplex *tau);
+/* Hessenberg::GenerateQ on device pointers, asynchronous (RNP/LA/Hessenberg.hpp:540-588). */
+int rnpx_dorghr_dev(rnpx_handle h, size_t n, size_t ilo, size_t ihi, double *a, size_t lda,
+                    const double *tau);
+int rnpx_zunghr_dev(rnpx_handle h, size_t n, size_t ilo, size_t ihi, rnpx_zcomplex *a, size_t lda,
+                    const rnpx_zcomplex *tau);
 int rnpx_dormqr_dev(rnpx_handle h, const char *side, const char *trans, size_t m, size_t n,
                     size_t k, const double *a, size_t lda, const double *tau, double *c,
                     size_t ldc);

@@ -80,6 +80,9 @@ def _declare(L):
                                                       _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]
         getattr(L, f"orc_{pre}_lq_generateq").argtypes = [_sz, _sz, _sz, _p, _sz, _p,
                                                           ctypes.POINTER(_sz), _p]
+        getattr(L, f"orc_{pre}_hess_multq").argtypes = [ctypes.c_char_p, ctypes.c_char_p, _sz, _sz, _sz, _sz,
+                                                        _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]
+        getattr(L, f"orc_{pre}_hess_generateq").argtypes = [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]
         for fam in ("ql", "rq"):
             getattr(L, f"orc_{pre}_{fam}_factor").argtypes = [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]
             getattr(L, f"orc_{pre}_{fam}_multq").argtypes = [ctypes.c_char_p, ctypes.c_char_p, _sz, _sz, _sz,
@@ -478,6 +481,53 @@ def rq_generateq(a, tau, k=None):
     """RQ::GenerateQ (RQ.hpp:519-580): a (m x n, m <= n, reflectors in its
     last k rows) becomes the last m rows of Q."""
     return _fam_generateq("rq", a, tau, k)
+
+
+def hess_multq(side, trans, afac, tau, c, ilo, ihi):
+    """Hessenberg::MultQ (Hessenberg.hpp:478-514): op(Q) C or C op(Q) for the
+    Q of a Hessenberg reduction held in afac/tau (0-based ilo, exclusive ihi)."""
+    c = _fortran(c, np.result_type(c, afac, np.float64))
+    pre = _pre(c.dtype)
+    afac = _fortran(afac, c.dtype)
+    tau = np.ascontiguousarray(tau, dtype=c.dtype)
+    m, n = c.shape
+    f = getattr(lib(), f"orc_{pre}_hess_multq")
+    lw = _sz(0)
+    f(side.encode(), trans.encode(), m, n, ilo, ihi, _ptr(afac), afac.shape[0], _ptr(tau), _ptr(c), max(m, 1),
+      ctypes.byref(lw), None)
+    work = np.zeros(max(lw.value, m, n, 1), dtype=c.dtype)
+    lw = _sz(work.size)
+    f(side.encode(), trans.encode(), m, n, ilo, ihi, _ptr(afac), afac.shape[0], _ptr(tau), _ptr(c), max(m, 1),
+      ctypes.byref(lw), _ptr(work))
+    return c
+
+
+def hess_generateq(afac, tau, ilo, ihi):
+    """Hessenberg::GenerateQ (Hessenberg.hpp:540-588): the n x n Q."""
+    a = _fortran(afac, np.result_type(afac, np.float64))
+    pre = _pre(a.dtype)
+    n = a.shape[0]
+    tau = np.ascontiguousarray(tau, dtype=a.dtype)
+    f = getattr(lib(), f"orc_{pre}_hess_generateq")
+    lw = _sz(0)
+    f(n, ilo, ihi, _ptr(a), max(n, 1), _ptr(tau), ctypes.byref(lw), None)
+    work = np.zeros(max(lw.value, n, 1), dtype=a.dtype)
+    lw = _sz(work.size)
+    f(n, ilo, ihi, _ptr(a), max(n, 1), _ptr(tau), ctypes.byref(lw), _ptr(work))
+    return a
+
+
+def hess_query(what, *args, dtype=np.float64):
+    lw = _sz(0)
+    if what == "generateq":
+        n, ilo, ihi = args
+        getattr(lib(), f"orc_{_pre(dtype)}_hess_generateq")(n, ilo, ihi, None, max(n, 1), None, ctypes.byref(lw),
+                                                             None)
+    else:
+        side, trans, m, n, ilo, ihi = args
+        getattr(lib(), f"orc_{_pre(dtype)}_hess_multq")(side.encode(), trans.encode(), m, n, ilo, ihi, None,
+                                                         max(m, n, 1), None, None, max(m, 1), ctypes.byref(lw), None)
+    return lw.value
 
 
 def ql_query(what, *args, dtype=np.float64):

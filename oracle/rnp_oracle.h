@@ -164,6 +164,17 @@ void orc_z_rq_multq(const char *side, const char *trans, size_t m, size_t n, siz
 void orc_z_rq_generateq(size_t m, size_t n, size_t k, orc_z *a, size_t lda, const orc_z *tau,
                         size_t *lwork, orc_z *work);
 
+/* ---- Hessenberg (RNP/LA/Hessenberg.hpp) ----
+ *   Hessenberg::MultQ  Hessenberg.hpp:478-514, Hessenberg::GenerateQ Hessenberg.hpp:540-588 */
+void orc_d_hess_multq(const char *side, const char *trans, size_t m, size_t n, size_t ilo, size_t ihi,
+                      const double *a, size_t lda, const double *tau, double *c, size_t ldc, size_t *lwork, double *work);
+void orc_d_hess_generateq(size_t n, size_t ilo, size_t ihi, double *a, size_t lda, const double *tau,
+                          size_t *lwork, double *work);
+void orc_z_hess_multq(const char *side, const char *trans, size_t m, size_t n, size_t ilo, size_t ihi,
+                      const orc_z *a, size_t lda, const orc_z *tau, orc_z *c, size_t ldc, size_t *lwork, orc_z *work);
+void orc_z_hess_generateq(size_t n, size_t ilo, size_t ihi, orc_z *a, size_t lda, const orc_z *tau,
+                          size_t *lwork, orc_z *work);
+
 /* Least squares min ||A x - B|| for m >= n by Factor, MultQ("L","C") and an
  * upper-triangular back substitution with R (BLAS dtrsm "L","U","N","N";
  * the composition of QR.hpp:129, QR.hpp:337 and BLAS_mix.hpp:2096).

@@ -1034,6 +1034,66 @@ void FN(rq_generateq)(size_t m, size_t n, size_t k, T *a, size_t lda, const T *t
 	FN(rq_generateq_unblocked)(m, n, k, a, lda, tau, work);
 }
 
+/* ---------------------------------------------------------------------
+ * Hessenberg (RNP/LA/Hessenberg.hpp): the Q of a reduction, through QR.
+ * ------------------------------------------------------------------- */
+
+/* Hessenberg::MultQ, Hessenberg.hpp:478-514: QR::MultQ on the reflector
+ * block one row below the diagonal (nh = ihi-ilo-1 reflectors from column
+ * ilo), applied to rows (side "L") or columns (side "R") ilo+1.. of C. */
+void FN(hess_multq)(const char *side, const char *trans, size_t m, size_t n, size_t ilo, size_t ihi,
+                    const T *a, size_t lda, const T *tau, T *c, size_t ldc, size_t *lwork, T *work){
+	const int left = ('L' == side[0]);
+	const size_t nh = ihi-ilo-1;
+	const size_t nq = left ? m : n, nw = left ? n : m;
+	if(!('L' == side[0] || 'R' == side[0])){ orc_assert_fail("side", __LINE__); }
+	if(!('N' == trans[0] || 'T' == trans[0] || 'C' == trans[0])){ orc_assert_fail("trans", __LINE__); }
+	if(!(ilo <= nq)){ orc_assert_fail("ilo <= nq", __LINE__); }
+	if(!(ilo < ihi)){ orc_assert_fail("ilo < ihi", __LINE__); }
+	if(!(ihi <= nq)){ orc_assert_fail("ihi <= nq", __LINE__); }
+	if(NULL == lwork){ orc_assert_fail("NULL != lwork", __LINE__); return; }
+	if(!(0 == *lwork || *lwork >= nw)){ orc_assert_fail("0 == *lwork || *lwork >= nw", __LINE__); }
+	const size_t mi = left ? nh : m, ni = left ? n : nh;
+	const size_t i1 = left ? ilo+1 : 0, i2 = left ? 0 : ilo+1;
+	FN(multq)(side, trans, mi, ni, nh, &a[ilo+1+ilo*lda], lda, &tau[ilo], &c[i1+i2*ldc], ldc, lwork, work, NULL);
+}
+
+/* Hessenberg::GenerateQ, Hessenberg.hpp:540-588: the reflectors shifted one
+ * column right, identity outside rows/columns [ilo+1, ihi), then
+ * QR::GenerateQ on the nh x nh block. */
+void FN(hess_generateq)(size_t n, size_t ilo, size_t ihi, T *a, size_t lda, const T *tau, size_t *lwork, T *work){
+	const size_t nh = ihi-ilo-1;
+	if(!(ilo <= n)){ orc_assert_fail("ilo <= n", __LINE__); }
+	if(!(ilo < ihi)){ orc_assert_fail("ilo < ihi", __LINE__); }
+	if(!(ihi <= n)){ orc_assert_fail("ihi <= n", __LINE__); }
+	if(!(lda >= n)){ orc_assert_fail("lda >= n", __LINE__); }
+	if(NULL == lwork){ orc_assert_fail("NULL != lwork", __LINE__); return; }
+	if(!(0 == *lwork || *lwork >= nh)){ orc_assert_fail("0 == *lwork || *lwork >= nh", __LINE__); }
+	if(0 == *lwork || NULL == work){
+		*lwork = nh*32;
+		return;
+	}
+	if(0 == n){ return; }
+	size_t j = ihi;
+	while(j-- > ilo+1){
+		for(size_t i = 0; i < j; ++i){ a[i+j*lda] = 0; }
+		for(size_t i = j+1; i < ihi; ++i){ a[i+j*lda] = a[i+(j-1)*lda]; }
+		for(size_t i = ihi; i < n; ++i){ a[i+j*lda] = 0; }
+	}
+	for(j = 0; j <= ilo; ++j){
+		for(size_t i = 0; i < n; ++i){ a[i+j*lda] = 0; }
+		a[j+j*lda] = 1;
+	}
+	for(j = ihi; j < n; ++j){
+		for(size_t i = 0; i < n; ++i){ a[i+j*lda] = 0; }
+		a[j+j*lda] = 1;
+	}
+	if(nh > 0){
+		size_t lw = *lwork;
+		FN(generateq)(nh, nh, nh, &a[ilo+1+(ilo+1)*lda], lda, &tau[ilo], &lw, work, NULL);
+	}
+}
+
 /* Least squares by QR: Factor (QR.hpp:129), MultQ("L","C") on B
  * (QR.hpp:337), then R x = (Q^H B)(0:n,:) by back substitution
  * (SolveTrM "L","U","N","N", BLAS_mix.hpp:2096). */

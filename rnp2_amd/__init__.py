@@ -104,6 +104,12 @@ SIGNATURES = {
     "rnpx_zgeqrf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_dgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_zgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_dormhr": (_i, [_s, _s, _sz, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "rnpx_zunmhr": (_i, [_s, _s, _sz, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "rnpx_dorghr": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_zunghr": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_dorghr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p]),
+    "rnpx_zunghr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p]),
     "rnpx_dgeqlf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_zgeqlf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_dgerqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
@@ -423,6 +429,40 @@ def rq_generateq(a, tau, k=None):
     """RQ::GenerateQ: a (m x n, m <= n, reflectors in its last k rows)
     becomes the last m rows of Q."""
     return _fam_generateq("rq", a, tau, k)
+
+
+def hess_multq(side, trans, afac, tau, c, ilo, ihi):
+    """Hessenberg::MultQ: op(Q) C or C op(Q) for the Q of a Hessenberg
+    reduction in (afac, tau); 0-based ilo, exclusive ihi."""
+    c = np.array(c, dtype=np.result_type(c, afac, np.float64), order="F", copy=True)
+    afac = np.asfortranarray(afac, dtype=c.dtype)
+    tau = np.ascontiguousarray(tau, dtype=c.dtype)
+    m, n = c.shape
+    fn = getattr(lib(), "rnpx_dormhr" if _pre(c.dtype) == "d" else "rnpx_zunmhr")
+    lw = _sz(0)
+    _check(fn(side.encode(), trans.encode(), m, n, ilo, ihi, _ptr(afac), afac.shape[0], _ptr(tau), _ptr(c),
+              max(m, 1), ctypes.byref(lw), None), "Hessenberg::MultQ query")
+    work = np.zeros(max(lw.value, m, n, 1), dtype=c.dtype)
+    lw = _sz(work.size)
+    _check(fn(side.encode(), trans.encode(), m, n, ilo, ihi, _ptr(afac), afac.shape[0], _ptr(tau), _ptr(c),
+              max(m, 1), ctypes.byref(lw), _ptr(work)), "Hessenberg::MultQ")
+    return c
+
+
+def hess_generateq(afac, tau, ilo, ihi):
+    """Hessenberg::GenerateQ: the n x n Q of a Hessenberg reduction."""
+    a = np.array(afac, dtype=np.result_type(afac, np.float64), order="F", copy=True)
+    n = a.shape[0]
+    tau = np.ascontiguousarray(tau, dtype=a.dtype)
+    if tau.size == 0:
+        tau = np.zeros(1, dtype=a.dtype)
+    fn = getattr(lib(), "rnpx_dorghr" if _pre(a.dtype) == "d" else "rnpx_zunghr")
+    lw = _sz(0)
+    _check(fn(n, ilo, ihi, _ptr(a), max(n, 1), _ptr(tau), ctypes.byref(lw), None), "Hessenberg::GenerateQ query")
+    work = np.zeros(max(lw.value, n, 1), dtype=a.dtype)
+    lw = _sz(work.size)
+    _check(fn(n, ilo, ihi, _ptr(a), max(n, 1), _ptr(tau), ctypes.byref(lw), _ptr(work)), "Hessenberg::GenerateQ")
+    return a
 
 
 def version():

@@ -327,6 +327,70 @@ int gexxf_entry(Fam f, size_t m, size_t n, T* a, size_t lda, T* tau, size_t* lwo
   return dropin_gexxf<T>(f, m, n, a, lda, tau, lwork);
 }
 
+// Hessenberg drop-ins (RNP/LA/Hessenberg.hpp): the checks of MultQ
+// (Hessenberg.hpp:488-494; the query is QR::MultQ's, which returns without
+// one when nh = 0) and GenerateQ (Hessenberg.hpp:545-556: query nh nb).
+template <typename T>
+int dropin_ormhr(const char* side, const char* trans, size_t m, size_t n, size_t ilo, size_t ihi, const T* a,
+                 size_t lda, const T* tau, T* c, size_t ldc, size_t* lwork, bool query) {
+  if (!side || (side[0] != 'L' && side[0] != 'R')) return -1;
+  if (!trans || (trans[0] != 'N' && trans[0] != 'C' && trans[0] != 'T')) return -2;
+  const bool left = ('L' == side[0]);
+  const size_t nq = left ? m : n, nw = left ? n : m;
+  if (ilo > nq) return -5;
+  if (ihi <= ilo || ihi > nq) return -6;
+  if (!lwork) return -12;
+  if (!(*lwork == 0 || *lwork >= nw)) return -12;
+  const size_t nh = ihi - ilo - 1;
+  if (ldc < m || ldc == 0) return -11;
+  if (m == 0 || n == 0 || nh == 0) return RNPX_OK;
+  if (lda < nq) return -8;
+  if (query) {
+    *lwork = kRefNb * nw + kRefNb * kRefNb;
+    return RNPX_OK;
+  }
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU, C;
+  if ((st = A.in(h, h->stage1, const_cast<T*>(a), nq, nq, lda, true))) return st;
+  const size_t tl = nq > 1 ? nq - 1 : 1;  // tau holds nq-1 scalars (Hessenberg.hpp:466-467)
+  if ((st = TAU.in(h, h->stage2, const_cast<T*>(tau), tl, 1, tl, nq > 1))) return st;
+  if ((st = C.in(h, h->stage3, c, m, n, ldc, true))) return st;
+  st = ormhr_dev<T>(h, side[0], trans[0], (int)m, (int)n, (int)ilo, (int)ihi, A.dev, A.ld, TAU.dev, C.dev, C.ld);
+  if (st) return st;
+  if ((st = C.out(h, ldc))) return st;
+  return sync_handle(h);
+}
+
+template <typename T>
+int dropin_orghr(size_t n, size_t ilo, size_t ihi, T* a, size_t lda, const T* tau, size_t* lwork, bool query) {
+  if (ilo > n) return -2;
+  if (ihi <= ilo || ihi > n) return -3;
+  if (lda < n || lda == 0) return -5;
+  if (!lwork) return -7;
+  const size_t nh = ihi - ilo - 1;
+  if (!(*lwork == 0 || *lwork >= nh)) return -7;
+  if (query) {
+    *lwork = nh * kRefNb;
+    return RNPX_OK;
+  }
+  if (n == 0) return RNPX_OK;
+  if (!fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, TAU;
+  if ((st = A.in(h, h->stage1, a, n, n, lda, true))) return st;
+  const size_t tl = n > 1 ? n - 1 : 1;  // tau holds n-1 scalars (Hessenberg.hpp:535)
+  if ((st = TAU.in(h, h->stage2, const_cast<T*>(tau), tl, 1, tl, n > 1))) return st;
+  st = orghr_dev<T>(h, (int)n, (int)ilo, (int)ihi, A.dev, A.ld, TAU.dev);
+  if (st) return st;
+  if ((st = A.out(h, lda))) return st;
+  return sync_handle(h);
+}
+
 Handle* H(rnpx_handle h) { return reinterpret_cast<Handle*>(h); }
 
 }  // namespace
@@ -447,6 +511,27 @@ int rnpx_dorgrq(size_t m, size_t n, size_t k, double* a, size_t lda, const doubl
 int rnpx_zungrq(size_t m, size_t n, size_t k, rnpx_zcomplex* a, size_t lda, const rnpx_zcomplex* tau, size_t* lwork,
                 rnpx_zcomplex* work) {
   return dropin_orgxx<zd>(Fam::RQ, m, n, k, reinterpret_cast<zd*>(a), lda, reinterpret_cast<const zd*>(tau), lwork,
+                          lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_dormhr(const char* side, const char* trans, size_t m, size_t n, size_t ilo, size_t ihi, const double* a,
+                size_t lda, const double* tau, double* c, size_t ldc, size_t* lwork, double* work) {
+  return dropin_ormhr<double>(side, trans, m, n, ilo, ihi, a, lda, tau, c, ldc, lwork,
+                              lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zunmhr(const char* side, const char* trans, size_t m, size_t n, size_t ilo, size_t ihi,
+                const rnpx_zcomplex* a, size_t lda, const rnpx_zcomplex* tau, rnpx_zcomplex* c, size_t ldc,
+                size_t* lwork, rnpx_zcomplex* work) {
+  return dropin_ormhr<zd>(side, trans, m, n, ilo, ihi, reinterpret_cast<const zd*>(a), lda,
+                          reinterpret_cast<const zd*>(tau), reinterpret_cast<zd*>(c), ldc, lwork,
+                          lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_dorghr(size_t n, size_t ilo, size_t ihi, double* a, size_t lda, const double* tau, size_t* lwork,
+                double* work) {
+  return dropin_orghr<double>(n, ilo, ihi, a, lda, tau, lwork, lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zunghr(size_t n, size_t ilo, size_t ihi, rnpx_zcomplex* a, size_t lda, const rnpx_zcomplex* tau,
+                size_t* lwork, rnpx_zcomplex* work) {
+  return dropin_orghr<zd>(n, ilo, ihi, reinterpret_cast<zd*>(a), lda, reinterpret_cast<const zd*>(tau), lwork,
                           lwork && (work == nullptr || *lwork == 0));
 }
 int rnpx_dgelqf(size_t m, size_t n, double* a, size_t lda, double* tau, size_t* lwork, double* work) {
@@ -572,6 +657,16 @@ int rnpx_dgerqf_dev(rnpx_handle h, size_t m, size_t n, double* a, size_t lda, do
 int rnpx_zgerqf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex* tau) {
   RNPX_DEV_CHECK(h, m, n);
   return gerqf_dev<zd>(H(h), (int)m, (int)n, reinterpret_cast<zd*>(a), (long)lda, reinterpret_cast<zd*>(tau));
+}
+int rnpx_dorghr_dev(rnpx_handle h, size_t n, size_t ilo, size_t ihi, double* a, size_t lda, const double* tau) {
+  RNPX_DEV_CHECK(h, n, n);
+  return orghr_dev<double>(H(h), (int)n, (int)ilo, (int)ihi, a, (long)lda, tau);
+}
+int rnpx_zunghr_dev(rnpx_handle h, size_t n, size_t ilo, size_t ihi, rnpx_zcomplex* a, size_t lda,
+                    const rnpx_zcomplex* tau) {
+  RNPX_DEV_CHECK(h, n, n);
+  return orghr_dev<zd>(H(h), (int)n, (int)ilo, (int)ihi, reinterpret_cast<zd*>(a), (long)lda,
+                       reinterpret_cast<const zd*>(tau));
 }
 int rnpx_dormqr_dev(rnpx_handle h, const char* side, const char* trans, size_t m, size_t n, size_t k, const double* a,
                     size_t lda, const double* tau, double* c, size_t ldc) {

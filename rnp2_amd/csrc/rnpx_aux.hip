@@ -211,11 +211,45 @@ hipError_t launch_flip_copy(int rows, int cols, const T* A, long lda, T* B, long
   return hipGetLastError();
 }
 
+// Hessenberg::GenerateQ's set-up (RNP/LA/Hessenberg.hpp:555-580) on the
+// n x n A: identity outside rows/columns [ilo+1, ihi), and inside that
+// block the reflectors shifted one column right (V(i, j) = old A(i, j-1),
+// from the copy S of the old block A[ilo+1.., ilo..], below the diagonal;
+// zero on and above it).
+template <typename T>
+__global__ __launch_bounds__(256) void hessq_setup_kernel(int n, int ilo, int ihi, T* A, long lda, const T* Sv,
+                                                          long lds) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int i = (int)(e % n), j = (int)(e / n);
+  const bool in_i = (i > ilo && i < ihi), in_j = (j > ilo && j < ihi);
+  T v = S<T>::zero();
+  if (in_i && in_j) {
+    if (i > j) v = Sv[(size_t)(i - ilo - 1) + (size_t)(j - 1 - ilo) * lds];
+  } else if (i == j) {
+    v = S<T>::one();
+  }
+  A[(size_t)i + (size_t)j * lda] = v;
+}
+
+template <typename T>
+hipError_t launch_hessq_setup(int n, int ilo, int ihi, T* A, long lda, const T* Sv, long lds, hipStream_t st) {
+  if (n < 0 || ilo < 0 || ilo >= ihi || ihi > n || lda < std::max(1, n) || lds < std::max(1, ihi - ilo - 1))
+    return hipErrorInvalidValue;
+  const long long total = (long long)n * n;
+  if (total == 0) return hipSuccess;
+  const long long blocks = (total + 255) / 256;
+  if (blocks > 2147483647LL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((hessq_setup_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, n, ilo, ihi, A, lda, Sv, lds);
+  return hipGetLastError();
+}
+
 #define RNPX_INST(T)                                                                  \
   template hipError_t launch_set<T>(int, int, T, T, T*, long, hipStream_t);           \
   template hipError_t launch_copy<T>(int, int, const T*, long, T*, long, hipStream_t); \
   template hipError_t launch_conj_transpose<T>(int, int, const T*, long, T*, long, hipStream_t); \
   template hipError_t launch_flip<T>(int, int, T*, long, bool, bool, hipStream_t);    \
+  template hipError_t launch_hessq_setup<T>(int, int, int, T*, long, const T*, long, hipStream_t); \
   template hipError_t launch_flip_copy<T>(int, int, const T*, long, T*, long, bool, bool, hipStream_t); \
   template hipError_t launch_extract_v<T>(int, int, const T*, long, T*, long, hipStream_t); \
   template hipError_t launch_zero_rows<T>(int, int, T*, long, hipStream_t);           \

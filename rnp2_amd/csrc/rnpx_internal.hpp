@@ -129,6 +129,12 @@ template <typename T>
 int ormrq_dev(Handle* h, char side, char trans, int m, int n, int k, const T* A, long lda, const T* tau, T* C,
               long ldc);
 template <typename T> int orgrq_dev(Handle* h, int m, int n, int k, T* A, long lda, const T* tau);
+// Hessenberg::MultQ and GenerateQ (rnpx_hess.cpp): the QR routines on the
+// reflector block one row below the diagonal.
+template <typename T>
+int ormhr_dev(Handle* h, char side, char trans, int m, int n, int ilo, int ihi, const T* A, long lda, const T* tau,
+              T* C, long ldc);
+template <typename T> int orghr_dev(Handle* h, int n, int ilo, int ihi, T* A, long lda, const T* tau);
 template <typename T>
 int larft_dev(Handle* h, int n, int k, const T* V, long ldv, const T* tau, T* Tout, long ldt);
 template <typename T>

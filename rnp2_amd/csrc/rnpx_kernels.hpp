@@ -168,6 +168,10 @@ hipError_t launch_copy(int m, int n, const T* A, long lda, T* B, long ldb, hipSt
 // into B (rows x cols, leading dimension ldb).
 template <typename T>
 hipError_t launch_flip(int rows, int cols, T* A, long lda, bool fr, bool fc, hipStream_t st);
+// Hessenberg::GenerateQ's set-up of the n x n A from S, a copy of the old
+// reflector block A[ilo+1..ihi-1, ilo..ihi-2] (ld lds).
+template <typename T>
+hipError_t launch_hessq_setup(int n, int ilo, int ihi, T* A, long lda, const T* S, long lds, hipStream_t st);
 template <typename T>
 hipError_t launch_flip_copy(int rows, int cols, const T* A, long lda, T* B, long ldb, bool fr, bool fc, hipStream_t st);
 

@@ -313,3 +313,22 @@ def test_cpp_ql_rq_surfaces_compile_and_query():
     assert "Assertion failed in" in out and "lda >= m" in out
     if not has_gpu():
         assert f"status {R.RNPX_ERR_NOGPU}" in out and "rnpx: QL::Factor: status" in out
+
+
+def test_hessenberg_q_workspace_queries_and_errors():
+    """Hessenberg::GenerateQ's query nh*nb (Hessenberg.hpp:553-556) and
+    MultQ's pass-through of QR::MultQ's (nb*nw + nb^2; none when nh = 0)."""
+    L = R.lib()
+    sz = ctypes.c_size_t
+    lw = sz(0)
+    assert L.rnpx_dorghr(10, 2, 8, None, 10, None, ctypes.byref(lw), None) == 0 and lw.value == 5 * 32
+    lw = sz(0)
+    assert L.rnpx_zunghr(10, 0, 10, None, 10, None, ctypes.byref(lw), None) == 0 and lw.value == 9 * 32
+    lw = sz(0)
+    assert L.rnpx_dormhr(b"L", b"N", 10, 5, 2, 8, None, 10, None, None, 10, ctypes.byref(lw), None) == 0
+    assert lw.value == 32 * 5 + 32 * 32
+    lw = sz(3)
+    assert L.rnpx_zunmhr(b"R", b"C", 6, 10, 4, 5, None, 10, None, None, 6, ctypes.byref(sz(0)), None) == 0
+    assert L.rnpx_dorghr(10, 8, 8, None, 10, None, ctypes.byref(sz(0)), None) == -3        # ilo == ihi
+    assert L.rnpx_dorghr(10, 2, 11, None, 10, None, ctypes.byref(sz(0)), None) == -3       # ihi > n
+    assert L.rnpx_dormhr(b"X", b"N", 10, 5, 2, 8, None, 10, None, None, 10, ctypes.byref(sz(0)), None) == -1

@@ -811,3 +811,63 @@ def test_ql_rq_workspace_queries():
     assert O.rq_query("multq", "R", "C", 40, 9, 5) == 32 * 40 + 32 * 32
     assert O.ql_query("generateq", 40, 30, 20) == 30 * 32
     assert O.rq_query("generateq", 30, 40, 20) == 30 * 32
+
+
+# ---------------------------------------------------------------------------
+# Hessenberg::MultQ / GenerateQ (Hessenberg.hpp:478-588).  Inputs are LAPACK
+# gehrd reductions (RNP's 0-based ilo and exclusive ihi are LAPACK's ilo-1
+# and ihi); pins: LAPACK orghr/unghr for GenerateQ, Q^H A Q upper Hessenberg,
+# and the four MultQ products against the explicit Q.
+# ---------------------------------------------------------------------------
+
+HESS_CASES = [(12, 0, 12), (12, 2, 9), (40, 0, 40), (40, 5, 31), (7, 3, 4), (1, 0, 1)]
+
+
+def _gehrd(rng, n, ilo, ihi, cplx):
+    a = _rand(rng, n, n, cplx)
+    # outside [ilo, ihi) the matrix must already be triangular for gehrd's
+    # contract; zero those entries below the diagonal
+    for j in range(n):
+        for i in range(j + 1, n):
+            if j < ilo or i >= ihi:
+                a[i, j] = 0.0
+    ht, tau, info = (lapack.zgehrd if cplx else lapack.dgehrd)(a, lo=ilo, hi=max(ihi - 1, 0))
+    assert info == 0
+    return a, np.asfortranarray(ht), tau
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("n,ilo,ihi", HESS_CASES)
+def test_hess_generateq_matches_lapack_orghr(n, ilo, ihi, cplx):
+    rng = np.random.default_rng(n * 31 + ilo)
+    a, ht, tau = _gehrd(rng, n, ilo, ihi, cplx)
+    got = O.hess_generateq(ht.copy(), tau, ilo, ihi)
+    if n == 1:
+        qw = np.ones((1, 1))   # scipy's wrapper rejects the empty tau; Q = I
+    else:
+        qw, info = (lapack.zunghr if cplx else lapack.dorghr)(ht.copy(), tau, lo=ilo, hi=max(ihi - 1, 0))
+        assert info == 0
+    np.testing.assert_allclose(got, qw, atol=1e-13)
+    h = got.conj().T @ a @ got
+    assert np.abs(np.tril(h, -2)).max(initial=0.0) <= 1e-12 * max(1.0, np.abs(a).max())
+    np.testing.assert_allclose(got.conj().T @ got, np.eye(n), atol=1e-13)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("side,trans", [("L", "N"), ("L", "C"), ("R", "N"), ("R", "C")])
+def test_hess_multq_against_explicit_q(side, trans, cplx):
+    rng = np.random.default_rng(41)
+    for n, ilo, ihi in ((12, 2, 9), (40, 0, 40)):
+        a, ht, tau = _gehrd(rng, n, ilo, ihi, cplx)
+        Q = O.hess_generateq(ht.copy(), tau, ilo, ihi)
+        c = _rand(rng, n, 7, cplx) if side == "L" else _rand(rng, 7, n, cplx)
+        got = O.hess_multq(side, trans, ht, tau, c, ilo, ihi)
+        op = Q if trans == "N" else Q.conj().T
+        want = op @ c if side == "L" else c @ op
+        np.testing.assert_allclose(got, want, atol=1e-13 * max(1.0, np.abs(want).max()))
+
+
+def test_hess_workspace_queries():
+    assert O.hess_query("generateq", 10, 2, 8) == 5 * 32
+    assert O.hess_query("multq", "L", "N", 10, 5, 2, 8) == 32 * 5 + 32 * 32
+    assert O.hess_query("multq", "R", "C", 6, 10, 2, 8) == 32 * 6 + 32 * 32
