@@ -192,6 +192,22 @@ int rnpx_dorghr(size_t n, size_t ilo, size_t ihi, double *a, size_t lda, const d
 int rnpx_zunghr(size_t n, size_t ilo, size_t ihi, rnpx_zcomplex *a, size_t lda,
                 const rnpx_zcomplex *tau, size_t *lwork, rnpx_zcomplex *work);
 
+/* LU (RNP/LA/LU.hpp): partial pivoting, pivots size_t 0-based absolute rows
+ * (row i was interchanged with row pivots[i]).  Factor returns the
+ * reference's value (>= 0; LU.hpp:63-74 and 112-113: the last exactly-zero
+ * pivot column of the first panel that has one, with the reference's
+ * offsets) or a negative status. */
+/* LU::Factor<double> -- RNP/LA/LU.hpp:77-115 (dgetrf). */
+int rnpx_dgetrf(size_t m, size_t n, double *a, size_t lda, size_t *pivots);
+/* LU::Factor<std::complex<double>> -- RNP/LA/LU.hpp:77-115 (zgetrf). */
+int rnpx_zgetrf(size_t m, size_t n, rnpx_zcomplex *a, size_t lda, size_t *pivots);
+/* LU::Solve<double> -- RNP/LA/LU.hpp:117-144 (dgetrs): op(A) X = B from the factors. */
+int rnpx_dgetrs(const char *trans, size_t n, size_t nrhs, const double *a, size_t lda,
+                const size_t *ipiv, double *b, size_t ldb);
+/* LU::Solve<std::complex<double>> -- RNP/LA/LU.hpp:117-144 (zgetrs). */
+int rnpx_zgetrs(const char *trans, size_t n, size_t nrhs, const rnpx_zcomplex *a, size_t lda,
+                const size_t *ipiv, rnpx_zcomplex *b, size_t ldb);
+
 /* ---------------------------------------------------------------------
  * Handles (device-resident work on one HIP stream).
  * ------------------------------------------------------------------- */

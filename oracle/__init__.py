@@ -83,6 +83,11 @@ def _declare(L):
         getattr(L, f"orc_{pre}_hess_multq").argtypes = [ctypes.c_char_p, ctypes.c_char_p, _sz, _sz, _sz, _sz,
                                                         _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]
         getattr(L, f"orc_{pre}_hess_generateq").argtypes = [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]
+        for nm in ("lu_factor", "lu_factor_unblocked"):
+            f = getattr(L, f"orc_{pre}_{nm}")
+            f.restype = ctypes.c_int
+            f.argtypes = [_sz, _sz, _p, _sz, _p]
+        getattr(L, f"orc_{pre}_lu_solve").argtypes = [ctypes.c_char_p, _sz, _sz, _p, _sz, _p, _p, _sz]
         for fam in ("ql", "rq"):
             getattr(L, f"orc_{pre}_{fam}_factor").argtypes = [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]
             getattr(L, f"orc_{pre}_{fam}_multq").argtypes = [ctypes.c_char_p, ctypes.c_char_p, _sz, _sz, _sz,
@@ -528,6 +533,30 @@ def hess_query(what, *args, dtype=np.float64):
         getattr(lib(), f"orc_{_pre(dtype)}_hess_multq")(side.encode(), trans.encode(), m, n, ilo, ihi, None,
                                                          max(m, n, 1), None, None, max(m, 1), ctypes.byref(lw), None)
     return lw.value
+
+
+def lu_factor(a, unblocked=False):
+    """LU::Factor (LU.hpp:77-115): returns (LU packed, pivots (0-based
+    absolute rows, uint64), the reference's return value)."""
+    a = _fortran(a, np.result_type(a, np.float64))
+    pre = _pre(a.dtype)
+    m, n = a.shape
+    piv = np.zeros(max(min(m, n), 1), dtype=np.uint64)
+    f = getattr(lib(), f"orc_{pre}_lu_factor_unblocked" if unblocked else f"orc_{pre}_lu_factor")
+    info = f(m, n, _ptr(a), max(m, 1), piv.ctypes.data_as(ctypes.c_void_p))
+    return a, piv[:min(m, n)], info
+
+
+def lu_solve(trans, lu, piv, b):
+    """LU::Solve (LU.hpp:117-144): op(A) X = B from the factors."""
+    b = _fortran(b, np.result_type(b, lu, np.float64))
+    pre = _pre(b.dtype)
+    lu = _fortran(lu, b.dtype)
+    piv = np.ascontiguousarray(piv, dtype=np.uint64)
+    n, nrhs = b.shape
+    getattr(lib(), f"orc_{pre}_lu_solve")(trans.encode(), n, nrhs, _ptr(lu), lu.shape[0],
+                                          piv.ctypes.data_as(ctypes.c_void_p), _ptr(b), max(n, 1))
+    return b
 
 
 def ql_query(what, *args, dtype=np.float64):

@@ -175,6 +175,18 @@ void orc_z_hess_multq(const char *side, const char *trans, size_t m, size_t n, s
 void orc_z_hess_generateq(size_t n, size_t ilo, size_t ihi, orc_z *a, size_t lda, const orc_z *tau,
                           size_t *lwork, orc_z *work);
 
+/* ---- LU (RNP/LA/LU.hpp) ----
+ *   LU::Factor_unblocked LU.hpp:52-75, LU::Factor LU.hpp:77-115, LU::Solve LU.hpp:117-144
+ *   BLAS::MaximumIndex BLAS_mix.hpp:798-810, ApplyPermutations Permutation.hpp:33-52 */
+int orc_d_lu_factor_unblocked(size_t m, size_t n, double *a, size_t lda, size_t *piv);
+int orc_d_lu_factor(size_t m, size_t n, double *a, size_t lda, size_t *piv);
+void orc_d_lu_solve(const char *trans, size_t n, size_t nrhs, const double *a, size_t lda, const size_t *ipiv,
+                    double *b, size_t ldb);
+int orc_z_lu_factor_unblocked(size_t m, size_t n, orc_z *a, size_t lda, size_t *piv);
+int orc_z_lu_factor(size_t m, size_t n, orc_z *a, size_t lda, size_t *piv);
+void orc_z_lu_solve(const char *trans, size_t n, size_t nrhs, const orc_z *a, size_t lda, const size_t *ipiv,
+                    orc_z *b, size_t ldb);
+
 /* Least squares min ||A x - B|| for m >= n by Factor, MultQ("L","C") and an
  * upper-triangular back substitution with R (BLAS dtrsm "L","U","N","N";
  * the composition of QR.hpp:129, QR.hpp:337 and BLAS_mix.hpp:2096).

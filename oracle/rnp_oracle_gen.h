@@ -1094,6 +1094,155 @@ void FN(hess_generateq)(size_t n, size_t ilo, size_t ihi, T *a, size_t lda, cons
 	}
 }
 
+/* ---------------------------------------------------------------------
+ * LU (RNP/LA/LU.hpp): partial pivoting, pivots are 0-based absolute rows.
+ * ------------------------------------------------------------------- */
+
+/* Traits<T>::norm1: |re| + |im|. */
+static double FN(norm1s)(T x){ return fabs(RE(x)) + fabs(IM(x)); }
+
+/* BLAS::MaximumIndex, BLAS_mix.hpp:798-810: the first index of the largest
+ * |re| + |im| (a NaN is never preferred to the first entry). */
+static size_t FN(max_index)(size_t n, const T *x, size_t incx){
+	if(n < 1){ return 0; }
+	double mv = FN(norm1s)(x[0]);
+	size_t mi = 0;
+	for(size_t i = 1; i < n; ++i){
+		const double cv = FN(norm1s)(x[i*incx]);
+		if(cv > mv){ mi = i; mv = cv; }
+	}
+	return mi;
+}
+
+static void FN(swap_rows)(size_t ncols, T *a, size_t lda, size_t r1, size_t r2){
+	for(size_t c = 0; c < ncols; ++c){
+		const T t = a[r1+c*lda];
+		a[r1+c*lda] = a[r2+c*lda];
+		a[r2+c*lda] = t;
+	}
+}
+
+/* LU::Factor_unblocked, LU.hpp:52-75, with its return value as written
+ * there: the last column j whose pivot is exactly zero, returned as j+1,
+ * which reads as success (0) when that column is column 0. */
+int FN(lu_factor_unblocked)(size_t m, size_t n, T *a, size_t lda, size_t *piv){
+	int info = 0;
+	const size_t k = m < n ? m : n;
+	for(size_t j = 0; j < k; ++j){
+		const size_t jp = j + FN(max_index)(m-j, &a[j+j*lda], 1);
+		piv[j] = jp;
+		if(0.0 != a[jp+j*lda]){
+			if(jp != j){ FN(swap_rows)(n, a, lda, j, jp); }
+			const T r = 1.0 / a[j+j*lda];
+			for(size_t i = j+1; i < m; ++i){ a[i+j*lda] *= r; }
+		}else{
+			info = (int)j;
+		}
+		/* Rank1Update(m-j-1, n-j-1, -1, x = A(j+1:, j), y = A(j, j+1:)) */
+		for(size_t c = j+1; c < n; ++c){
+			const T t = -a[j+c*lda];
+			for(size_t i = j+1; i < m; ++i){ a[i+c*lda] += a[i+j*lda] * t; }
+		}
+	}
+	return info ? info+1 : 0;
+}
+
+/* SolveTrM("L","L","N","U"): B <- L^-1 B, L unit lower (m x m). */
+static void FN(trsm_llnu)(size_t m, size_t n, const T *a, size_t lda, T *b, size_t ldb){
+	for(size_t c = 0; c < n; ++c){
+		for(size_t k = 0; k < m; ++k){
+			const T bk = b[k+c*ldb];
+			if(0.0 == bk){ continue; }
+			for(size_t i = k+1; i < m; ++i){ b[i+c*ldb] -= bk * a[i+k*lda]; }
+		}
+	}
+}
+
+/* LU::Factor, LU.hpp:77-115: blocks of nb = 64 columns (LU::Tuning): the
+ * unblocked panel, pivots made absolute, the panel's swaps applied to the
+ * columns left and right of it (ApplyPermutations "L","F", Permutation.hpp:
+ * 33-52), U12 = L11^-1 A12, A22 -= L21 U12; unblocked throughout when
+ * min(m,n) <= 64.  The return value is the reference's arithmetic: the
+ * first panel whose Factor_unblocked reported iinfo > 0 gives info =
+ * iinfo + j, returned as info + 1. */
+int FN(lu_factor)(size_t m, size_t n, T *a, size_t lda, size_t *piv){
+	const size_t nb = 64;
+	const size_t k = m < n ? m : n;
+	if(k <= nb){ return FN(lu_factor_unblocked)(m, n, a, lda, piv); }
+	int info = 0;
+	const T one = 1, mone = -1;
+	for(size_t j = 0; j < k; j += nb){
+		const size_t jb = (k < nb+j ? k-j : nb);
+		const int iinfo = FN(lu_factor_unblocked)(m-j, jb, &a[j+j*lda], lda, &piv[j]);
+		if(0 == info && iinfo > 0){ info = iinfo + (int)j; }
+		const size_t ilim = (m < j+jb ? m : j+jb);
+		for(size_t i = j; i < ilim; ++i){ piv[i] += j; }
+		for(size_t i = j; i < j+jb; ++i){
+			if(piv[i] != i){ FN(swap_rows)(j, a, lda, i, piv[i]); }
+		}
+		if(j+jb < n){
+			for(size_t i = j; i < j+jb; ++i){
+				if(piv[i] != i){ FN(swap_rows)(n-j-jb, &a[(j+jb)*lda], lda, i, piv[i]); }
+			}
+			FN(trsm_llnu)(jb, n-j-jb, &a[j+j*lda], lda, &a[j+(j+jb)*lda], lda);
+			if(j+jb < m){
+				FN(gemm)("N", "N", m-j-jb, n-j-jb, jb, mone, &a[j+jb+j*lda], lda, &a[j+(j+jb)*lda], lda, one,
+				         &a[j+jb+(j+jb)*lda], lda);
+			}
+		}
+	}
+	return info ? info+1 : 0;
+}
+
+/* One triangular solve of LU::Solve (LU.hpp:117-144) on every column of b:
+ * op(M) x = b with M the unit lower (lower) or upper part of a. */
+static void FN(lu_trsv)(int lower, int unit, char op, size_t n, size_t nrhs, const T *a, size_t lda, T *b,
+                        size_t ldb){
+	for(size_t c = 0; c < nrhs; ++c){
+		T *x = &b[c*ldb];
+		/* op(M) is lower for (lower, N) and (upper, T/C): forward */
+		const int fwd = (lower && 'N' == op) || (!lower && 'N' != op);
+		for(size_t s = 0; s < n; ++s){
+			const size_t i = fwd ? s : n-1-s;
+			T acc = x[i];
+			for(size_t t = 0; t < s; ++t){
+				const size_t q = fwd ? t : n-1-t;  /* already solved */
+				T mij = ('N' == op) ? a[i+q*lda] : a[q+i*lda];
+				if('C' == op){ mij = CONJ(mij); }
+				acc -= mij * x[q];
+			}
+			if(!unit){
+				T d = a[i+i*lda];
+				if('C' == op){ d = CONJ(d); }
+				acc = acc / d;
+			}
+			x[i] = acc;
+		}
+	}
+}
+
+/* LU::Solve, LU.hpp:117-144: "N": the swaps forward, L (unit) then U;
+ * "T"/"C": op(U) then op(L), the swaps backward. */
+void FN(lu_solve)(const char *trans, size_t n, size_t nrhs, const T *a, size_t lda, const size_t *ipiv, T *b,
+                  size_t ldb){
+	if(0 == n || 0 == nrhs){ return; }
+	if('N' == trans[0]){
+		for(size_t i = 0; i < n; ++i){
+			if(ipiv[i] != i){ FN(swap_rows)(nrhs, b, ldb, i, ipiv[i]); }
+		}
+		FN(lu_trsv)(1, 1, 'N', n, nrhs, a, lda, b, ldb);
+		FN(lu_trsv)(0, 0, 'N', n, nrhs, a, lda, b, ldb);
+	}else{
+		const char op = ('T' == trans[0]) ? 'T' : 'C';
+		FN(lu_trsv)(0, 0, op, n, nrhs, a, lda, b, ldb);
+		FN(lu_trsv)(1, 1, op, n, nrhs, a, lda, b, ldb);
+		size_t i = n;
+		while(i-- > 0){
+			if(ipiv[i] != i){ FN(swap_rows)(nrhs, b, ldb, i, ipiv[i]); }
+		}
+	}
+}
+
 /* Least squares by QR: Factor (QR.hpp:129), MultQ("L","C") on B
  * (QR.hpp:337), then R x = (Q^H B)(0:n,:) by back substitution
  * (SolveTrM "L","U","N","N", BLAS_mix.hpp:2096). */

@@ -104,6 +104,10 @@ SIGNATURES = {
     "rnpx_zgeqrf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_dgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_zgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_dgetrf": (_i, [_sz, _sz, _p, _sz, _p]),
+    "rnpx_zgetrf": (_i, [_sz, _sz, _p, _sz, _p]),
+    "rnpx_dgetrs": (_i, [_s, _sz, _sz, _p, _sz, _p, _p, _sz]),
+    "rnpx_zgetrs": (_i, [_s, _sz, _sz, _p, _sz, _p, _p, _sz]),
     "rnpx_dormhr": (_i, [_s, _s, _sz, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
     "rnpx_zunmhr": (_i, [_s, _s, _sz, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
     "rnpx_dorghr": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
@@ -429,6 +433,32 @@ def rq_generateq(a, tau, k=None):
     """RQ::GenerateQ: a (m x n, m <= n, reflectors in its last k rows)
     becomes the last m rows of Q."""
     return _fam_generateq("rq", a, tau, k)
+
+
+def lu_factor(a):
+    """LU::Factor (LU.hpp:77-115) of a numpy matrix on the GPU: returns (LU
+    packed, pivots (uint64, 0-based absolute rows), the reference's return
+    value)."""
+    a = np.array(a, dtype=np.result_type(a, np.float64), order="F", copy=True)
+    m, n = a.shape
+    piv = np.zeros(max(min(m, n), 1), dtype=np.uint64)
+    fn = getattr(lib(), f"rnpx_{_pre(a.dtype)}getrf")
+    info = fn(m, n, _ptr(a), max(m, 1), piv.ctypes.data_as(ctypes.c_void_p))
+    if info < 0:
+        _check(info, "LU::Factor")
+    return a, piv[:min(m, n)], info
+
+
+def lu_solve(trans, lu, piv, b):
+    """LU::Solve (LU.hpp:117-144): op(A) X = B from the factors."""
+    b = np.array(b, dtype=np.result_type(b, lu, np.float64), order="F", copy=True)
+    lu = np.asfortranarray(lu, dtype=b.dtype)
+    piv = np.ascontiguousarray(piv, dtype=np.uint64)
+    n, nrhs = b.shape
+    fn = getattr(lib(), f"rnpx_{_pre(b.dtype)}getrs")
+    _check(fn(trans.encode(), n, nrhs, _ptr(lu), lu.shape[0], piv.ctypes.data_as(ctypes.c_void_p), _ptr(b),
+              max(n, 1)), "LU::Solve")
+    return b
 
 
 def hess_multq(side, trans, afac, tau, c, ilo, ihi):

@@ -20,13 +20,13 @@ ARCH = "gfx950"
 
 SOURCES = [
     "rnpx_panel.hip",
-    "rnpx_cqr.hip", "rnpx_trsm.hip",
+    "rnpx_cqr.hip", "rnpx_trsm.hip", "rnpx_lu.hip",
     "rnpx_tfactor.hip",
     "rnpx_gemm.hip",
     "rnpx_aux.hip",
     "rnpx_batched.hip",
     "rnpx_mixed.hip",
-    "rnpx_qr.cpp", "rnpx_blas.cpp", "rnpx_qlrq.cpp", "rnpx_hess.cpp",
+    "rnpx_qr.cpp", "rnpx_blas.cpp", "rnpx_qlrq.cpp", "rnpx_hess.cpp", "rnpx_lu_driver.cpp",
     "rnpx_lq.cpp",
     "rnpx_ls.cpp",
     "rnpx_serve.cpp",

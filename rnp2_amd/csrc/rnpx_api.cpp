@@ -391,6 +391,57 @@ int dropin_orghr(size_t n, size_t ilo, size_t ihi, T* a, size_t lda, const T* ta
   return sync_handle(h);
 }
 
+// LU drop-ins (RNP/LA/LU.hpp): Factor returns the reference's value
+// (LU.hpp:63-74, 112-113; >= 0) or a negative status; pivots are size_t,
+// 0-based absolute rows, as there.
+template <typename T>
+int dropin_getrf(size_t m, size_t n, T* a, size_t lda, size_t* pivots) {
+  if (lda < m || lda == 0) return -4;
+  const size_t k = m < n ? m : n;
+  if (k == 0) return RNPX_OK;
+  if (!pivots) return -5;
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A;
+  if ((st = A.in(h, h->stage1, a, m, n, lda, true))) return st;
+  std::vector<int> piv(k);
+  int info = 0;
+  st = getrf_dev<T>(h, (int)m, (int)n, A.dev, A.ld, piv.data(), &info);
+  if (st) return st;
+  if ((st = A.out(h, lda))) return st;
+  if ((st = sync_handle(h))) return st;
+  for (size_t i = 0; i < k; ++i) pivots[i] = (size_t)piv[i];
+  return info;
+}
+
+template <typename T>
+int dropin_getrs(const char* trans, size_t n, size_t nrhs, const T* a, size_t lda, const size_t* ipiv, T* b,
+                 size_t ldb) {
+  if (!trans || (trans[0] != 'N' && trans[0] != 'T' && trans[0] != 'C')) return -1;
+  if (n == 0 || nrhs == 0) return RNPX_OK;
+  if (lda < n) return -5;
+  if (!ipiv) return -6;
+  if (ldb < n) return -8;
+  if (!fits_int(n) || !fits_int(nrhs)) return RNPX_ERR_UNSUPPORTED;
+  std::vector<int> piv(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (ipiv[i] >= n) return -6;
+    piv[i] = (int)ipiv[i];
+  }
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, B;
+  if ((st = A.in(h, h->stage1, const_cast<T*>(a), n, n, lda, true))) return st;
+  if ((st = B.in(h, h->stage3, b, n, nrhs, ldb, true))) return st;
+  st = getrs_dev<T>(h, trans[0], (int)n, (int)nrhs, A.dev, A.ld, piv.data(), B.dev, B.ld);
+  if (st) return st;
+  if ((st = B.out(h, ldb))) return st;
+  return sync_handle(h);
+}
+
 Handle* H(rnpx_handle h) { return reinterpret_cast<Handle*>(h); }
 
 }  // namespace
@@ -533,6 +584,20 @@ int rnpx_zunghr(size_t n, size_t ilo, size_t ihi, rnpx_zcomplex* a, size_t lda, 
                 size_t* lwork, rnpx_zcomplex* work) {
   return dropin_orghr<zd>(n, ilo, ihi, reinterpret_cast<zd*>(a), lda, reinterpret_cast<const zd*>(tau), lwork,
                           lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_dgetrf(size_t m, size_t n, double* a, size_t lda, size_t* pivots) {
+  return dropin_getrf<double>(m, n, a, lda, pivots);
+}
+int rnpx_zgetrf(size_t m, size_t n, rnpx_zcomplex* a, size_t lda, size_t* pivots) {
+  return dropin_getrf<zd>(m, n, reinterpret_cast<zd*>(a), lda, pivots);
+}
+int rnpx_dgetrs(const char* trans, size_t n, size_t nrhs, const double* a, size_t lda, const size_t* ipiv, double* b,
+                size_t ldb) {
+  return dropin_getrs<double>(trans, n, nrhs, a, lda, ipiv, b, ldb);
+}
+int rnpx_zgetrs(const char* trans, size_t n, size_t nrhs, const rnpx_zcomplex* a, size_t lda, const size_t* ipiv,
+                rnpx_zcomplex* b, size_t ldb) {
+  return dropin_getrs<zd>(trans, n, nrhs, reinterpret_cast<const zd*>(a), lda, ipiv, reinterpret_cast<zd*>(b), ldb);
 }
 int rnpx_dgelqf(size_t m, size_t n, double* a, size_t lda, double* tau, size_t* lwork, double* work) {
   if (lwork && (work == nullptr || *lwork == 0)) {

@@ -73,45 +73,6 @@ __device__ __forceinline__ void tile_of(int t, int& ti, int& tj) {
   tj = ti + (t - first);
 }
 
-// Wait until this wave's global stores have completed (vmcnt(0)).
-__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-
-// The workgroup arrives on an agent-scope counter after all its global
-// stores are done (release by thread 0, acquire after); returns the count
-// before this arrival.
-__device__ __forceinline__ unsigned arrive(unsigned* counter, unsigned* slot) {
-  wait_stores();
-  __syncthreads();
-  if (threadIdx.x == 0) *slot = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  return *slot;
-}
-
-// Set a flag after all the workgroup's global stores are done.
-__device__ __forceinline__ void publish(unsigned* flag) {
-  wait_stores();
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Bounded wait for a flag (abort flag and ~2 s wall-clock limit), then
-// acquire for the whole workgroup.
-__device__ __forceinline__ void wait_flag(unsigned* flag, unsigned* abortf) {
-  if (threadIdx.x == 0) {
-    const uint64_t t0 = wall_clock64();
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-      if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
-      if (wall_clock64() - t0 > kBarrierTimeoutTicks) {
-        __hip_atomic_store(abortf, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-}
-
 // Partial Gram P = Ab^T Ab of this workgroup's RW rows; tile t of the 10 is
 // written row-major (16x16) to out[t*256 ...].
 template <int RW>

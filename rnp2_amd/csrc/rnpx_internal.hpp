@@ -56,12 +56,13 @@ struct Handle {
   Buffer trsm_flags;                       // per-block flags of the blocked triangular solve
   Buffer lqbuf;                            // A^H for the LQ routines
   Buffer qlbuf;                            // reversed reflectors and tau (QL, RQ)
+  Buffer lubuf;                            // LU panel scratch, Solve's permutation and gather
   unsigned trsm_epoch = 0;                 // value the flags take in the current solve
   // Every buffer above, for accounting and release.
   std::vector<Buffer*> buffers() {
     return {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc,
             &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
-            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf};
+            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf};
   }
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
@@ -135,6 +136,14 @@ template <typename T>
 int ormhr_dev(Handle* h, char side, char trans, int m, int n, int ilo, int ihi, const T* A, long lda, const T* tau,
               T* C, long ldc);
 template <typename T> int orghr_dev(Handle* h, int n, int ilo, int ihi, T* A, long lda, const T* tau);
+// LU::Factor and LU::Solve (rnpx_lu_driver.cpp); pivots 0-based absolute rows on
+// the host; getrf_dev synchronizes (its info needs the panels' results).
+template <typename T> int getrf_dev(Handle* h, int m, int n, T* A, long lda, int* piv_host, int* info);
+template <typename T>
+int getrs_dev(Handle* h, char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb);
+// op(M) X = B on the handle's stream (rnpx_ls.cpp), M the uplo triangle of R.
+template <typename T>
+int trsm_tri_dev(Handle* h, char uplo, char op, char diag, int n, int nrhs, const T* R, long ldr, T* B, long ldb);
 template <typename T>
 int larft_dev(Handle* h, int n, int k, const T* V, long ldv, const T* tau, T* Tout, long ldt);
 template <typename T>

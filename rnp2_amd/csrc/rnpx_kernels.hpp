@@ -42,6 +42,79 @@ __device__ inline void grid_barrier(unsigned* counter, unsigned* abortf, unsigne
   __syncthreads();
 }
 
+// Cross-workgroup hand-offs of the cooperative kernels (CholeskyQR2, LU panel).
+// Wait until this wave's global stores have completed (vmcnt(0)).
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// The workgroup arrives on an agent-scope counter after all its global
+// stores are done (release by thread 0, acquire after); returns the count
+// before this arrival.
+__device__ __forceinline__ unsigned arrive(unsigned* counter, unsigned* slot) {
+  wait_stores();
+  __syncthreads();
+  if (threadIdx.x == 0) *slot = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  return *slot;
+}
+
+// Set a flag after all the workgroup's global stores are done.
+__device__ __forceinline__ void publish(unsigned* flag) {
+  wait_stores();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded wait for a flag (abort flag and ~2 s wall-clock limit), then
+// acquire for the whole workgroup.
+__device__ __forceinline__ void wait_flag(unsigned* flag, unsigned* abortf) {
+  if (threadIdx.x == 0) {
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+      if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+      if (wall_clock64() - t0 > kBarrierTimeoutTicks) {
+        __hip_atomic_store(abortf, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// LU panel (rnpx_lu.hip): one cooperative launch factors an mp x jb panel
+// (jb <= 64) with partial pivoting.
+struct LuPanelArgs {
+  int mp, jb;           // panel rows and columns
+  void* A; long lda;    // panel top-left (T = double or zd)
+  int* piv;             // out: jb pivot rows, relative to the panel top
+  int* zero_last;       // out: the last column with an exact zero pivot, -1 if none
+  double* part;         // scratch: 2 * G * (2 + 2*64) doubles (candidates)
+  double* rowc;         // scratch: 2 * 2*64 doubles (row c)
+  double* res;          // scratch: 2*64 doubles (pivot row, winner per column)
+  unsigned* sync;       // 2 words per column, zero at launch
+  unsigned* abortf;     // sticky error flag
+};
+#define RNPX_LU_GLOBAL(a)                                                                              \
+  do {                                                                                                 \
+    (a).A = as_global((a).A); (a).piv = as_global((a).piv); (a).zero_last = as_global((a).zero_last); \
+    (a).part = as_global((a).part); (a).rowc = as_global((a).rowc); (a).res = as_global((a).res);      \
+    (a).sync = as_global((a).sync); (a).abortf = as_global((a).abortf);                               \
+  } while (0)
+constexpr int kLuPanelNB = 64;
+constexpr int kLuPartPerGroup = 2 + 2 * kLuPanelNB;
+// Rows per workgroup for an mp-row panel within max_groups workgroups (64,
+// 128 or 256; 0 when the panel is too tall).
+int lu_panel_rows(int mp, int max_groups, int cplx);
+template <typename T>
+hipError_t launch_lu_panel(const LuPanelArgs& a, int rw, hipStream_t st);
+template <typename T>
+hipError_t launch_lu_laswp(int j, int jb, const int* piv, T* A, long lda, int c0, int ncols, hipStream_t st);
+template <typename T>
+hipError_t launch_lu_trsm(int jb, const T* L, long ldl, T* B, long ldb, int ncols, hipStream_t st);
+template <typename T>
+hipError_t launch_lu_gather(int n, int nrhs, const int* src, const T* X, long ldx, T* Y, long ldy, hipStream_t st);
+
 template <typename T>
 struct PanelArgs {
   int mp, nbp;         // panel rows, panel columns (nbp <= NB)
@@ -196,6 +269,11 @@ hipError_t launch_larnv(int dist, const int iseed[4], long long count, T* x, hip
 // trans 'N' R X = B, 'C' R^H X = B.  flags: trsm_flag_words(n, nrhs) words
 // that never hold `epoch` at launch (epoch != 0, a new value per launch).
 int trsm_flag_words(int n, int nrhs);
+// op(M) X = B, M the upper ('U') or lower ('L') triangle of R, op 'N', 'T'
+// or 'C', unit diagonal for diag 'U' (same flags protocol as below).
+template <typename T>
+hipError_t launch_trsm_tri(char uplo, char op, char diag, int n, int nrhs, const T* R, long ldr, T* B, long ldb,
+                           unsigned* flags, unsigned epoch, unsigned* abortf, hipStream_t st);
 template <typename T>
 hipError_t launch_trsm_blocked(char trans, int n, int nrhs, const T* R, long ldr, T* B, long ldb, unsigned* flags,
                                unsigned epoch, unsigned* abortf, hipStream_t st);

@@ -24,11 +24,12 @@ namespace rnpx {
     if (_e != hipSuccess) return hip_status(_e);  \
   } while (0)
 
-// Triangular solve with the upper triangle R of A (n x n): trans 'N'
-// R X = B, 'C' R^H X = B, in place on B.  Each call takes a new flag epoch,
-// so a solve is not replayable from a captured graph.
+// Triangular solve op(M) X = B in place on B, M the upper ('U') or lower
+// ('L') triangle of R (n x n), op 'N', 'T' or 'C', unit diagonal for diag
+// 'U'.  Each call takes a new flag epoch, so a solve is not replayable from
+// a captured graph.
 template <typename T>
-static int trsm_dev(Handle* h, char trans, int n, int nrhs, const T* R, long ldr, T* B, long ldb) {
+int trsm_tri_dev(Handle* h, char uplo, char op, char diag, int n, int nrhs, const T* R, long ldr, T* B, long ldb) {
   if (n == 0 || nrhs == 0) return RNPX_OK;
   const size_t words = (size_t)trsm_flag_words(n, nrhs);
   const size_t before = h->trsm_flags.bytes;
@@ -39,9 +40,17 @@ static int trsm_dev(Handle* h, char trans, int n, int nrhs, const T* R, long ldr
     h->trsm_epoch = 0;
   }
   ++h->trsm_epoch;
-  RNPX_TRY(launch_trsm_blocked<T>(trans, n, nrhs, R, ldr, B, ldb, (unsigned*)h->trsm_flags.p, h->trsm_epoch,
-                                  (unsigned*)h->flag.p, h->stream));
+  RNPX_TRY(launch_trsm_tri<T>(uplo, op, diag, n, nrhs, R, ldr, B, ldb, (unsigned*)h->trsm_flags.p, h->trsm_epoch,
+                              (unsigned*)h->flag.p, h->stream));
   return RNPX_OK;
+}
+template int trsm_tri_dev<double>(Handle*, char, char, char, int, int, const double*, long, double*, long);
+template int trsm_tri_dev<zd>(Handle*, char, char, char, int, int, const zd*, long, zd*, long);
+
+// With the upper triangle R of A: trans 'N' R X = B, 'C' R^H X = B.
+template <typename T>
+static int trsm_dev(Handle* h, char trans, int n, int nrhs, const T* R, long ldr, T* B, long ldb) {
+  return trsm_tri_dev<T>(h, 'U', trans, 'N', n, nrhs, R, ldr, B, ldb);
 }
 
 template <typename T>

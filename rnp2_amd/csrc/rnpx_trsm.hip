@@ -7,7 +7,9 @@
 //                            equations' first solve)
 //
 // R is upper triangular (n x n, leading dimension ldr) -- the upper triangle
-// of the factored A.
+// of the factored A.  The same kernel, templated on the triangle, the
+// operation (N, T, C) and a unit diagonal, does LU::Solve's four solves
+// (SolveTrM "L","L"/"U", LU.hpp:117-144).
 //
 // Blocked and sync-free: one workgroup per (64-row block p, right-hand side
 // c).  Workgroup w solves block p = nblk-1-w ('N') or p = w ('C'), so it
@@ -29,39 +31,45 @@ constexpr int TB = 64;  // block rows per workgroup
 
 __device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-template <typename T, bool CONJ>
-__global__ __launch_bounds__(256) void trsm_blocked_kernel(int n, const T* __restrict__ R, long ldr, T* B, long ldb,
-                                                           unsigned* flags, unsigned epoch, unsigned* abortf) {
-  __shared__ T Rs[TB][TB + 1];  // N: R_pp;  C: R_pp^H (lower)
+// op(M) X = B with M the upper (UPPER) or lower triangle of R, op = N, T
+// (transpose, OP = 1) or C (conjugate transpose, OP = 2), unit diagonal
+// when UNIT.  E = op(M) is upper when UPPER == (OP == 0): backward
+// substitution, blocks from the last; forward otherwise.
+template <typename T, bool UPPER, int OP, bool UNIT>
+__global__ __launch_bounds__(256) void trsm_tri_kernel(int n, const T* __restrict__ R, long ldr, T* B, long ldb,
+                                                       unsigned* flags, unsigned epoch, unsigned* abortf) {
+  constexpr bool EUP = (UPPER == (OP == 0));
+  __shared__ T Rs[TB][TB + 1];  // E's diagonal block, row-major
   __shared__ T rinv[TB];
   __shared__ T red[4][TB];
   __shared__ int bad;
   const int nblk = (n + TB - 1) / TB;
   const int w = blockIdx.x, c = blockIdx.y;
-  const int p = CONJ ? w : nblk - 1 - w;
+  const int p = EUP ? nblk - 1 - w : w;
   const int i0 = p * TB, bs = min(TB, n - i0);
   T* b = B + (size_t)c * ldb;
   unsigned* fl = flags + (size_t)c * nblk;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const T zero = S<T>::zero(), one = S<T>::one();
   if (t == 0) bad = 0;
-  // Diagonal block, identity beyond bs (coalesced down R's columns).
+  // Diagonal block E(i0 + r, i0 + q), identity beyond bs (read down R's columns).
   for (int e = t; e < TB * TB; e += 256) {
     const int r = e % TB, q = e / TB;  // R(i0 + r, i0 + q)
     T v = (r == q) ? one : zero;
-    if (r < bs && q < bs && r <= q) v = R[(size_t)(i0 + r) + (size_t)(i0 + q) * ldr];
-    if (CONJ)
-      Rs[q][r] = S<T>::conj(v);
-    else
+    const bool in_tri = UPPER ? (r <= q) : (r >= q);
+    if (r < bs && q < bs && in_tri && !(UNIT && r == q)) v = R[(size_t)(i0 + r) + (size_t)(i0 + q) * ldr];
+    if (OP == 0)
       Rs[r][q] = v;
+    else
+      Rs[q][r] = (OP == 2) ? S<T>::conj(v) : v;
   }
   __syncthreads();
-  if (t < TB) rinv[t] = recip_t(Rs[t][t]);
+  if (t < TB) rinv[t] = UNIT ? one : recip_t(Rs[t][t]);
 
   // Off-diagonal blocks, in the order they are finished.
   T acc = zero;
   for (int d = 0; d < w; ++d) {
-    const int q = CONJ ? d : nblk - 1 - d;
+    const int q = EUP ? nblk - 1 - d : d;
     if (t == 0) {
       const uint64_t t0 = wall_clock64();
       while (__hip_atomic_load(&fl[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
@@ -87,8 +95,13 @@ __global__ __launch_bounds__(256) void trsm_blocked_kernel(int n, const T* __res
         const int j = wave * (TB / 4) + jj;
         if (j < bq) {
           const T xj = b[j0 + j];
-          const T rij = CONJ ? S<T>::conj(R[(size_t)(j0 + j) + (size_t)(i0 + lane) * ldr])
-                             : R[(size_t)(i0 + lane) + (size_t)(j0 + j) * ldr];
+          T rij;
+          if (OP == 0) {
+            rij = R[(size_t)(i0 + lane) + (size_t)(j0 + j) * ldr];
+          } else {
+            rij = R[(size_t)(j0 + j) + (size_t)(i0 + lane) * ldr];
+            if (OP == 2) rij = S<T>::conj(rij);
+          }
           acc = acc + rij * xj;
         }
       }
@@ -99,13 +112,13 @@ __global__ __launch_bounds__(256) void trsm_blocked_kernel(int n, const T* __res
   if (wave != 0) return;
   T y = zero;
   if (lane < bs) y = b[i0 + lane] - ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
-  if (!CONJ) {
+  if (EUP) {
 #pragma unroll
     for (int i = TB - 1; i >= 0; --i) {
       const T ri = Rs[lane][i];
       const T yi = readlane_t(y, i);
       T xi = yi * rinv[i];
-      xi = xi + (yi - xi * Rs[i][i]) * rinv[i];
+      if (!UNIT) xi = xi + (yi - xi * Rs[i][i]) * rinv[i];
       if (lane == i) y = xi;
       else if (lane < i) y = y - ri * xi;
     }
@@ -115,7 +128,7 @@ __global__ __launch_bounds__(256) void trsm_blocked_kernel(int n, const T* __res
       const T ri = Rs[lane][i];
       const T yi = readlane_t(y, i);
       T xi = yi * rinv[i];
-      xi = xi + (yi - xi * Rs[i][i]) * rinv[i];
+      if (!UNIT) xi = xi + (yi - xi * Rs[i][i]) * rinv[i];
       if (lane == i) y = xi;
       else if (lane > i) y = y - ri * xi;
     }
@@ -129,25 +142,49 @@ __global__ __launch_bounds__(256) void trsm_blocked_kernel(int n, const T* __res
 
 int trsm_flag_words(int n, int nrhs) { return ((n + TB - 1) / TB) * nrhs; }
 
+template <typename T, bool UPPER, int OP, bool UNIT>
+static void launch_tri(const dim3& grid, hipStream_t st, int n, const T* R, long ldr, T* B, long ldb,
+                       unsigned* flags, unsigned epoch, unsigned* abortf) {
+  hipLaunchKernelGGL((trsm_tri_kernel<T, UPPER, OP, UNIT>), grid, dim3(256), 0, st, n, R, ldr, B, ldb, flags, epoch,
+                     abortf);
+}
+
 template <typename T>
-hipError_t launch_trsm_blocked(char trans, int n, int nrhs, const T* R, long ldr, T* B, long ldb, unsigned* flags,
-                               unsigned epoch, unsigned* abortf, hipStream_t st) {
+hipError_t launch_trsm_tri(char uplo, char op, char diag, int n, int nrhs, const T* R, long ldr, T* B, long ldb,
+                           unsigned* flags, unsigned epoch, unsigned* abortf, hipStream_t st) {
   if (n < 0 || nrhs < 0 || ldr < std::max(1, n) || ldb < std::max(1, n) || epoch == 0u || !flags || !abortf)
     return hipErrorInvalidValue;
-  if (trans != 'N' && trans != 'C') return hipErrorInvalidValue;
+  if ((uplo != 'U' && uplo != 'L') || (op != 'N' && op != 'T' && op != 'C') || (diag != 'N' && diag != 'U'))
+    return hipErrorInvalidValue;
   if (n == 0 || nrhs == 0) return hipSuccess;
   if (nrhs > 65535) return hipErrorInvalidValue;
   const dim3 grid((n + TB - 1) / TB, nrhs);
-  if (trans == 'N')
-    hipLaunchKernelGGL((trsm_blocked_kernel<T, false>), grid, dim3(256), 0, st, n, R, ldr, B, ldb, flags, epoch, abortf);
-  else
-    hipLaunchKernelGGL((trsm_blocked_kernel<T, true>), grid, dim3(256), 0, st, n, R, ldr, B, ldb, flags, epoch, abortf);
-  return hipGetLastError();
+  const bool up = (uplo == 'U'), unit = (diag == 'U');
+  const int o = (op == 'N') ? 0 : (op == 'T' ? 1 : 2);
+#define RNPX_TRI(U, O, D) \
+  if (up == U && o == O && unit == D) { launch_tri<T, U, O, D>(grid, st, n, R, ldr, B, ldb, flags, epoch, abortf); return hipGetLastError(); }
+  RNPX_TRI(true, 0, false) RNPX_TRI(true, 1, false) RNPX_TRI(true, 2, false)
+  RNPX_TRI(true, 0, true) RNPX_TRI(true, 1, true) RNPX_TRI(true, 2, true)
+  RNPX_TRI(false, 0, false) RNPX_TRI(false, 1, false) RNPX_TRI(false, 2, false)
+  RNPX_TRI(false, 0, true) RNPX_TRI(false, 1, true) RNPX_TRI(false, 2, true)
+#undef RNPX_TRI
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_trsm_blocked(char trans, int n, int nrhs, const T* R, long ldr, T* B, long ldb, unsigned* flags,
+                               unsigned epoch, unsigned* abortf, hipStream_t st) {
+  if (trans != 'N' && trans != 'C') return hipErrorInvalidValue;
+  return launch_trsm_tri<T>('U', trans, 'N', n, nrhs, R, ldr, B, ldb, flags, epoch, abortf, st);
 }
 
 template hipError_t launch_trsm_blocked<double>(char, int, int, const double*, long, double*, long, unsigned*, unsigned,
                                                 unsigned*, hipStream_t);
 template hipError_t launch_trsm_blocked<zd>(char, int, int, const zd*, long, zd*, long, unsigned*, unsigned, unsigned*,
                                             hipStream_t);
+template hipError_t launch_trsm_tri<double>(char, char, char, int, int, const double*, long, double*, long, unsigned*,
+                                            unsigned, unsigned*, hipStream_t);
+template hipError_t launch_trsm_tri<zd>(char, char, char, int, int, const zd*, long, zd*, long, unsigned*, unsigned,
+                                        unsigned*, hipStream_t);
 
 }  // namespace rnpx

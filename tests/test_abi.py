@@ -332,3 +332,16 @@ def test_hessenberg_q_workspace_queries_and_errors():
     assert L.rnpx_dorghr(10, 8, 8, None, 10, None, ctypes.byref(sz(0)), None) == -3        # ilo == ihi
     assert L.rnpx_dorghr(10, 2, 11, None, 10, None, ctypes.byref(sz(0)), None) == -3       # ihi > n
     assert L.rnpx_dormhr(b"X", b"N", 10, 5, 2, 8, None, 10, None, None, 10, ctypes.byref(sz(0)), None) == -1
+
+
+def test_lu_argument_errors_and_empty():
+    L = R.lib()
+    piv = np.zeros(4, dtype=np.uint64)
+    pp = piv.ctypes.data_as(ctypes.c_void_p)
+    assert L.rnpx_dgetrf(10, 5, None, 9, pp) == -4                    # lda < m
+    assert L.rnpx_zgetrf(0, 5, None, 1, pp) == 0                      # empty: nothing to do
+    assert L.rnpx_dgetrs(b"X", 4, 1, None, 4, pp, None, 4) == -1       # bad trans
+    assert L.rnpx_dgetrs(b"N", 0, 1, None, 1, pp, None, 1) == 0        # empty
+    assert L.rnpx_dgetrs(b"N", 4, 1, None, 3, pp, None, 4) == -5       # lda < n
+    bad = np.array([0, 9, 2, 3], dtype=np.uint64)
+    assert L.rnpx_dgetrs(b"N", 4, 1, None, 4, bad.ctypes.data_as(ctypes.c_void_p), None, 4) == -6  # pivot out of range

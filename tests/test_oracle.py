@@ -871,3 +871,77 @@ def test_hess_workspace_queries():
     assert O.hess_query("generateq", 10, 2, 8) == 5 * 32
     assert O.hess_query("multq", "L", "N", 10, 5, 2, 8) == 32 * 5 + 32 * 32
     assert O.hess_query("multq", "R", "C", 6, 10, 2, 8) == 32 * 6 + 32 * 32
+
+
+# ---------------------------------------------------------------------------
+# LU (LU.hpp): LAPACK getrf/getrs (same partial pivoting rule, so the same
+# pivots), P A = L U, and the reference's return value arithmetic.
+# ---------------------------------------------------------------------------
+
+LU_SHAPES = [(5, 5), (7, 5), (5, 7), (64, 64), (65, 65), (200, 200), (300, 150), (150, 300), (1, 1), (1, 9), (9, 1)]
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("shape", LU_SHAPES)
+def test_lu_factor_matches_lapack_getrf(shape, cplx):
+    m, n = shape
+    rng = np.random.default_rng(m * 7 + n)
+    a = _rand(rng, m, n, cplx)
+    lu, piv, info = O.lu_factor(a)
+    lw, pw, iw = (lapack.zgetrf if cplx else lapack.dgetrf)(a.copy())
+    assert info == 0 and iw == 0
+    k = min(m, n)
+    assert np.array_equal(piv, pw[:k].astype(np.uint64))
+    assert np.abs(lu - lw).max() <= 1e-12 * max(1.0, np.abs(a).max()) * max(m, n)
+    # P A = L U
+    L = np.tril(lu[:, :k], -1) + np.eye(m, k)
+    U = np.triu(lu[:k, :])
+    pa = a.copy()
+    for i in range(k):
+        pa[[i, int(piv[i])]] = pa[[int(piv[i]), i]]
+    np.testing.assert_allclose(L @ U, pa, atol=1e-12 * max(1.0, np.abs(a).max()) * max(m, n))
+
+
+def test_lu_unblocked_equals_blocked_to_rounding():
+    rng = np.random.default_rng(3)
+    a = _rand(rng, 150, 150, False)
+    lb, pb, _ = O.lu_factor(a)
+    lu_, pu, _ = O.lu_factor(a, unblocked=True)
+    assert np.array_equal(pb, pu)
+    assert np.abs(lb - lu_).max() <= 1e-12
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("trans", ["N", "T", "C"])
+def test_lu_solve_matches_lapack_getrs(trans, cplx):
+    rng = np.random.default_rng(5)
+    for n in (6, 130):
+        a = _rand(rng, n, n, cplx)
+        b = _rand(rng, n, 4, cplx)
+        lu, piv, _ = O.lu_factor(a)
+        x = O.lu_solve(trans, lu, piv, b)
+        op = a if trans == "N" else (a.T if trans == "T" else a.conj().T)
+        np.testing.assert_allclose(op @ x, b, atol=1e-11 * np.abs(b).max())
+        lw, pw, _ = (lapack.zgetrf if cplx else lapack.dgetrf)(a.copy())
+        tcode = {"N": 0, "T": 1, "C": 2}[trans]
+        xw, info = (lapack.zgetrs if cplx else lapack.dgetrs)(lw, pw, b.copy(), trans=tcode)
+        assert info == 0
+        np.testing.assert_allclose(x, xw, atol=1e-11 * max(1.0, np.abs(xw).max()))
+
+
+def test_lu_return_value_follows_the_reference():
+    """Factor_unblocked returns (last zero pivot j) + 1, which reads as 0 when
+    j = 0 (LU.hpp:63-74); the blocked Factor adds the panel offset and 1 more
+    (LU.hpp:94-96, 112-113).  LAPACK's getrf returns (first zero pivot) + 1."""
+    rng = np.random.default_rng(9)
+    a = _rand(rng, 20, 20, False)
+    a[:, 0] = 0.0
+    assert O.lu_factor(a)[2] == 0 and lapack.dgetrf(a.copy())[2] == 1
+    a = _rand(rng, 20, 20, False)
+    a[:, 6] = 0.0       # an exact zero pivot at j = 6
+    _, _, info = O.lu_factor(a)
+    assert info == 7
+    a = _rand(rng, 100, 100, False)
+    a[:, 70] = 0.0      # an exact zero pivot at j = 70, in the panel starting at 64
+    _, _, info = O.lu_factor(a)
+    assert info == 72 and lapack.dgetrf(a.copy())[2] == 71
