@@ -171,6 +171,41 @@ def leg_qlrq(R, torch, m=8192, n=4096, reps=3):
             "rq_of_transpose_ms": round(t_rq * 1e3, 3), "qr_ms": round(t_qr * 1e3, 3)}
 
 
+def leg_lu(R, torch, n=8192, reps=3):
+    """LU::Factor (RNP/LA/LU.hpp:77) of an n x n N(0,1) matrix through the C
+    ABI with the matrix resident on the device (pivots returned to the host),
+    and LU::Solve with 16 right-hand sides; flops 2n^3/3 and 2 n^2 nrhs."""
+    import ctypes
+    h = R.Handle()
+    a0 = R.colmajor_empty(n, n, torch.float64)
+    h.larnv(2, [1, 6, 1, 8], a0)
+    a = R.colmajor_empty(n, n, torch.float64)
+    h.synchronize()
+    L = R.lib()
+    piv = np.zeros(n, dtype=np.uint64)
+    pp = piv.ctypes.data_as(ctypes.c_void_p)
+    ts = []
+    for r in range(reps + 1):
+        a.copy_(a0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = L.rnpx_dgetrf(n, n, ctypes.c_void_p(a.data_ptr()), n, pp)
+        torch.cuda.synchronize()
+        if r:
+            ts.append(time.perf_counter() - t0)
+    assert info == 0, info
+    b = torch.randn(n, 16, dtype=torch.float64, device="cuda").t().contiguous().t()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = L.rnpx_dgetrs(b"N", n, 16, ctypes.c_void_p(a.data_ptr()), n, pp, ctypes.c_void_p(b.data_ptr()), n)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    assert st == 0, st
+    t = float(np.median(ts))
+    return {"shape": [n, n], "dtype": "f64", "factor_ms": round(t * 1e3, 3),
+            "factor_gflops": round((2.0 * n ** 3 / 3.0) / t / 1e9, 1), "solve16_ms": round(t_solve * 1e3, 3)}
+
+
 def c3_shapes(count, seed):
     rng = np.random.default_rng(seed)
     m = np.rint(np.exp(rng.uniform(np.log(16), np.log(256), count))).astype(int)
@@ -426,7 +461,7 @@ def main():
         "panel_hbm_gbs": round(pan["bytes"] / (pan["ms"] * 1e-3) / 1e9, 1) if pan["ms"] > 0 else None,
         "tau_sane": ok,
     }
-    legs = [] if args.legs == "none" else (["complex", "tall", "batched", "mixed", "serving", "lq", "qlrq"]
+    legs = [] if args.legs == "none" else (["complex", "tall", "batched", "mixed", "serving", "lq", "qlrq", "lu"]
                                            if args.legs == "all" else args.legs.split(","))
     extra = {}
     for leg in legs:
@@ -446,6 +481,8 @@ def main():
                 extra["lq_4096x8192"] = leg_lq(R, torch)
             elif leg == "qlrq":
                 extra["ql_rq_8192x4096"] = leg_qlrq(R, torch)
+            elif leg == "lu":
+                extra["lu_8192"] = leg_lu(R, torch)
         except Exception as e:  # a failing leg is reported, the headline line still prints
             extra[leg] = {"error": repr(e)}
     if extra:

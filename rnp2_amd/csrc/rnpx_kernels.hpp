@@ -91,14 +91,13 @@ struct LuPanelArgs {
   int* zero_last;       // out: the last column with an exact zero pivot, -1 if none
   double* part;         // scratch: 2 * G * (2 + 2*64) doubles (candidates)
   double* rowc;         // scratch: 2 * 2*64 doubles (row c)
-  double* res;          // scratch: 2*64 doubles (pivot row, winner per column)
-  unsigned* sync;       // 2 words per column, zero at launch
+  unsigned* sync;       // 2 words per column (an arrival counter used), zero at launch
   unsigned* abortf;     // sticky error flag
 };
 #define RNPX_LU_GLOBAL(a)                                                                              \
   do {                                                                                                 \
     (a).A = as_global((a).A); (a).piv = as_global((a).piv); (a).zero_last = as_global((a).zero_last); \
-    (a).part = as_global((a).part); (a).rowc = as_global((a).rowc); (a).res = as_global((a).res);      \
+    (a).part = as_global((a).part); (a).rowc = as_global((a).rowc);                                    \
     (a).sync = as_global((a).sync); (a).abortf = as_global((a).abortf);                               \
   } while (0)
 constexpr int kLuPanelNB = 64;

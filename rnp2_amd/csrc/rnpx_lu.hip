@@ -8,9 +8,9 @@
 //      BLAS_mix.hpp:798-810: the first row of largest |re|+|im|) and writes
 //      it, with that row, to a per-column-parity slot; the owner of row c
 //      also writes row c;
-//   2. the last workgroup to arrive reduces the candidates in workgroup
-//      order (ties to the lower row, so the result is the sequential scan's)
-//      and publishes the pivot;
+//   2. every workgroup waits until all have arrived and reduces the
+//      candidates itself, in workgroup order (ties to the lower row, so the
+//      result is the sequential scan's);
 //   3. every workgroup swaps (the owner of row c takes the pivot row, the
 //      owner of the pivot row takes old row c), scales its part of column c
 //      by 1/pivot and applies the rank-1 update to its rows.
@@ -55,8 +55,7 @@ __device__ __forceinline__ void st_part(double* p, T v) {
 }  // namespace
 
 // Scratch layout (doubles): part[2][G][kLuSlot] candidate slots (value,
-// row, then the row's kLuNB entries), rowc[2][2*kLuNB] (row c), res[kLuNB][2]
-// (pivot row and winning workgroup per column, as doubles).
+// row, then the row's kLuNB entries), rowc[2][2*kLuNB] (row c).
 constexpr int kLuSlot = 2 + 2 * kLuNB;
 
 template <typename T, int RW>
@@ -66,7 +65,6 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuPanelArgs a) {
   __shared__ T ubuf[NB], ocbuf[NB];
   __shared__ double redv[4];
   __shared__ int redi[4], redw[4];
-  __shared__ unsigned slot;
   __shared__ int pw[2];
   RNPX_LU_GLOBAL(a);
   T* A = reinterpret_cast<T*>(a.A);
@@ -134,9 +132,26 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuPanelArgs a) {
       for (int q = tid; q < NB; q += 256) st_part<T>(my + 2 + TE * q, Sl[bi - gr0][q]);
     if (c >= gr0 && c < gr0 + RW)
       for (int q = tid; q < NB; q += 256) st_part<T>(rowc + TE * q, Sl[c - gr0][q]);
-    unsigned* sw = a.sync + 2 * c;
-    if (arrive(&sw[0], &slot) == (unsigned)(G - 1)) {
-      // last arrival: reduce the candidates in workgroup order
+    // every workgroup waits for all the candidates, then reduces them in
+    // workgroup order itself (no second hand-off)
+    unsigned* cnt = a.sync + 2 * c;
+    wait_stores();
+    __syncthreads();
+    if (tid == 0) {
+      __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t t0 = wall_clock64();
+      while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (unsigned)G) {
+        if (__hip_atomic_load(a.abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+        if (wall_clock64() - t0 > kBarrierTimeoutTicks) {
+          __hip_atomic_store(a.abortf, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    {
       double v = -2.0;
       int i = 0x7fffffff, w = -1;
       for (int q = tid; q < G; q += 256) {
@@ -160,7 +175,6 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuPanelArgs a) {
           w = ow;
         }
       }
-      __syncthreads();
       if (lane == 0) {
         redv[wave] = v;
         redi[wave] = i;
@@ -183,17 +197,11 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuPanelArgs a) {
           bi2 = c;
           bw2 = -1;
         }
-        a.res[2 * c] = (double)bi2;
-        a.res[2 * c + 1] = (double)bw2;
+        pw[0] = bi2;
+        pw[1] = bw2;
       }
-      publish(&sw[1]);
+      __syncthreads();
     }
-    wait_flag(&sw[1], a.abortf);
-    if (tid == 0) {
-      pw[0] = (int)a.res[2 * c];
-      pw[1] = (int)a.res[2 * c + 1];
-    }
-    __syncthreads();
     const int p = pw[0], w = pw[1];
     // 3. pivot row u (the winner's row, or row c itself), old row c
     const double* urow = (w >= 0) ? part + (size_t)w * kLuSlot + 2 : rowc;
@@ -246,39 +254,50 @@ __global__ __launch_bounds__(256) void lu_laswp_kernel(int j, int jb, const int*
   __shared__ int rows[MAXT], src[MAXT];
   __shared__ int nt;
   __shared__ T buf[MAXT][65];
+  __shared__ int tp[kLuNB], pv[kLuNB], fq[kLuNB];
   const int tid = threadIdx.x;
-  if (tid == 0) {
-    int n = 0;
-    for (int i = 0; i < jb; ++i) {
-      rows[n] = j + i;
-      src[n] = j + i;
-      ++n;
-    }
-    for (int i = 0; i < jb; ++i) {
-      const int p = piv[i] + j;
-      if (p >= j + jb) {
-        bool seen = false;
-        for (int t = jb; t < n; ++t) seen = seen || (rows[t] == p);
-        if (!seen) {
-          rows[n] = p;
-          src[n] = p;
-          ++n;
+  // Touched rows: the panel rows j..j+jb-1 (slots 0..jb-1), then each pivot
+  // row below them at its first appearance (slots jb.., in that order).
+  if (tid < jb) pv[tid] = piv[tid] + j;
+  __syncthreads();
+  if (tid < jb) {  // first appearance of this pivot's row
+    const int p = pv[tid];
+    int f = tid;
+    if (p >= j + jb)
+      for (int q = 0; q < tid; ++q)
+        if (pv[q] == p) {
+          f = q;
+          break;
         }
-      }
+    fq[tid] = f;
+  }
+  __syncthreads();
+  if (tid < jb) {  // its slot
+    const int p = pv[tid];
+    int t;
+    if (p < j + jb) {
+      t = p - j;
+    } else {
+      int cnt = 0;
+      for (int q = 0; q < fq[tid]; ++q) cnt += (pv[q] >= j + jb && fq[q] == q) ? 1 : 0;
+      t = jb + cnt;
+      if (fq[tid] == tid) rows[t] = p;
     }
+    tp[tid] = t;
+    rows[tid] = j + tid;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int n = jb;
+    for (int i = 0; i < jb; ++i) n = max(n, tp[i] + 1);
+    for (int t = 0; t < n; ++t) src[t] = rows[t];
+    // the swaps in order, on the slot contents
     for (int i = 0; i < jb; ++i) {
-      const int p = piv[i] + j;
-      if (p == j + i) continue;
-      int tp = 0;
-      if (p < j + jb) {
-        tp = p - j;
-      } else {
-        for (int t = jb; t < n; ++t)
-          if (rows[t] == p) tp = t;
-      }
+      const int t2 = tp[i];
+      if (t2 == i) continue;
       const int x = src[i];
-      src[i] = src[tp];
-      src[tp] = x;
+      src[i] = src[t2];
+      src[t2] = x;
     }
     nt = n;
   }
@@ -300,7 +319,7 @@ __global__ __launch_bounds__(256) void lu_laswp_kernel(int j, int jb, const int*
 // U12 = L11^-1 A12 (L11 the jb x jb unit lower triangle at L, A12 jb x ncols
 // at B): a wave per column, lane = row, forward substitution with x_k
 // broadcast by v_readlane and L11 read from LDS (each lane its own row).
-// A workgroup takes 64 columns.
+// A workgroup takes 16 columns.
 template <typename T>
 __global__ __launch_bounds__(256) void lu_trsm_kernel(int jb, const T* __restrict__ L, long ldl, T* B, long ldb,
                                                       int ncols) {
@@ -312,8 +331,8 @@ __global__ __launch_bounds__(256) void lu_trsm_kernel(int jb, const T* __restric
     Ls[r][c] = (r < jb && c < jb && r > c) ? L[(size_t)r + (size_t)c * ldl] : S<T>::zero();
   }
   __syncthreads();
-  const int cb = blockIdx.x * 64;
-  for (int q = wave; q < 64; q += 4) {
+  const int cb = blockIdx.x * 16;
+  for (int q = wave; q < 16; q += 4) {
     const int col = cb + q;
     if (col >= ncols) break;
     T* b = B + (size_t)col * ldb;
@@ -377,7 +396,7 @@ template <typename T>
 hipError_t launch_lu_trsm(int jb, const T* L, long ldl, T* B, long ldb, int ncols, hipStream_t st) {
   if (jb <= 0 || jb > kLuNB || ncols < 0 || ldl < jb || ldb < jb) return hipErrorInvalidValue;
   if (ncols == 0) return hipSuccess;
-  hipLaunchKernelGGL((lu_trsm_kernel<T>), dim3((ncols + 63) / 64), dim3(256), 0, st, jb, L, ldl, B, ldb, ncols);
+  hipLaunchKernelGGL((lu_trsm_kernel<T>), dim3((ncols + 15) / 16), dim3(256), 0, st, jb, L, ldl, B, ldb, ncols);
   return hipGetLastError();
 }
 

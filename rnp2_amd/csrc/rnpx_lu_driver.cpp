@@ -43,8 +43,7 @@ int getrf_dev(Handle* h, int m, int n, T* A, long lda, int* piv_host, int* info)
   RNPX_TRY(h->lubuf.ensure(bytes, h));
   double* part = (double*)h->lubuf.p;
   double* rowc = part + (size_t)2 * 256 * kLuPartPerGroup;
-  double* res = rowc + 4 * nb;
-  unsigned* sync = (unsigned*)(res + 2 * nb);
+  unsigned* sync = (unsigned*)(rowc + 6 * nb);
   int* zl = (int*)(sync + 2 * nb);
   int* prel = zl + npan;
   for (int pi = 0; pi < npan; ++pi) {
@@ -59,7 +58,6 @@ int getrf_dev(Handle* h, int m, int n, T* A, long lda, int* piv_host, int* info)
     pa.zero_last = zl + pi;
     pa.part = part;
     pa.rowc = rowc;
-    pa.res = res;
     pa.sync = sync;
     pa.abortf = (unsigned*)h->flag.p;
     RNPX_TRY(launch_lu_panel<T>(pa, lu_panel_rows(mp, maxg, S<T>::cplx ? 1 : 0), h->stream));
