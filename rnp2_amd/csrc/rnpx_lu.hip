@@ -22,6 +22,8 @@
 // update A22 -= L21 U12 is a GEMM.
 #include "rnpx_kernels.hpp"
 
+#include <cstdlib>
+
 namespace rnpx {
 
 namespace {
@@ -358,6 +360,16 @@ __global__ __launch_bounds__(256) void lu_gather_kernel(int n, int nrhs, const i
 }
 
 int lu_panel_rows(int mp, int max_groups, int cplx) {
+  // The smallest row block that fits (measured at 8192^2: 64 rows 77.9 ms,
+  // 128 rows 78.1 ms, 256 rows 87.4 ms); RNPX_LU_PANEL_RW overrides.
+  static const int pref = [] {
+    const char* e = getenv("RNPX_LU_PANEL_RW");
+    return e ? atoi(e) : 0;
+  }();
+  if (pref == 64 || pref == 128 || (pref == 256 && !cplx)) {
+    const int G = (mp + pref - 1) / pref;
+    if (G <= max_groups) return pref;
+  }
   for (int rw : {64, 128, 256}) {
     if (cplx && rw > 128) break;
     const int G = (mp + rw - 1) / rw;
