@@ -207,6 +207,18 @@ int rnpx_dgetrs(const char *trans, size_t n, size_t nrhs, const double *a, size_
 /* LU::Solve<std::complex<double>> -- RNP/LA/LU.hpp:117-144 (zgetrs). */
 int rnpx_zgetrs(const char *trans, size_t n, size_t nrhs, const rnpx_zcomplex *a, size_t lda,
                 const size_t *ipiv, rnpx_zcomplex *b, size_t ldb);
+/* LU::RefineSolution<double> -- RNP/LA/LU.hpp:216-366 (dgerfs): refines x
+ * (n x nrhs) for op(A) x = b from the factors af/ipiv, and returns the
+ * componentwise backward error berr and the estimated forward error ferr
+ * per right-hand side.  work and rwork are accepted and not used. */
+int rnpx_dgerfs(const char *trans, size_t n, size_t nrhs, const double *a, size_t lda,
+                const double *af, size_t ldaf, const size_t *ipiv, const double *b, size_t ldb,
+                double *x, size_t ldx, double *ferr, double *berr, double *work, double *rwork);
+/* LU::RefineSolution<std::complex<double>> -- RNP/LA/LU.hpp:216-366 (zgerfs). */
+int rnpx_zgerfs(const char *trans, size_t n, size_t nrhs, const rnpx_zcomplex *a, size_t lda,
+                const rnpx_zcomplex *af, size_t ldaf, const size_t *ipiv, const rnpx_zcomplex *b,
+                size_t ldb, rnpx_zcomplex *x, size_t ldx, double *ferr, double *berr,
+                rnpx_zcomplex *work, double *rwork);
 
 /* ---------------------------------------------------------------------
  * Handles (device-resident work on one HIP stream).

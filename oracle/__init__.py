@@ -88,6 +88,8 @@ def _declare(L):
             f.restype = ctypes.c_int
             f.argtypes = [_sz, _sz, _p, _sz, _p]
         getattr(L, f"orc_{pre}_lu_solve").argtypes = [ctypes.c_char_p, _sz, _sz, _p, _sz, _p, _p, _sz]
+        getattr(L, f"orc_{pre}_lu_refine").argtypes = [ctypes.c_char_p, _sz, _sz, _p, _sz, _p, _sz, _p, _p, _sz,
+                                                       _p, _sz, _p, _p]
         for fam in ("ql", "rq"):
             getattr(L, f"orc_{pre}_{fam}_factor").argtypes = [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]
             getattr(L, f"orc_{pre}_{fam}_multq").argtypes = [ctypes.c_char_p, ctypes.c_char_p, _sz, _sz, _sz,
@@ -486,6 +488,22 @@ def rq_generateq(a, tau, k=None):
     """RQ::GenerateQ (RQ.hpp:519-580): a (m x n, m <= n, reflectors in its
     last k rows) becomes the last m rows of Q."""
     return _fam_generateq("rq", a, tau, k)
+
+
+def lu_refine(trans, a, lu, piv, b, x):
+    """LU::RefineSolution (LU.hpp:216-366): returns (refined x, ferr, berr)."""
+    x = _fortran(x, np.result_type(x, a, np.float64))
+    a = _fortran(a, x.dtype)
+    lu = _fortran(lu, x.dtype)
+    b = _fortran(b, x.dtype)
+    piv = np.ascontiguousarray(piv, dtype=np.uint64)
+    n, nrhs = x.shape
+    ferr = np.zeros(max(nrhs, 1))
+    berr = np.zeros(max(nrhs, 1))
+    getattr(lib(), f"orc_{_pre(x.dtype)}_lu_refine")(trans.encode(), n, nrhs, _ptr(a), a.shape[0], _ptr(lu),
+                                                     lu.shape[0], piv.ctypes.data_as(ctypes.c_void_p), _ptr(b),
+                                                     b.shape[0], _ptr(x), max(n, 1), _ptr(ferr), _ptr(berr))
+    return x, ferr[:nrhs], berr[:nrhs]
 
 
 def hess_multq(side, trans, afac, tau, c, ilo, ihi):

@@ -164,6 +164,15 @@ void orc_z_rq_multq(const char *side, const char *trans, size_t m, size_t n, siz
 void orc_z_rq_generateq(size_t m, size_t n, size_t k, orc_z *a, size_t lda, const orc_z *tau,
                         size_t *lwork, orc_z *work);
 
+/* LU::RefineSolution LU.hpp:216-366 (MatrixNorm1Estimate_unblocked,
+ * MatrixNormEstimates.hpp:17-107, for the forward error). */
+void orc_d_lu_refine(const char *trans, size_t n, size_t nrhs, const double *a, size_t lda, const double *af,
+                     size_t ldaf, const size_t *ipiv, const double *b, size_t ldb, double *x, size_t ldx,
+                     double *ferr, double *berr);
+void orc_z_lu_refine(const char *trans, size_t n, size_t nrhs, const orc_z *a, size_t lda, const orc_z *af,
+                     size_t ldaf, const size_t *ipiv, const orc_z *b, size_t ldb, orc_z *x, size_t ldx,
+                     double *ferr, double *berr);
+
 /* ---- Hessenberg (RNP/LA/Hessenberg.hpp) ----
  *   Hessenberg::MultQ  Hessenberg.hpp:478-514, Hessenberg::GenerateQ Hessenberg.hpp:540-588 */
 void orc_d_hess_multq(const char *side, const char *trans, size_t m, size_t n, size_t ilo, size_t ihi,

@@ -1243,6 +1243,168 @@ void FN(lu_solve)(const char *trans, size_t n, size_t nrhs, const T *a, size_t l
 	}
 }
 
+/* MatrixNorm1Estimate_unblocked, MatrixNormEstimates.hpp:17-107: Hager's
+ * 1-norm estimate with Higham's refinements (zlacn2), for an operator
+ * given as aop(trans, n, x, data) that overwrites x with op(B) x. */
+static double FN(norm1_estimate)(size_t n, void (*aop)(char, size_t, T *, void *), void *data, T *work){
+	const double safmin = 2.2250738585072014e-308;
+	T *v = work, *x = work + n;
+	for(size_t i = 0; i < n; ++i){ x[i] = 1.0 / (double)n; }
+	aop('N', n, x, data);
+	if(1 == n){ return TABS(x[0]); }
+	double est = 0.0;
+	for(size_t i = 0; i < n; ++i){ est += TABS(x[i]); }
+	for(size_t i = 0; i < n; ++i){
+		const double ax = TABS(x[i]);
+		if(ax > safmin){ x[i] /= ax; }else{ x[i] = 1.0; }
+	}
+	aop('C', n, x, data);
+	size_t imax = 0;
+	{
+		double armax = 0.0;
+		for(size_t i = 0; i < n; ++i){
+			const double ar = fabs(RE(x[i]));
+			if(ar > armax){ armax = ar; imax = i; }
+		}
+	}
+	size_t iter = 1, jlast = 0;
+	do{
+		++iter;
+		for(size_t i = 0; i < n; ++i){ x[i] = 0.0; }
+		x[imax] = 1.0;
+		aop('N', n, x, data);
+		for(size_t i = 0; i < n; ++i){ v[i] = x[i]; }
+		const double estold = est;
+		est = 0.0;
+		for(size_t i = 0; i < n; ++i){ est += TABS(v[i]); }
+		if(est <= estold){ break; }
+		for(size_t i = 0; i < n; ++i){
+			const double ax = TABS(x[i]);
+			if(ax > safmin){ x[i] /= ax; }else{ x[i] = 1.0; }
+		}
+		aop('C', n, x, data);
+		jlast = imax;
+		imax = 0;
+		double armax = 0.0;
+		for(size_t i = 0; i < n; ++i){
+			const double ar = fabs(RE(x[i]));
+			if(ar > armax){ armax = ar; imax = i; }
+		}
+	}while(TABS(x[jlast]) != TABS(x[imax]) && iter < 5);
+	double altsgn = 1.0;
+	for(size_t i = 0; i < n; ++i){
+		x[i] = altsgn * (1.0 + (double)i / (double)(n-1));
+		altsgn = -altsgn;
+	}
+	aop('N', n, x, data);
+	double t = 0.0;
+	for(size_t i = 0; i < n; ++i){ t += TABS(x[i]); }
+	t = 2.0 * t / (double)(3*n);
+	if(t >= est){ est = t; }
+	return est;
+}
+
+/* Util::Norm1Estimator, LU.hpp:34-50 (with the LU solve it intends: the
+ * reference calls an undeclared PLUSolve and reads data->ldav): "N" solves
+ * with op_t then scales by w; "C" scales then solves with op_n. */
+typedef struct { const T *af; size_t ldaf; const size_t *ipiv; char tn, tt; const double *w; } FN(refine_est);
+static void FN(refine_aop)(char trans, size_t n, T *x, void *data_){
+	const FN(refine_est) *d = (const FN(refine_est) *)data_;
+	if('N' == trans){
+		const char t[2] = {d->tt, 0};
+		FN(lu_solve)(t, n, 1, d->af, d->ldaf, d->ipiv, x, n);
+		for(size_t i = 0; i < n; ++i){ x[i] *= d->w[i]; }
+	}else{
+		for(size_t i = 0; i < n; ++i){ x[i] *= d->w[i]; }
+		const char t[2] = {d->tn, 0};
+		FN(lu_solve)(t, n, 1, d->af, d->ldaf, d->ipiv, x, n);
+	}
+}
+
+/* LU::RefineSolution, LU.hpp:216-366: per right-hand side, residual
+ * r = b - op(A) x, componentwise backward error
+ * berr = max_i |r_i| / (|op(A)| |x| + |b|)_i (safe1 added where the
+ * denominator is below safe2), a correction by LU::Solve while berr > eps,
+ * berr at least halves and fewer than refine_max_iters = 5 tries; then
+ * ferr = ||inv(op(A)) diag(w)||_1 estimate / max |x_i| with
+ * w = |r| + nz eps (|op(A)||x| + |b|) (+ safe1 where the latter is below
+ * safe2, as the comment at LU.hpp:322-323 and LAPACK's gerfs state; the
+ * line at LU.hpp:329 multiplies by safe1 instead). */
+void FN(lu_refine)(const char *trans, size_t n, size_t nrhs, const T *a, size_t lda, const T *af, size_t ldaf,
+                   const size_t *ipiv, const T *b, size_t ldb, T *x, size_t ldx, double *ferr, double *berr){
+	if(0 == n || 0 == nrhs){
+		for(size_t j = 0; j < nrhs; ++j){ if(ferr){ ferr[j] = 0; } if(berr){ berr[j] = 0; } }
+		return;
+	}
+	const double eps = 2.220446049250313e-16;  /* Traits<double>::eps(): numeric_limits epsilon, 2^-52 */
+	const double dmin = 2.2250738585072014e-308;
+	const size_t nz = n + 1;
+	const double safe1 = (double)nz * dmin, safe2 = safe1 / eps;
+	const char tr = trans[0];
+	T *work = (T *)malloc(sizeof(T) * 2 * n);
+	T *est_work = (T *)malloc(sizeof(T) * 2 * n);
+	double *rw = (double *)malloc(sizeof(double) * n);
+	for(size_t j = 0; j < nrhs; ++j){
+		size_t count = 1;
+		double lstres = 3.0, s = 0.0;
+		for(;;){
+			for(size_t i = 0; i < n; ++i){ work[i] = b[i+j*ldb]; }
+			for(size_t k = 0; k < n; ++k){
+				for(size_t i = 0; i < n; ++i){
+					if('N' == tr){ work[i] -= a[i+k*lda] * x[k+j*ldx]; }
+					else{
+						const T aik = ('C' == tr) ? CONJ(a[k+i*lda]) : a[k+i*lda];
+						work[i] -= aik * x[k+j*ldx];
+					}
+				}
+			}
+			for(size_t i = 0; i < n; ++i){ rw[i] = FN(norm1s)(b[i+j*ldb]); }
+			if('N' == tr){
+				for(size_t k = 0; k < n; ++k){
+					const double xk = FN(norm1s)(x[k+j*ldx]);
+					for(size_t i = 0; i < n; ++i){ rw[i] += FN(norm1s)(a[i+k*lda]) * xk; }
+				}
+			}else{
+				for(size_t k = 0; k < n; ++k){
+					double t = 0.0;
+					for(size_t i = 0; i < n; ++i){ t += FN(norm1s)(a[i+k*lda]) * FN(norm1s)(x[i+j*ldx]); }
+					rw[k] += t;
+				}
+			}
+			s = 0.0;
+			for(size_t i = 0; i < n; ++i){
+				const double r = (rw[i] > safe2) ? FN(norm1s)(work[i]) / rw[i]
+				                                 : (FN(norm1s)(work[i]) + safe1) / (rw[i] + safe1);
+				if(r > s){ s = r; }
+			}
+			if(berr){ berr[j] = s; }
+			if(s > eps && 2.0 * s <= lstres && count < 5){
+				FN(lu_solve)(trans, n, 1, af, ldaf, ipiv, work, n);
+				for(size_t i = 0; i < n; ++i){ x[i+j*ldx] += work[i]; }
+				lstres = s;
+				++count;
+				continue;
+			}
+			break;
+		}
+		for(size_t i = 0; i < n; ++i){
+			if(rw[i] > safe2){ rw[i] = FN(norm1s)(work[i]) + (double)nz * eps * rw[i]; }
+			else{ rw[i] = FN(norm1s)(work[i]) + (double)nz * eps * rw[i] + safe1; }
+		}
+		FN(refine_est) ed = {af, ldaf, ipiv, ('N' == tr) ? 'N' : 'C', ('N' == tr) ? 'C' : 'N', rw};
+		const double fe = FN(norm1_estimate)(n, FN(refine_aop), &ed, est_work);
+		double xm = 0.0;
+		for(size_t i = 0; i < n; ++i){
+			const double v = FN(norm1s)(x[i+j*ldx]);
+			if(v > xm){ xm = v; }
+		}
+		if(ferr){ ferr[j] = (0.0 != xm) ? fe / xm : fe; }
+	}
+	free(work);
+	free(est_work);
+	free(rw);
+}
+
 /* Least squares by QR: Factor (QR.hpp:129), MultQ("L","C") on B
  * (QR.hpp:337), then R x = (Q^H B)(0:n,:) by back substitution
  * (SolveTrM "L","U","N","N", BLAS_mix.hpp:2096). */

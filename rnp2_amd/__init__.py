@@ -104,6 +104,8 @@ SIGNATURES = {
     "rnpx_zgeqrf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_dgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_zgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
+    "rnpx_dgerfs": (_i, [_s, _sz, _sz, _p, _sz, _p, _sz, _p, _p, _sz, _p, _sz, _p, _p, _p, _p]),
+    "rnpx_zgerfs": (_i, [_s, _sz, _sz, _p, _sz, _p, _sz, _p, _p, _sz, _p, _sz, _p, _p, _p, _p]),
     "rnpx_dgetrf": (_i, [_sz, _sz, _p, _sz, _p]),
     "rnpx_zgetrf": (_i, [_sz, _sz, _p, _sz, _p]),
     "rnpx_dgetrs": (_i, [_s, _sz, _sz, _p, _sz, _p, _p, _sz]),
@@ -459,6 +461,24 @@ def lu_solve(trans, lu, piv, b):
     _check(fn(trans.encode(), n, nrhs, _ptr(lu), lu.shape[0], piv.ctypes.data_as(ctypes.c_void_p), _ptr(b),
               max(n, 1)), "LU::Solve")
     return b
+
+
+def lu_refine(trans, a, lu, piv, b, x):
+    """LU::RefineSolution (LU.hpp:216-366) on the GPU: returns (refined x,
+    ferr, berr)."""
+    x = np.array(x, dtype=np.result_type(x, a, np.float64), order="F", copy=True)
+    a = np.asfortranarray(a, dtype=x.dtype)
+    lu = np.asfortranarray(lu, dtype=x.dtype)
+    b = np.asfortranarray(b, dtype=x.dtype)
+    piv = np.ascontiguousarray(piv, dtype=np.uint64)
+    n, nrhs = x.shape
+    ferr = np.zeros(max(nrhs, 1))
+    berr = np.zeros(max(nrhs, 1))
+    fn = getattr(lib(), f"rnpx_{_pre(x.dtype)}gerfs")
+    _check(fn(trans.encode(), n, nrhs, _ptr(a), a.shape[0], _ptr(lu), lu.shape[0],
+              piv.ctypes.data_as(ctypes.c_void_p), _ptr(b), b.shape[0], _ptr(x), max(n, 1), _ptr(ferr), _ptr(berr),
+              None, None), "LU::RefineSolution")
+    return x, ferr[:nrhs], berr[:nrhs]
 
 
 def hess_multq(side, trans, afac, tau, c, ilo, ihi):

@@ -442,6 +442,43 @@ int dropin_getrs(const char* trans, size_t n, size_t nrhs, const T* a, size_t ld
   return sync_handle(h);
 }
 
+template <typename T>
+int dropin_gerfs(const char* trans, size_t n, size_t nrhs, const T* a, size_t lda, const T* af, size_t ldaf,
+                 const size_t* ipiv, const T* b, size_t ldb, T* x, size_t ldx, double* ferr, double* berr) {
+  if (!trans || (trans[0] != 'N' && trans[0] != 'T' && trans[0] != 'C')) return -1;
+  if (n == 0 || nrhs == 0) {
+    for (size_t j = 0; j < nrhs; ++j) {
+      if (ferr) ferr[j] = 0.0;
+      if (berr) berr[j] = 0.0;
+    }
+    return RNPX_OK;
+  }
+  if (lda < n) return -5;
+  if (ldaf < n) return -7;
+  if (!ipiv) return -8;
+  if (ldb < n) return -10;
+  if (ldx < n) return -12;
+  if (!fits_int(n) || !fits_int(nrhs)) return RNPX_ERR_UNSUPPORTED;
+  std::vector<int> piv(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (ipiv[i] >= n) return -8;
+    piv[i] = (int)ipiv[i];
+  }
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A, AF, B, X;
+  if ((st = A.in(h, h->stage1, const_cast<T*>(a), n, n, lda, true))) return st;
+  if ((st = AF.in(h, h->stage2, const_cast<T*>(af), n, n, ldaf, true))) return st;
+  if ((st = B.in(h, h->stage3, const_cast<T*>(b), n, nrhs, ldb, true))) return st;
+  if ((st = X.in(h, h->lsbuf, x, n, nrhs, ldx, true))) return st;
+  st = gerfs_dev<T>(h, trans[0], (int)n, (int)nrhs, A.dev, A.ld, AF.dev, AF.ld, piv.data(), B.dev, B.ld, X.dev, X.ld,
+                    ferr, berr);
+  if (st) return st;
+  if ((st = X.out(h, ldx))) return st;
+  return sync_handle(h);
+}
+
 Handle* H(rnpx_handle h) { return reinterpret_cast<Handle*>(h); }
 
 }  // namespace
@@ -598,6 +635,21 @@ int rnpx_dgetrs(const char* trans, size_t n, size_t nrhs, const double* a, size_
 int rnpx_zgetrs(const char* trans, size_t n, size_t nrhs, const rnpx_zcomplex* a, size_t lda, const size_t* ipiv,
                 rnpx_zcomplex* b, size_t ldb) {
   return dropin_getrs<zd>(trans, n, nrhs, reinterpret_cast<const zd*>(a), lda, ipiv, reinterpret_cast<zd*>(b), ldb);
+}
+int rnpx_dgerfs(const char* trans, size_t n, size_t nrhs, const double* a, size_t lda, const double* af, size_t ldaf,
+                const size_t* ipiv, const double* b, size_t ldb, double* x, size_t ldx, double* ferr, double* berr,
+                double* work, double* rwork) {
+  (void)work;
+  (void)rwork;
+  return dropin_gerfs<double>(trans, n, nrhs, a, lda, af, ldaf, ipiv, b, ldb, x, ldx, ferr, berr);
+}
+int rnpx_zgerfs(const char* trans, size_t n, size_t nrhs, const rnpx_zcomplex* a, size_t lda, const rnpx_zcomplex* af,
+                size_t ldaf, const size_t* ipiv, const rnpx_zcomplex* b, size_t ldb, rnpx_zcomplex* x, size_t ldx,
+                double* ferr, double* berr, rnpx_zcomplex* work, double* rwork) {
+  (void)work;
+  (void)rwork;
+  return dropin_gerfs<zd>(trans, n, nrhs, reinterpret_cast<const zd*>(a), lda, reinterpret_cast<const zd*>(af), ldaf,
+                          ipiv, reinterpret_cast<const zd*>(b), ldb, reinterpret_cast<zd*>(x), ldx, ferr, berr);
 }
 int rnpx_dgelqf(size_t m, size_t n, double* a, size_t lda, double* tau, size_t* lwork, double* work) {
   if (lwork && (work == nullptr || *lwork == 0)) {

@@ -57,12 +57,13 @@ struct Handle {
   Buffer lqbuf;                            // A^H for the LQ routines
   Buffer qlbuf;                            // reversed reflectors and tau (QL, RQ)
   Buffer lubuf;                            // LU panel scratch, Solve's permutation and gather
+  Buffer rfbuf;                            // RefineSolution's residual and |op(A)||x| + |b|
   unsigned trsm_epoch = 0;                 // value the flags take in the current solve
   // Every buffer above, for accounting and release.
   std::vector<Buffer*> buffers() {
     return {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc,
             &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
-            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf};
+            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf};
   }
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
@@ -141,6 +142,11 @@ template <typename T> int orghr_dev(Handle* h, int n, int ilo, int ihi, T* A, lo
 template <typename T> int getrf_dev(Handle* h, int m, int n, T* A, long lda, int* piv_host, int* info);
 template <typename T>
 int getrs_dev(Handle* h, char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb);
+// LU::RefineSolution (rnpx_lu_driver.cpp); A, AF, B, X on the device, ipiv,
+// ferr and berr on the host.
+template <typename T>
+int gerfs_dev(Handle* h, char trans, int n, int nrhs, const T* A, long lda, const T* AF, long ldaf, const int* ipiv,
+              const T* B, long ldb, T* X, long ldx, double* ferr, double* berr);
 // op(M) X = B on the handle's stream (rnpx_ls.cpp), M the uplo triangle of R.
 template <typename T>
 int trsm_tri_dev(Handle* h, char uplo, char op, char diag, int n, int nrhs, const T* R, long ldr, T* B, long ldb);

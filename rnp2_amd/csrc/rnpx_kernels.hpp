@@ -113,6 +113,12 @@ template <typename T>
 hipError_t launch_lu_trsm(int jb, const T* L, long ldl, T* B, long ldb, int ncols, hipStream_t st);
 template <typename T>
 hipError_t launch_lu_gather(int n, int nrhs, const int* src, const T* X, long ldx, T* Y, long ldy, hipStream_t st);
+// RefineSolution's |op(A)| |x| + |b| (rw, doubles) and y += x.
+template <typename T>
+hipError_t launch_lu_absmv(int n, bool transposed, const T* A, long lda, const T* x, const T* b, double* rw,
+                           hipStream_t st);
+template <typename T>
+hipError_t launch_lu_axpy(int n, const T* x, T* y, hipStream_t st);
 
 template <typename T>
 struct PanelArgs {

@@ -359,6 +359,56 @@ __global__ __launch_bounds__(256) void lu_gather_kernel(int n, int nrhs, const i
   Y[(size_t)i + (size_t)c * ldy] = X[(size_t)src[i] + (size_t)c * ldx];
 }
 
+// RefineSolution's denominators: rw_i = |b_i| + sum_k |A(i,k)| |x_k| for
+// op "N" (a thread per row, coalesced down the columns) and
+// rw_k = |b_k| + sum_i |A(i,k)| |x_i| for "T"/"C" (a wave per column);
+// |.| is |re| + |im| (Traits<T>::norm1).
+template <typename T>
+__global__ __launch_bounds__(256) void lu_absmv_kernel(int n, int transposed, const T* __restrict__ A, long lda,
+                                                       const T* __restrict__ x, const T* __restrict__ b,
+                                                       double* __restrict__ rw) {
+  if (!transposed) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += norm1_of(A[(size_t)i + (size_t)k * lda]) * norm1_of(x[k]);
+    rw[i] = norm1_of(b[i]) + s;
+  } else {
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (k >= n) return;
+    double s = 0.0;
+    for (int i = lane; i < n; i += 64) s += norm1_of(A[(size_t)i + (size_t)k * lda]) * norm1_of(x[i]);
+    s = wave_sum(s);
+    if (lane == 0) rw[k] = norm1_of(b[k]) + s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void lu_axpy_kernel(int n, const T* __restrict__ x, T* __restrict__ y) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = y[i] + x[i];
+}
+
+template <typename T>
+hipError_t launch_lu_absmv(int n, bool transposed, const T* A, long lda, const T* x, const T* b, double* rw,
+                           hipStream_t st) {
+  if (n < 0 || lda < std::max(1, n)) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  if (!transposed)
+    hipLaunchKernelGGL((lu_absmv_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, st, n, 0, A, lda, x, b, rw);
+  else
+    hipLaunchKernelGGL((lu_absmv_kernel<T>), dim3((n + 3) / 4), dim3(256), 0, st, n, 1, A, lda, x, b, rw);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_lu_axpy(int n, const T* x, T* y, hipStream_t st) {
+  if (n < 0) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL((lu_axpy_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, st, n, x, y);
+  return hipGetLastError();
+}
+
 int lu_panel_rows(int mp, int max_groups, int cplx) {
   // The smallest row block that fits (measured at 8192^2: 64 rows 77.9 ms,
   // 128 rows 78.1 ms, 256 rows 87.4 ms); RNPX_LU_PANEL_RW overrides.
@@ -426,7 +476,9 @@ hipError_t launch_lu_gather(int n, int nrhs, const int* src, const T* X, long ld
   template hipError_t launch_lu_panel<T>(const LuPanelArgs&, int, hipStream_t);                         \
   template hipError_t launch_lu_laswp<T>(int, int, const int*, T*, long, int, int, hipStream_t);        \
   template hipError_t launch_lu_trsm<T>(int, const T*, long, T*, long, int, hipStream_t);               \
-  template hipError_t launch_lu_gather<T>(int, int, const int*, const T*, long, T*, long, hipStream_t);
+  template hipError_t launch_lu_gather<T>(int, int, const int*, const T*, long, T*, long, hipStream_t);      \
+  template hipError_t launch_lu_absmv<T>(int, bool, const T*, long, const T*, const T*, double*, hipStream_t); \
+  template hipError_t launch_lu_axpy<T>(int, const T*, T*, hipStream_t);
 RNPX_INST(double)
 RNPX_INST(zd)
 #undef RNPX_INST

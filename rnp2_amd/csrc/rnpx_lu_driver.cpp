@@ -12,6 +12,8 @@
 #include "rnpx_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <numeric>
 #include <vector>
 
@@ -150,6 +152,174 @@ int getrs_dev(Handle* h, char trans, int n, int nrhs, const T* A, long lda, cons
   RNPX_TRY(hipStreamSynchronize(h->stream));
   return RNPX_OK;
 }
+
+namespace {
+
+template <typename T> struct HostT;
+template <> struct HostT<double> {
+  using type = double;
+  static double norm1(double v) { return std::fabs(v); }
+  static double abs(double v) { return std::fabs(v); }
+  static double re(double v) { return v; }
+};
+template <> struct HostT<zd> {
+  using type = zd;
+  static double norm1(zd v) { return std::fabs(v.x) + std::fabs(v.y); }
+  static double abs(zd v) { return std::hypot(v.x, v.y); }
+  static double re(zd v) { return v.x; }
+};
+inline double scal(double v, double s) { return v * s; }
+inline zd scal(zd v, double s) { return mkz(v.x * s, v.y * s); }
+inline double setv(double, double r) { return r; }
+inline zd setv(zd, double r) { return mkz(r, 0.0); }
+
+}  // namespace
+
+// LU::RefineSolution (LU.hpp:216-366) with its documented intent where the
+// reference does not compile (the LU solve it calls PLUSolve; the estimator
+// it calls MatrixNorm1Estimate with MatrixNorm1Estimate_unblocked's
+// argument list, MatrixNormEstimates.hpp:17-107; the safe1 term added, as
+// the comment at LU.hpp:322-323 says).  The residuals, |op(A)||x| + |b| and
+// the corrections run on the device; the per-column scalars (berr, the
+// stopping test, the estimator's vectors of length n) on the host.
+template <typename T>
+int gerfs_dev(Handle* h, char trans, int n, int nrhs, const T* A, long lda, const T* AF, long ldaf, const int* ipiv,
+              const T* B, long ldb, T* X, long ldx, double* ferr, double* berr) {
+  using H = HostT<T>;
+  if (trans != 'N' && trans != 'T' && trans != 'C') return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (n == 0 || nrhs == 0) {
+    for (int j = 0; j < nrhs; ++j) {
+      if (ferr) ferr[j] = 0.0;
+      if (berr) berr[j] = 0.0;
+    }
+    return RNPX_OK;
+  }
+  const double eps = std::numeric_limits<double>::epsilon(), dmin = std::numeric_limits<double>::min();
+  const double nz = (double)(n + 1), safe1 = nz * dmin, safe2 = safe1 / eps;
+  RNPX_TRY(h->rfbuf.ensure(sizeof(T) * (size_t)n + sizeof(double) * (size_t)n + 256, h));
+  T* r = (T*)h->rfbuf.p;
+  double* rwd = (double*)((char*)h->rfbuf.p + ((sizeof(T) * (size_t)n + 255) / 256) * 256);
+  std::vector<T> rh(n), xh(n);
+  std::vector<double> rw(n);
+  const T one = S<T>::one(), mone = S<T>::zero() - S<T>::one();
+  const char tn = (trans == 'N') ? 'N' : 'C', tt = (trans == 'N') ? 'C' : 'N';
+  for (int j = 0; j < nrhs; ++j) {
+    T* xj = X + (size_t)j * ldx;
+    const T* bj = B + (size_t)j * ldb;
+    int count = 1;
+    double lstres = 3.0, s = 0.0;
+    for (;;) {
+      RNPX_TRY(launch_copy<T>(n, 1, bj, std::max(1L, ldb), r, n, h->stream));
+      RNPX_TRY(gemm<T>(trans, 'N', n, 1, n, mone, A, lda, xj, std::max(1L, ldx), one, r, n, 1, nullptr, h->stream));
+      RNPX_TRY(launch_lu_absmv<T>(n, trans != 'N', A, lda, xj, bj, rwd, h->stream));
+      RNPX_TRY(hipMemcpyAsync(rh.data(), r, sizeof(T) * n, hipMemcpyDeviceToHost, h->stream));
+      RNPX_TRY(hipMemcpyAsync(rw.data(), rwd, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+      RNPX_TRY(hipStreamSynchronize(h->stream));
+      s = 0.0;
+      for (int i = 0; i < n; ++i) {
+        const double q = (rw[i] > safe2) ? H::norm1(rh[i]) / rw[i] : (H::norm1(rh[i]) + safe1) / (rw[i] + safe1);
+        if (q > s) s = q;
+      }
+      if (berr) berr[j] = s;
+      if (s > eps && 2.0 * s <= lstres && count < 5) {
+        int st = getrs_dev<T>(h, trans, n, 1, AF, ldaf, ipiv, r, n);
+        if (st) return st;
+        RNPX_TRY(launch_lu_axpy<T>(n, r, xj, h->stream));
+        lstres = s;
+        ++count;
+        continue;
+      }
+      break;
+    }
+    // w = |r| + nz eps (|op(A)||x| + |b|), + safe1 where the latter <= safe2
+    for (int i = 0; i < n; ++i)
+      rw[i] = (rw[i] > safe2) ? H::norm1(rh[i]) + nz * eps * rw[i] : H::norm1(rh[i]) + nz * eps * rw[i] + safe1;
+    // the estimator's operator on host vectors: "N" solves with op_t then
+    // scales by w; "C" scales then solves with op_n (LU.hpp:34-50)
+    int est_status = 0;
+    auto aop = [&](char op, std::vector<T>& v) {
+      if (est_status) return;
+      if (op == 'C')
+        for (int i = 0; i < n; ++i) v[i] = scal(v[i], rw[i]);
+      hipError_t e = hipMemcpyAsync(r, v.data(), sizeof(T) * n, hipMemcpyHostToDevice, h->stream);
+      if (e != hipSuccess) { est_status = hip_status(e); return; }
+      est_status = getrs_dev<T>(h, op == 'N' ? tt : tn, n, 1, AF, ldaf, ipiv, r, n);
+      if (est_status) return;
+      e = hipMemcpyAsync(v.data(), r, sizeof(T) * n, hipMemcpyDeviceToHost, h->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) { est_status = hip_status(e); return; }
+      if (op == 'N')
+        for (int i = 0; i < n; ++i) v[i] = scal(v[i], rw[i]);
+    };
+    // MatrixNorm1Estimate_unblocked (MatrixNormEstimates.hpp:17-107)
+    std::vector<T> v(n), x(n);
+    double est = 0.0;
+    for (int i = 0; i < n; ++i) x[i] = setv(x[i], 1.0 / (double)n);
+    aop('N', x);
+    if (n == 1) {
+      est = H::abs(x[0]);
+    } else {
+      for (int i = 0; i < n; ++i) est += H::abs(x[i]);
+      auto sgn = [&]() {
+        for (int i = 0; i < n; ++i) {
+          const double ax = H::abs(x[i]);
+          x[i] = (ax > dmin) ? scal(x[i], 1.0 / ax) : setv(x[i], 1.0);
+        }
+      };
+      auto argmax_re = [&]() {
+        int im = 0;
+        double am = 0.0;
+        for (int i = 0; i < n; ++i) {
+          const double ar = std::fabs(H::re(x[i]));
+          if (ar > am) { am = ar; im = i; }
+        }
+        return im;
+      };
+      sgn();
+      aop('C', x);
+      int imax = argmax_re(), jlast = 0, iter = 1;
+      do {
+        ++iter;
+        for (int i = 0; i < n; ++i) x[i] = setv(x[i], 0.0);
+        x[imax] = setv(x[imax], 1.0);
+        aop('N', x);
+        v = x;
+        const double estold = est;
+        est = 0.0;
+        for (int i = 0; i < n; ++i) est += H::abs(v[i]);
+        if (est <= estold) break;
+        sgn();
+        aop('C', x);
+        jlast = imax;
+        imax = argmax_re();
+      } while (H::abs(x[jlast]) != H::abs(x[imax]) && iter < 5);
+      double alt = 1.0;
+      for (int i = 0; i < n; ++i) {
+        x[i] = setv(x[i], alt * (1.0 + (double)i / (double)(n - 1)));
+        alt = -alt;
+      }
+      aop('N', x);
+      double t = 0.0;
+      for (int i = 0; i < n; ++i) t += H::abs(x[i]);
+      t = 2.0 * t / (double)(3 * n);
+      if (t >= est) est = t;
+    }
+    if (est_status) return est_status;
+    RNPX_TRY(hipMemcpyAsync(xh.data(), xj, sizeof(T) * n, hipMemcpyDeviceToHost, h->stream));
+    RNPX_TRY(hipStreamSynchronize(h->stream));
+    double xm = 0.0;
+    for (int i = 0; i < n; ++i) xm = std::max(xm, H::norm1(xh[i]));
+    if (ferr) ferr[j] = (xm != 0.0) ? est / xm : est;
+  }
+  return RNPX_OK;
+}
+
+template int gerfs_dev<double>(Handle*, char, int, int, const double*, long, const double*, long, const int*,
+                               const double*, long, double*, long, double*, double*);
+template int gerfs_dev<zd>(Handle*, char, int, int, const zd*, long, const zd*, long, const int*, const zd*, long,
+                           zd*, long, double*, double*);
 
 template int getrf_dev<double>(Handle*, int, int, double*, long, int*, int*);
 template int getrf_dev<zd>(Handle*, int, int, zd*, long, int*, int*);

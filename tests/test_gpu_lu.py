@@ -78,3 +78,30 @@ def test_lu_solve_matches_oracle(trans, cplx):
         err = float(np.max(np.abs(got - want)) / max(np.abs(want).max(), 1e-300))
         print(f"        n={n}: {err:.2e}", flush=True)
         assert err <= 1e-11
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("trans", ["N", "T", "C"])
+def test_lu_refine_matches_oracle(trans, cplx):
+    """RefineSolution from the same starting x: the refined x within 1e-12
+    of the oracle's, berr at working precision in both, ferr within 5% (its
+    weights |r| + (n+1) eps (|op(A)||x| + |b|) carry the final residual r,
+    which is rounding noise of relative size up to ~1/(n+1))."""
+    print(f"[case] lu refine {trans} cplx={cplx}", flush=True)
+    rng = np.random.default_rng(21)
+    for n in (30, 500):
+        a = seeded(n, n, cplx, n + 1)
+        b = rng.standard_normal((n, 2))
+        if cplx:
+            b = b + 1j * rng.standard_normal((n, 2))
+        lo, po, _ = O.lu_factor(a)
+        x0 = O.lu_solve(trans, lo, po, b)
+        # start from a perturbed solution so that refinement has work to do
+        x0 = x0 * (1.0 + 1e-9 * rng.standard_normal(x0.shape))
+        xo, fo, bo = O.lu_refine(trans, a, lo, po, b, x0.copy())
+        xg, fg, bg = R.lu_refine(trans, a, lo, po, b, x0.copy())
+        ex = float(np.max(np.abs(xg - xo)) / np.abs(xo).max())
+        ef = float(np.max(np.abs(fg - fo) / fo))
+        print(f"        n={n}: x {ex:.2e} ferr {ef:.2e} berr {bg} (oracle {bo})", flush=True)
+        assert ex <= 1e-12 and ef <= 0.05
+        assert np.all(bg <= 4 * np.finfo(float).eps) and np.all(bo <= 4 * np.finfo(float).eps)

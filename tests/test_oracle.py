@@ -945,3 +945,36 @@ def test_lu_return_value_follows_the_reference():
     a[:, 70] = 0.0      # an exact zero pivot at j = 70, in the panel starting at 64
     _, _, info = O.lu_factor(a)
     assert info == 72 and lapack.dgetrf(a.copy())[2] == 71
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("trans", ["N", "T", "C"])
+def test_lu_refine_pins(trans, cplx):
+    """RefineSolution: berr equals its closed form on the returned x, the
+    refined x is at working accuracy, ferr bounds the true forward error,
+    and ferr is twice LAPACK gesvx's (the same Hager/Higham estimate of the
+    same weighted inverse, with RNP's eps = 2^-52 in the weights against
+    LAPACK's 2^-53)."""
+    if cplx and trans == "T":
+        pytest.skip("LAPACK gesvx trans='T' on complex is covered by 'C'")
+    rng = np.random.default_rng(13)
+    for n in (8, 120):
+        a = _rand(rng, n, n, cplx)
+        b = _rand(rng, n, 3, cplx)
+        lu, piv, _ = O.lu_factor(a)
+        x0 = O.lu_solve(trans, lu, piv, b)
+        x, ferr, berr = O.lu_refine(trans, a, lu, piv, b, x0.copy())
+        op = a if trans == "N" else (a.T if trans == "T" else a.conj().T)
+        n1 = lambda z: np.abs(z.real) + np.abs(z.imag)  # noqa: E731
+        for j in range(3):
+            r = b[:, j] - op @ x[:, j]
+            den = n1(op) @ n1(x[:, j]) + n1(b[:, j])
+            assert berr[j] <= 4 * np.finfo(float).eps
+            assert abs(berr[j] - np.max(n1(r) / den)) <= 2 * np.finfo(float).eps
+        xt = np.linalg.solve(op, b)
+        fwd = np.max(np.abs(x - xt), axis=0) / np.max(np.abs(x), axis=0)
+        assert np.all(fwd <= ferr)
+        res = (lapack.zgesvx if cplx else lapack.dgesvx)(a, b, fact="N", trans=trans)
+        # real: the same estimate to 2%; complex: zgerfs' refinement stops at a
+        # different x, which moves the estimate by ~10%
+        np.testing.assert_allclose(ferr, 2 * res[9], rtol=0.25 if cplx else 0.02)
