@@ -23,6 +23,11 @@ bool vendor_update_enabled() {
 
 int vendor_dgemm_nn(Handle* h, hipStream_t st, int m, int n, int k, double alpha, const double* A, long lda,
                     const double* B, long ldb, double beta, double* C, long ldc) {
+  return vendor_dgemm(h, st, 'N', 'N', m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int vendor_dgemm(Handle* h, hipStream_t st, char ta, char tb, int m, int n, int k, double alpha, const double* A,
+                 long lda, const double* B, long ldb, double beta, double* C, long ldc) {
   if (!h->blas) {
     rocblas_handle rh = nullptr;
     if (rocblas_create_handle(&rh) != rocblas_status_success) return RNPX_ERR_HIP;
@@ -30,8 +35,9 @@ int vendor_dgemm_nn(Handle* h, hipStream_t st, int m, int n, int k, double alpha
   }
   rocblas_handle rh = (rocblas_handle)h->blas;
   if (rocblas_set_stream(rh, st) != rocblas_status_success) return RNPX_ERR_HIP;
-  const rocblas_status s = rocblas_dgemm(rh, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A,
-                                         (rocblas_int)lda, B, (rocblas_int)ldb, &beta, C, (rocblas_int)ldc);
+  auto op = [](char t) { return t == 'N' ? rocblas_operation_none : rocblas_operation_transpose; };
+  const rocblas_status s = rocblas_dgemm(rh, op(ta), op(tb), m, n, k, &alpha, A, (rocblas_int)lda, B,
+                                         (rocblas_int)ldb, &beta, C, (rocblas_int)ldc);
   return s == rocblas_status_success ? RNPX_OK : RNPX_ERR_HIP;
 }
 

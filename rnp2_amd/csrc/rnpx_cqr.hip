@@ -475,6 +475,7 @@ constexpr int kBlkPassWords = 32;                 // per reduction pass
 constexpr int kBlkPub1 = 6 * kBlkPassWords;        // panel 1's LU published
 constexpr int kBlkPubT1 = kBlkPub1 + 1;            // T1 published
 constexpr int kBlkPub2 = kBlkPub1 + 2;             // panel 2's LU published
+constexpr int kBlkResident = kBlkPub1 + 3;         // workgroups started (for the gate below)
 static_assert(kBlkPub2 < kCqrBlockSyncWords, "block sync words");
 constexpr int kCross = 16 * 256;                   // 64x64 product partial (16 tiles)
 // scratch (doubles)
@@ -653,6 +654,7 @@ __device__ __forceinline__ void blk_second_r(double (*M1)[LDA_S], double (*M2)[L
 
 __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
   RNPX_CQR_GLOBAL(a);
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&a.sync[kBlkResident], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __shared__ double Ab1[64][LDA_S];
   __shared__ double Ab2[64][LDA_S];
   __shared__ double M1[NB][LDA_S];
@@ -930,6 +932,28 @@ hipError_t launch_cqr_block(const CqrBlockArgs& a, int G, hipStream_t st) {
       !a.fail || !a.sync || !a.abortf)
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(cqr_block_kernel, dim3(G), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// Scheduling gate for the look-ahead: one thread waits until all G
+// workgroups of the block kernel whose sync words start at block_sync have
+// started, so that a GEMM queued behind the gate on another stream cannot
+// take the CUs those workgroups need first.  Only the order of dispatch
+// depends on it (the GEMM reads nothing the block kernel writes), so the
+// wait gives up after 2 ms.
+__global__ void cqr_gate_kernel(const unsigned* block_sync, unsigned G) {
+  if (threadIdx.x != 0) return;
+  const unsigned* c = block_sync + kBlkResident;
+  const uint64_t t0 = wall_clock64();
+  while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < G) {
+    if (wall_clock64() - t0 > 200000ull) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+hipError_t launch_cqr_gate(const unsigned* block_sync, int G, hipStream_t st) {
+  if (!block_sync || G < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(cqr_gate_kernel, dim3(1), dim3(64), 0, st, block_sync, (unsigned)G);
   return hipGetLastError();
 }
 

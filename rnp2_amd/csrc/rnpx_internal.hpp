@@ -87,6 +87,8 @@ int hip_status(hipError_t e);
 bool vendor_update_enabled();
 int vendor_dgemm_nn(Handle* h, hipStream_t st, int m, int n, int k, double alpha, const double* A, long lda,
                     const double* B, long ldb, double beta, double* C, long ldc);
+int vendor_dgemm(Handle* h, hipStream_t st, char ta, char tb, int m, int n, int k, double alpha, const double* A,
+                 long lda, const double* B, long ldb, double beta, double* C, long ldc);
 void vendor_release(Handle* h);
 int create_handle(Handle** out, size_t budget);
 Handle* default_handle(int* status);

@@ -611,6 +611,10 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
     Handle* h;
     ~CapReset() { h->panel_cap = 0; }
   } cap_reset{h};
+  static const int gate_rows = [] {
+    const char* e = getenv("RNPX_GATE_ROWS");
+    return e ? atoi(e) : 6144;
+  }();
   const GemmWork panel_work = {ps, &h->wbuf2, &h->zbuf2, &h->part2, false};
   const GemmWork upd_work = {us, &h->wbuf, &h->zbuf, &h->part, true};
   RNPX_TRY(hipEventRecord(h->ev_start, st));
@@ -632,6 +636,14 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
       RNPX_TRY(hipStreamWaitEvent(ps, h->ev_next, 0));
       if ((s = panel_phase(b + 1, ps, &panel_work))) return s;
       RNPX_TRY(hipEventRecord(h->ev_pan[(b + 1) & 1], ps));
+      if constexpr (!S<T>::cplx) {
+        // Once the panels are the longer chain, the rest of the update waits
+        // until every workgroup of block b+1's kernel has a CU.
+        const int Gb = use_block ? cqr_block_groups(m - c0, std::min(256, h->num_cus)) : 0;
+        if (wn == 2 * nb && Gb > 0 && m - c0 <= gate_rows && c0 + wn < n)
+          RNPX_TRY(launch_cqr_gate((const unsigned*)h->sync.p + 1 + 2 * npan + (size_t)kCqrSyncWords * (c0 / nb), Gb,
+                                   us));
+      }
       if ((s = update(b, c0 + wn, n, &upd_work))) return s;
     } else {
       if ((s = update(b, c0, n, &upd_work))) return s;

@@ -2,10 +2,13 @@
 // threads (4 waves), used by the CholeskyQR2 panel (rnpx_cqr.hip).
 //
 // The matrices live in LDS, row-major with row stride S (padded).  They are
-// processed as 4x4 blocks of 16x16: the diagonal blocks are factored by one
-// wave with one column (or row) per lane and v_readlane broadcasts (16-step
-// dependency chains), everything off the diagonal is 16x16x16 block products
-// on FP64 MFMA spread over the four waves.
+// processed as 4x4 blocks of 16x16.  Step b of a factorization is a 16-step
+// dependency chain on one wave over block row b (one column per lane, with
+// v_readlane broadcasts of the pivot row), which yields the diagonal block
+// and the blocks right of it at once; the trailing updates and the inverses'
+// off-diagonal blocks are 16x16x16 block products on FP64 MFMA spread over
+// the four waves, and the diagonal blocks' inverses run on an idle wave
+// beside the next chain.
 #pragma once
 #include "rnpx_kernels.hpp"
 
@@ -62,32 +65,6 @@ __device__ __forceinline__ void store_blk(const f64x4& acc, double* C, int br, i
   for (int r = 0; r < 4; ++r) C[(16 * br + (lane >> 4) + 4 * r) * S + 16 * bc + (lane & 15)] = scale * acc[r];
 }
 
-// One wave: Cholesky of diagonal block b of M in place (upper R, zeros below
-// within the block).  Acceptance: every pivot d_j > gdiag_j 2^-40 and
-// gdiag_j in [2^-600, 2^600] (gdiag = the original diagonal).
-template <int S>
-__device__ __forceinline__ bool chol16(double* M, int b, const double* gdiag, int lane) {
-  const int c = lane & 15;
-  double w[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) w[i] = M[(16 * b + i) * S + 16 * b + c];
-  bool ok = true;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const double d = readlane_t(w[j], j);
-    const double g = gdiag[16 * b + j];
-    ok = ok && (g >= 0x1p-600) && (g <= 0x1p600) && (d > g * 0x1p-40);
-    w[j] *= rsq_nr(d);
-#pragma unroll
-    for (int i = j + 1; i < 16; ++i) w[i] = fma(-readlane_t(w[j], i), w[j], w[i]);
-  }
-  if (lane < 16) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) M[(16 * b + i) * S + 16 * b + c] = (i <= c) ? w[i] : 0.0;
-  }
-  return ok;
-}
-
 // One wave: X(bx) = inverse of the upper triangle of block (b, b) of M
 // (zeros below the diagonal); X is written at block (b, b) of a matrix of
 // stride SX.
@@ -135,35 +112,6 @@ __device__ __forceinline__ void inv_unit_lower16(const double* M, int b, double*
   }
 }
 
-// One wave: LU without pivoting of block (b, b) of M minus D, D_j = -1 when
-// the running pivot is positive and +1 otherwise (into dsg[16 b + j]);
-// U on and above the diagonal, the unit lower L below.
-template <int S>
-__device__ __forceinline__ void lu16(double* M, int b, double* dsg, int lane) {
-  const int c = lane & 15;
-  double w[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) w[i] = M[(16 * b + i) * S + 16 * b + c];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const double p = readlane_t(w[j], j);
-    const double dj = (p > 0.0) ? -1.0 : 1.0;
-    const double u = p - dj;  // |u| = |p| + 1 >= 1
-    const double inv = rcp_nr(u);
-    if (c == j) w[j] = u;
-    if (lane == 0) dsg[16 * b + j] = dj;
-#pragma unroll
-    for (int i = j + 1; i < 16; ++i) {
-      const double lij = readlane_t(w[i], j) * inv;
-      w[i] = (c > j) ? fma(-lij, w[j], w[i]) : ((c == j) ? lij : w[i]);
-    }
-  }
-  if (lane < 16) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) M[(16 * b + i) * S + 16 * b + c] = w[i];
-  }
-}
-
 // All waves: off-diagonal blocks of X = U^-1 (block upper triangular) given
 // its diagonal blocks and U's blocks above the diagonal in M:
 // X_ij = -X_ii sum_{k=i+1..j} U_ik X_kj, by distance j - i = 1, 2, 3.
@@ -207,93 +155,166 @@ __device__ __forceinline__ void inv_lower_offdiag(const double* M, double* Z, do
   }
 }
 
+// One wave: rows 16b .. 16b+15 of the upper Cholesky factor from the
+// partially reduced M, one column per lane across the whole block row, so
+// that the diagonal block (zeros below its diagonal) and R_bc for every
+// c > b come out of the same 16-step chain.  Lanes left of the block row
+// compute on what lies there and store nothing.  Acceptance: every pivot
+// d_j > gdiag_j 2^-40 and gdiag_j in [2^-600, 2^600] (gdiag = the original
+// diagonal).
+template <int S>
+__device__ __forceinline__ bool chol_strip(double* M, int b, const double* gdiag, int lane) {
+  const int c = lane, c0 = 16 * b;
+  double w[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) w[i] = M[(c0 + i) * S + c];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const double d = readlane_t(w[j], c0 + j);
+    const double g = gdiag[c0 + j];
+    ok = ok && (g >= 0x1p-600) && (g <= 0x1p600) && (d > g * 0x1p-40);
+    w[j] *= rsq_nr(d);
+#pragma unroll
+    for (int i = j + 1; i < 16; ++i) w[i] = fma(-readlane_t(w[j], c0 + i), w[j], w[i]);
+  }
+  if (c >= c0) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) M[(c0 + i) * S + c] = (i <= c - c0) ? w[i] : 0.0;
+  }
+  return ok;
+}
+
+// One wave: the 16-step LU chain of M - D, D_j = -1 when the running pivot
+// is positive and +1 otherwise (into dsg), at diagonal block b over its
+// block row (ROW: one column per lane; U_bb, L_bb and U_bc for c > b) or its
+// block column (one row per lane; L_cb for c > b).  Both chains apply the
+// same operations to the diagonal block in the same order (l = w inv, then
+// fma(-l, u, w)), so they agree on it bit for bit, pivot signs included;
+// the row chain stores it.  w holds what lu_strip_load read; the two loads
+// must both precede either chain's stores.
+template <int S, bool ROW>
+__device__ __forceinline__ void lu_strip_load(const double* M, int b, double (&w)[16], int lane) {
+  const int x = lane, x0 = 16 * b;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) w[i] = ROW ? M[(x0 + i) * S + x] : M[x * S + x0 + i];
+}
+template <int S, bool ROW>
+__device__ __forceinline__ void lu_strip(double* M, int b, double (&w)[16], double* dsg, int lane) {
+  const int x = lane, x0 = 16 * b;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int pj = x0 + j;
+    const double p = readlane_t(w[j], pj);
+    const double dj = (p > 0.0) ? -1.0 : 1.0;
+    const double u = p - dj;  // |u| = |p| + 1 >= 1
+    const double inv = rcp_nr(u);
+    if (ROW) {
+      if (x == pj) w[j] = u;
+      if (lane == 0) dsg[pj] = dj;
+#pragma unroll
+      for (int i = j + 1; i < 16; ++i) {
+        const double lij = readlane_t(w[i], pj) * inv;
+        w[i] = (x > pj) ? fma(-lij, w[j], w[i]) : ((x == pj) ? lij : w[i]);
+      }
+    } else {
+      const double l = w[j] * inv;
+#pragma unroll
+      for (int jp = j + 1; jp < 16; ++jp) {
+        const double ujp = readlane_t(w[jp], pj);  // row pj is not updated at step j
+        w[jp] = (x > pj) ? fma(-l, ujp, w[jp]) : w[jp];
+      }
+      w[j] = (x > pj) ? l : ((x == pj) ? u : w[j]);
+    }
+  }
+  if (ROW) {
+    if (x >= x0) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) M[(x0 + i) * S + x] = w[i];
+    }
+  } else if (x >= x0 + 16) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) M[x * S + x0 + i] = w[i];
+  }
+}
+
 // All waves: Cholesky M = R^T R of the symmetric 64x64 M in place (R upper;
 // blocks below the diagonal left as they are) and X = R^-1 (upper blocks).
-// Returns the acceptance test (the same in every thread).
+// Wave 0 runs the block-row chain b while wave 1 inverts diagonal block
+// b-1; all waves share the trailing updates.  Returns the acceptance test
+// (the same in every thread).
 template <int S>
 __device__ __forceinline__ bool chol64(double* M, double* X, double* tmp, double* gdiag, int* okf, int tid) {
   const int lane = tid & 63, wave = tid >> 6;
-  double* rv = tmp + 4 * 16 * TS;  // 16 reciprocals
+  double* rv = tmp + 4 * 16 * TS;  // 4 x 16 reciprocals
   if (tid < 64) gdiag[tid] = M[tid * S + tid];
   if (tid == 0) *okf = 1;
   __syncthreads();
   bool ok = true;
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    if (wave == 0) {
-      ok = chol16<S>(M, b, gdiag, lane) && ok;
-      inv_upper16<S, S>(M, b, X, b, rv, lane);
-    }
+    if (wave == 0) ok = chol_strip<S>(M, b, gdiag, lane) && ok;
+    if (wave == 1 && b > 0) inv_upper16<S, S>(M, b - 1, X, b - 1, rv + 16 * (b - 1), lane);
     __syncthreads();
-    // R_bc = R_bb^-T G_bc for c > b
-    if (wave < 3 - b) {
-      const int c = b + 1 + wave;
-      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-      mma16<S, S>(acc, X, b, b, true, M, b, c, false, false, lane);
-      store_blk<S>(acc, M, b, c, 1.0, lane);
-    }
-    __syncthreads();
-    // G_cd -= R_bc^T R_bd for b < c <= d
-    const int n = 3 - b, npairs = n * (n + 1) / 2;
-    for (int e = wave; e < npairs; e += 4) {
-      int c = 0, rem = e;
-      while (rem >= n - c) {
-        rem -= n - c;
-        ++c;
+    if (b < 3) {
+      // G_cd -= R_bc^T R_bd for b < c <= d
+      const int n = 3 - b, npairs = n * (n + 1) / 2;
+      for (int e = wave; e < npairs; e += 4) {
+        int c = 0, rem = e;
+        while (rem >= n - c) {
+          rem -= n - c;
+          ++c;
+        }
+        const int cc = b + 1 + c, dd = cc + rem;
+        f64x4 acc;
+        load_blk<S>(acc, M, cc, dd, lane);
+        mma16<S, S>(acc, M, cc, b, true, M, b, dd, false, true, lane);
+        store_blk<S>(acc, M, cc, dd, 1.0, lane);
       }
-      const int cc = b + 1 + c, dd = cc + rem;
-      f64x4 acc;
-      load_blk<S>(acc, M, cc, dd, lane);
-      mma16<S, S>(acc, M, cc, b, true, M, b, dd, false, true, lane);
-      store_blk<S>(acc, M, cc, dd, 1.0, lane);
+      __syncthreads();
     }
-    __syncthreads();
   }
+  if (wave == 1) inv_upper16<S, S>(M, 3, X, 3, rv + 48, lane);
   if (wave == 0 && lane == 0 && !ok) *okf = 0;
+  __syncthreads();
   inv_upper_offdiag<S>(M, X, tmp, lane, wave);  // ends with a barrier
   return *okf != 0;
 }
 
 // All waves: M (64x64, holding Q_top) := the LU factors of M - D (U on and
 // above the diagonal, unit lower L below), D into dsg, X = U^-1 (upper
-// blocks).  tmp: 4 scratch blocks; the L_bb^-1 of each step uses block 0.
+// blocks).  The block-row and block-column chains of step b run side by
+// side on waves 0 and 1 while wave 2 inverts U's diagonal block b-1.
 template <int S>
 __device__ __forceinline__ void lu64(double* M, double* dsg, double* X, double* tmp, int tid) {
   const int lane = tid & 63, wave = tid >> 6;
-  double* rv = tmp + 4 * 16 * TS;  // 16 reciprocals
+  double* rv = tmp + 4 * 16 * TS;  // 4 x 16 reciprocals
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    if (wave == 0) lu16<S>(M, b, dsg, lane);
+    double w[16];
+    // the column chain reads the diagonal block the row chain overwrites
+    if (wave == 0) lu_strip_load<S, true>(M, b, w, lane);
+    if (wave == 1 && b < 3) lu_strip_load<S, false>(M, b, w, lane);
+    if (b < 3) __syncthreads();
+    if (wave == 0) lu_strip<S, true>(M, b, w, dsg, lane);
+    if (wave == 1 && b < 3) lu_strip<S, false>(M, b, w, dsg, lane);
+    if (wave == 2 && b > 0) inv_upper16<S, S>(M, b - 1, X, b - 1, rv + 16 * (b - 1), lane);
     __syncthreads();
-    // the two inverses of the diagonal block side by side
-    if (wave == 1) inv_upper16<S, S>(M, b, X, b, rv, lane);
-    if (wave == 2) inv_unit_lower16<S, TS>(M, b, tmp, 0, lane);
-    __syncthreads();
-    // L_cb = W_cb U_bb^-1 and U_bc = L_bb^-1 W_bc for c > b
-    const int n = 3 - b;
-    for (int e = wave; e < 2 * n; e += 4) {
-      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-      if (e < n) {
-        const int c = b + 1 + e;
-        mma16<S, S>(acc, M, c, b, false, X, b, b, false, false, lane);
-        store_blk<S>(acc, M, c, b, 1.0, lane);
-      } else {
-        const int c = b + 1 + (e - n);
-        mma16<TS, S>(acc, tmp, 0, 0, false, M, b, c, false, false, lane);
-        store_blk<S>(acc, M, b, c, 1.0, lane);
+    if (b < 3) {
+      // W_cd -= L_cb U_bd for c, d > b
+      const int n = 3 - b;
+      for (int e = wave; e < n * n; e += 4) {
+        const int c = b + 1 + e / n, d = b + 1 + e % n;
+        f64x4 acc;
+        load_blk<S>(acc, M, c, d, lane);
+        mma16<S, S>(acc, M, c, b, false, M, b, d, false, true, lane);
+        store_blk<S>(acc, M, c, d, 1.0, lane);
       }
+      __syncthreads();
     }
-    __syncthreads();
-    // W_cd -= L_cb U_bd for c, d > b
-    for (int e = wave; e < n * n; e += 4) {
-      const int c = b + 1 + e / n, d = b + 1 + e % n;
-      f64x4 acc;
-      load_blk<S>(acc, M, c, d, lane);
-      mma16<S, S>(acc, M, c, b, false, M, b, d, false, true, lane);
-      store_blk<S>(acc, M, c, d, 1.0, lane);
-    }
-    __syncthreads();
   }
+  if (wave == 2) inv_upper16<S, S>(M, 3, X, 3, rv + 48, lane);
+  __syncthreads();
   inv_upper_offdiag<S>(M, X, tmp, lane, wave);
 }
 
@@ -304,15 +325,6 @@ __device__ __forceinline__ void inv_unit_lower64(const double* M, double* Z, dou
   inv_unit_lower16<S, S>(M, wave, Z, wave, lane);  // each wave one diagonal block
   __syncthreads();
   inv_lower_offdiag<S>(M, Z, tmp, lane, wave);
-}
-
-// All waves: X = U^-1 for the upper triangle of M.
-template <int S>
-__device__ __forceinline__ void inv_upper64(const double* M, double* X, double* tmp, int tid) {
-  const int lane = tid & 63, wave = tid >> 6;
-  inv_upper16<S, S>(M, wave, X, wave, tmp + 4 * 16 * TS + 16 * wave, lane);
-  __syncthreads();
-  inv_upper_offdiag<S>(M, X, tmp, lane, wave);
 }
 
 }  // namespace small64
