@@ -241,8 +241,10 @@ size_t rnpx_workspace_bytes(rnpx_handle h);
 
 /* Live kernel timing on the handle's stream (HIP events around each launch
  * of a kernel family).  kernel: 0 trailing update C -= V Z, 1 W = V^H C,
- * 2 panel, 3 T factor.  flops/bytes are the algorithmic counts of the
- * recorded launches. */
+ * 2 panel, 3 T factor.  on: 0 off, 1 every family, otherwise bit (k + 1)
+ * of on selects family k (fewer events perturb the timed work less).
+ * Enabling resets the records.  flops/bytes are the algorithmic counts of
+ * the recorded launches. */
 int rnpx_profile_enable(rnpx_handle h, int on);
 int rnpx_profile_read(rnpx_handle h, int kernel, long *launches, double *ms, double *flops,
                       double *bytes);

@@ -592,8 +592,13 @@ class Handle:
 
     KERNELS = {"update": 0, "wgemm": 1, "panel": 2, "tfactor": 3}
 
-    def profile(self, on=True):
-        _check(lib().rnpx_profile_enable(self.h, 1 if on else 0), "rnpx_profile_enable")
+    def profile(self, on=True, kernels=None):
+        """Live HIP-event timing of the kernel families named in kernels
+        (keys of KERNELS; None = all of them).  Enabling resets the records."""
+        v = 0
+        if on:
+            v = 1 if kernels is None else sum(1 << (self.KERNELS[k] + 1) for k in kernels)
+        _check(lib().rnpx_profile_enable(self.h, v), "rnpx_profile_enable")
 
     def profile_read(self, kernel):
         n, ms, fl, by = ctypes.c_long(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

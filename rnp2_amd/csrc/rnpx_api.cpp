@@ -730,6 +730,7 @@ int rnpx_profile_enable(rnpx_handle h, int on) {
   if (!h) return -1;
   prof_reset(H(h));
   H(h)->prof = (on != 0);
+  H(h)->prof_mask = (on == 1) ? ~0u : ((unsigned)on >> 1);
   return RNPX_OK;
 }
 int rnpx_profile_read(rnpx_handle h, int kernel, long* launches, double* ms, double* flops, double* bytes) {

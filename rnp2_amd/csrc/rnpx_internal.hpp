@@ -70,11 +70,12 @@ struct Handle {
   int update_prec = 0;                 // 1: trailing-update products on FP32 MFMA
   void* blas = nullptr;                // rocBLAS handle of the vendor update product (lazy)
   bool prof = false;
+  unsigned prof_mask = ~0u;  // kernel ids (bit per KernelId) that are timed
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> event_pool;
   hipEvent_t take_event();
   // Bracket one launch: call prof_begin before and prof_end after.
-  int prof_begin(hipEvent_t* a, hipStream_t s = nullptr);
+  int prof_begin(int kid, hipEvent_t* a, hipStream_t s = nullptr);
   void prof_end(int kid, hipEvent_t a, double flops, double bytes, hipStream_t s = nullptr);
   int ensure_pipeline();
   int ensure_partition(int cus);

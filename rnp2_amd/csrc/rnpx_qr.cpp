@@ -56,9 +56,9 @@ hipEvent_t Handle::take_event() {
   if (hipEventCreate(&e) != hipSuccess) return nullptr;
   return e;
 }
-int Handle::prof_begin(hipEvent_t* a, hipStream_t s) {
+int Handle::prof_begin(int kid, hipEvent_t* a, hipStream_t s) {
   *a = nullptr;
-  if (!prof) return 0;
+  if (!prof || !((prof_mask >> kid) & 1u)) return 0;
   *a = take_event();
   if (*a) hipEventRecord(*a, s ? s : stream);
   return 0;
@@ -239,7 +239,7 @@ static int pick_rpt(const Handle* h, int mp, int nb, bool cplx) {
 
 static int pick_ksplit(const Handle* h, int rows, int cols_tiles) {
   if (h->tune.ksplit > 0) return h->tune.ksplit;
-  int want = 512 / std::max(1, cols_tiles);
+  int want = 1024 / std::max(1, cols_tiles);
   int maxs = std::max(1, rows / 512);
   return std::max(1, std::min(want, std::min(maxs, 32)));
 }
@@ -287,14 +287,14 @@ static int apply_block_dev(Handle* h, char side, char trans, int mc, int nc, int
     // Mixed precision (real only): the two big products on FP32 MFMA.
     bool f32 = false;
     if constexpr (!S<T>::cplx) f32 = (h->update_prec == 1);
-    if (gw->prof) h->prof_begin(&ev, st);
+    if (gw->prof) h->prof_begin(K_WGEMM, &ev, st);
     if constexpr (!S<T>::cplx) {
       if (f32) RNPX_TRY(gemm32('C', 'N', k, nc, mc, 1.0, V, ldv, C, ldc, 0.0, W, k, ks, (T*)pb.p, st));
     }
     if (!f32) RNPX_TRY(gemm<T>('C', 'N', k, nc, mc, one, V, ldv, C, ldc, zero, W, k, ks, (T*)pb.p, st));
     if (gw->prof) h->prof_end(K_WGEMM, ev, fl, esz * ((double)mc * nc + (double)mc * k + (double)k * nc), st);
     RNPX_TRY(gemm<T>(trans == 'N' ? 'N' : 'C', 'N', k, nc, k, one, Tm, ldt, W, k, zero, Z, k, 1, nullptr, st));
-    if (gw->prof) h->prof_begin(&ev, st);
+    if (gw->prof) h->prof_begin(K_UPDATE, &ev, st);
     if constexpr (!S<T>::cplx) {
       if (f32) RNPX_TRY(gemm32('N', 'N', mc, nc, k, -1.0, V, ldv, Z, k, 1.0, C, ldc, 1, nullptr, st));
     }
@@ -490,7 +490,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
     T* Vo = (T*)vb[q]->p;
     T* To = (T*)tb[q]->p;
     hipEvent_t ev;
-    h->prof_begin(&ev, ps);
+    h->prof_begin(K_PANEL, &ev, ps);
     if constexpr (!S<T>::cplx) {
       const int Gb = use_block ? cqr_block_groups(m - i, std::min(256, h->num_cus)) : 0;
       if (wo == 2 * nb && Gb > 0) {

@@ -206,3 +206,30 @@ def test_reference_test_identities():
         errs = [e1, e2, e3, e4, e5, e6, e7]
         print(f"        {['%.3f' % e for e in errs]}", flush=True)
         assert max(errs) < 30.0
+
+
+def test_profile_families():
+    """Live kernel timing selects kernel families: with only the trailing
+    update instrumented, no W = V^H C launch is recorded, and enabling resets
+    the records."""
+    import torch
+    m = n = 1024
+    print(f"[case] profile families {m}x{n}", flush=True)
+    h = R.Handle()
+    a = R.colmajor_empty(m, n, torch.float64)
+    tau = torch.empty(n, dtype=torch.float64, device="cuda")
+    h.larnv(2, [1, 2, 3, 5], a)
+    h.profile(True, kernels=["update"])
+    h.geqrf(a, tau)
+    h.synchronize()
+    upd, wg = h.profile_read("update"), h.profile_read("wgemm")
+    assert upd["launches"] > 0 and upd["ms"] > 0 and upd["flops"] > 0
+    assert wg["launches"] == 0
+    h.larnv(2, [1, 2, 3, 5], a)
+    h.profile(True)
+    h.geqrf(a, tau)
+    h.synchronize()
+    assert h.profile_read("wgemm")["launches"] > 0
+    assert h.profile_read("update")["launches"] == upd["launches"]
+    h.profile(False)
+    assert h.profile_read("update")["launches"] == 0
