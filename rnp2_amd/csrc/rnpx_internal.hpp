@@ -41,7 +41,7 @@ struct Handle {
   Buffer vbuf2, gram2, tbuf2;    // second set for the look-ahead pipeline
   hipStream_t pstream = nullptr;  // panel stream of the look-ahead pipeline
   hipStream_t ext_pstream = nullptr;  // shared panel stream supplied by the owner (not destroyed)
-  hipEvent_t ev_pan[2] = {nullptr, nullptr}, ev_next = nullptr, ev_start = nullptr;
+  hipEvent_t ev_pan[2] = {nullptr, nullptr}, ev_next = nullptr, ev_start = nullptr, ev_rest = nullptr;
   // CU-partitioned look-ahead: the panel stream and the update stream run on
   // disjoint sets of CUs (stream CU masks), so a panel never queues behind
   // the trailing update's workgroups.  part_cus = CUs of the panel set, 0 when

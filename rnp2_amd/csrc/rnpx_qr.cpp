@@ -79,7 +79,7 @@ int Handle::ensure_pipeline() {
     e = hipStreamCreateWithFlags(&pstream, hipStreamNonBlocking);
     if (e != hipSuccess) return hip_status(e);
   }
-  for (hipEvent_t* ev : {&ev_pan[0], &ev_pan[1], &ev_next, &ev_start}) {
+  for (hipEvent_t* ev : {&ev_pan[0], &ev_pan[1], &ev_next, &ev_start, &ev_rest}) {
     e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
     if (e != hipSuccess) return hip_status(e);
   }
@@ -133,7 +133,7 @@ Handle::~Handle() {
   }
   for (auto e : event_pool) hipEventDestroy(e);
   if (pstream) hipStreamSynchronize(pstream);
-  for (hipEvent_t ev : {ev_pan[0], ev_pan[1], ev_next, ev_start})
+  for (hipEvent_t ev : {ev_pan[0], ev_pan[1], ev_next, ev_start, ev_rest})
     if (ev) hipEventDestroy(ev);
   if (pstream && pstream != ext_pstream) hipStreamDestroy(pstream);
   for (hipStream_t ps2 : {part_ps, part_us}) {
@@ -239,8 +239,11 @@ static int pick_rpt(const Handle* h, int mp, int nb, bool cplx) {
 
 static int pick_ksplit(const Handle* h, int rows, int cols_tiles) {
   if (h->tune.ksplit > 0) return h->tune.ksplit;
+  // About 1024 workgroup tiles in all.  K slices of at least 512, or 128
+  // for the narrow products (one or two column tiles) that sit on the chain
+  // of panels, where latency matters more than the partial sums' traffic.
   int want = 1024 / std::max(1, cols_tiles);
-  int maxs = std::max(1, rows / 512);
+  int maxs = std::max(1, rows / (cols_tiles <= 2 ? 128 : 512));
   return std::max(1, std::min(want, std::min(maxs, 32)));
 }
 
@@ -418,8 +421,9 @@ static int panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, 
 // half as many passes over it at higher arithmetic intensity than K = 64).
 // With look-ahead (the default) the panel phase of block b+1 runs on a
 // second stream while the trailing update of block b runs: the update of
-// block b is split into block b+1's columns (done first, then the panel
-// stream is released) and the rest.  V and T are double-buffered between
+// block b is split into block b+1's columns, applied on the panel stream
+// right behind panel b so that the chain of panels crosses no stream, and
+// the rest, on the update stream.  V and T are double-buffered between
 // consecutive outer blocks; V uses leading dimension m.
 template <typename T>
 int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, long ldb) {
@@ -631,9 +635,11 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
     }
     if (b + 1 < nob) {
       const int wn = std::min(nbo, k - c0);  // block b+1's columns
-      if ((s = update(b, c0, c0 + wn, &upd_work))) return s;
-      RNPX_TRY(hipEventRecord(h->ev_next, us));
-      RNPX_TRY(hipStreamWaitEvent(ps, h->ev_next, 0));
+      // Block b+1's columns are brought up to date on the panel stream, right
+      // behind panel b: the rest of block b-1's update (which covered them)
+      // is the only work of the other stream it waits for.
+      if (b > 0) RNPX_TRY(hipStreamWaitEvent(ps, h->ev_rest, 0));
+      if ((s = update(b, c0, c0 + wn, &panel_work))) return s;
       if ((s = panel_phase(b + 1, ps, &panel_work))) return s;
       RNPX_TRY(hipEventRecord(h->ev_pan[(b + 1) & 1], ps));
       if constexpr (!S<T>::cplx) {
@@ -649,6 +655,8 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
       if ((s = update(b, c0, n, &upd_work))) return s;
     }
     if ((s = update_rhs(b, &upd_work))) return s;
+    // block b's V and T are free again once this stream is past here
+    RNPX_TRY(hipEventRecord(h->ev_rest, us));
   }
   if (us != st) {
     // the last panel is ordered before the update stream's last wait
