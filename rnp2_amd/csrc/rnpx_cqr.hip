@@ -150,16 +150,36 @@ __device__ __forceinline__ void gram_allreduce(const CqrArgs& a, int pass, int G
   wait_flag(&sw[kSyncDone], a.abortf);
 }
 
-// Load the reduced symmetric Gram into M (full 64x64).
+// Load the reduced symmetric Gram into M (full 64x64).  Here and in the
+// other global-to-LDS copies every load is issued before the first LDS
+// store, so that the loads' round trips overlap.
 __device__ __forceinline__ void load_gram(const double* gs, double (*M)[LDA_S], int tid) {
-  for (int e = tid; e < kGramElems; e += 256) {
+  constexpr int Q = kGramElems / 256;
+  double v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = gs[tid + 256 * q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + 256 * q;
     const int t = e >> 8, r = (e >> 4) & 15, c = e & 15;
     int ti, tj;
     tile_of(t, ti, tj);
     if (ti == tj && r > c) continue;  // one writer per entry of a diagonal tile
-    const double v = gs[e];
-    M[16 * ti + r][16 * tj + c] = v;
-    M[16 * tj + c][16 * ti + r] = v;
+    M[16 * ti + r][16 * tj + c] = v[q];
+    M[16 * tj + c][16 * ti + r] = v[q];
+  }
+}
+
+// M := the row-major 64x64 block at src (global memory).
+__device__ __forceinline__ void load64(const double* src, double (*M)[LDA_S], int tid) {
+  constexpr int Q = NB * NB / 256;
+  double v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = src[tid + 256 * q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + 256 * q;
+    M[e / NB][e % NB] = v[q];
   }
 }
 
@@ -535,9 +555,15 @@ __device__ __forceinline__ void cross_partial(const double (*X)[LDA_S], const do
 }
 
 __device__ __forceinline__ void load_tiles16(const double* gs, double (*M)[LDA_S], int tid) {
-  for (int e = tid; e < kCross; e += 256) {
+  constexpr int Q = kCross / 256;
+  double v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = gs[tid + 256 * q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + 256 * q;
     const int t = e >> 8, r = (e >> 4) & 15, c = e & 15;
-    M[16 * (t >> 2) + r][16 * (t & 3) + c] = gs[e];
+    M[16 * (t >> 2) + r][16 * (t & 3) + c] = v[q];
   }
 }
 
@@ -612,7 +638,7 @@ __device__ __noinline__ void blk_refactor(CqrBlockArgs a, double* sh, int tid) {
 // M1 holds R2.
 __device__ __forceinline__ void blk_r_product(const double (*M1)[LDA_S], double (*M2)[LDA_S], const double* r1,
                                               f64x4 (&acc)[4], int tid, int lane, int wave) {
-  for (int e = tid; e < NB * NB; e += 256) M2[e / NB][e % NB] = r1[e];
+  load64(r1, M2, tid);
   __syncthreads();
   upper_product_row(M1, M2, false, nullptr, acc, lane, wave);
 }
@@ -744,7 +770,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
         }
     }
     __syncthreads();
-    for (int e = tid; e < NB * NB; e += 256) M2[e / NB][e % NB] = sc[kBU1i + e];
+    load64(sc + kBU1i, M2, tid);
     __syncthreads();
     times_upper<64>(Ab1, M2, lane, wave);  // Ab1 = V1 rows
   }
@@ -769,7 +795,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
   blk_wait(a, 2);
   wait_flag(&a.sync[kBlkPubT1], a.abortf);
   load_tiles16(a.gsum + 2 * kCross, M2, tid);                                     // W
-  for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = sc[kBT1 + e];   // T1
+  load64(sc + kBT1, M1, tid);  // T1
   __syncthreads();
   {
     f64x4 acc[4];
@@ -872,7 +898,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
         }
     }
     __syncthreads();
-    for (int e = tid; e < NB * NB; e += 256) M2[e / NB][e % NB] = sc[kBU2i + e];
+    load64(sc + kBU2i, M2, tid);
     __syncthreads();
     times_upper<64>(Ab2, M2, lane, wave);  // V2 rows
     for (int e = tid; e < 64 * NB; e += 256) {
@@ -904,7 +930,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
   }
   blk_wait(a, 5);
   load_tiles16(a.gsum + 5 * kCross, M2, tid);                                     // S
-  for (int e = tid; e < NB * NB; e += 256) M1[e / NB][e % NB] = sc[kBT1 + e];   // T1
+  load64(sc + kBT1, M1, tid);  // T1
   __syncthreads();
   f64x4 acc[4];
   prod64<false>(M1, M2, acc, lane, wave);  // X = T1 S
