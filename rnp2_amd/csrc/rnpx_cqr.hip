@@ -503,37 +503,58 @@ constexpr int kBR1a = 0, kBU1i = 4096, kBD1 = 12288, kBTau1 = 12352, kBRF1 = 124
               kBR1b = 20608, kBU2i = 24704, kBD2 = 32896, kBG = 32960, kBTF = 49344;
 static_assert(kBTF + 16384 <= kCqrBlockScratch, "block scratch");
 
-// Reduction over the participating workgroups (index gi of Gp) of partials
-// of ELEMS doubles at part[gi]; the sum lands in gs and every workgroup
-// (participant or not) waits for it.
+// Reduction of Gp partials (ELEMS doubles each, kCross apart from a.part)
+// into gs, by reduce-scatter.  Contributors arrive on the pass's first
+// counter once their partial is stored.  Each helper workgroup (index hi of
+// Hn; hi < 0 for none) then waits for all Gp arrivals, sums its slice of
+// the entries over the partials in a fixed order (so the result does not
+// depend on timing), stores it and arrives on the second counter; the sum
+// is complete once Hn helpers have arrived (blk_wait).  red: 256 doubles of
+// LDS scratch.
 template <int ELEMS>
-__device__ __forceinline__ void blk_contribute(const CqrBlockArgs& a, int pass, int gi, int Gp, double* gs, int tid,
-                                               unsigned* slot) {
+__device__ __forceinline__ void blk_contribute(const CqrBlockArgs& a, int pass, bool contrib, int Gp, int hi, int Hn,
+                                               double* gs, double* red, int tid, unsigned* slot) {
   unsigned* sw = a.sync + pass * kBlkPassWords;
-  const int grp = gi / kGroup, ngroups = (Gp + kGroup - 1) / kGroup;
-  const int gbeg = grp * kGroup, gsize = min(kGroup, Gp - gbeg);
-  if (arrive(&sw[grp], slot) == (unsigned)(gsize - 1)) {
-    // partials and group sums both sit kCross doubles apart
-    if (ngroups == 1) {
-      sum_partials<ELEMS, kCross>(a.part, gsize, gs, tid);
-      publish(&sw[kSyncDone]);
-    } else {
-      sum_partials<ELEMS, kCross>(a.part + (size_t)gbeg * kCross, gsize, a.gpart + (size_t)grp * kCross, tid);
-      if (arrive(&sw[kSyncTop], slot) == (unsigned)(ngroups - 1)) {
-        sum_partials<ELEMS, kCross>(a.gpart, ngroups, gs, tid);
-        publish(&sw[kSyncDone]);
-      }
+  if (contrib) arrive(&sw[0], slot);
+  if (hi < 0) return;
+  wait_count(&sw[0], (unsigned)Gp, a.abortf);
+  const int S = (ELEMS + Hn - 1) / Hn;  // slice length
+  const int e0 = hi * S, len = min(S, ELEMS - e0);
+  if (S <= 128) {
+    // P threads per entry, each over every P-th partial; then the P sums
+    const int P = 256 / S, j = tid / S, x = tid % S;
+    double sum = 0.0;
+    if (j < P && x < len) {
+      const double* src = a.part + e0 + x;
+#pragma unroll 8
+      for (int q = j; q < Gp; q += P) sum += src[(size_t)q * kCross];
+    }
+    red[tid] = sum;
+    __syncthreads();
+    if (tid < len) {
+      double t = red[tid];
+      for (int q = 1; q < P; ++q) t += red[q * S + tid];
+      gs[e0 + tid] = t;
+    }
+  } else {
+    for (int x = tid; x < len; x += 256) {
+      const double* src = a.part + e0 + x;
+      double sum = 0.0;
+#pragma unroll 8
+      for (int q = 0; q < Gp; ++q) sum += src[(size_t)q * kCross];
+      gs[e0 + x] = sum;
     }
   }
+  arrive(&sw[1], slot);
 }
-__device__ __forceinline__ void blk_wait(const CqrBlockArgs& a, int pass) {
-  wait_flag(&a.sync[pass * kBlkPassWords + kSyncDone], a.abortf);
+__device__ __forceinline__ void blk_wait(const CqrBlockArgs& a, int pass, int Hn) {
+  wait_count(&a.sync[pass * kBlkPassWords + 1], (unsigned)Hn, a.abortf);
 }
 template <int ELEMS>
-__device__ __forceinline__ void blk_allreduce(const CqrBlockArgs& a, int pass, bool participate, int gi, int Gp,
-                                              double* gs, int tid, unsigned* slot) {
-  if (participate) blk_contribute<ELEMS>(a, pass, gi, Gp, gs, tid, slot);
-  blk_wait(a, pass);
+__device__ __forceinline__ void blk_allreduce(const CqrBlockArgs& a, int pass, bool contrib, int Gp, int hi, int Hn,
+                                              double* gs, double* red, int tid, unsigned* slot) {
+  blk_contribute<ELEMS>(a, pass, contrib, Gp, hi, Hn, gs, red, tid, slot);
+  blk_wait(a, pass, Hn);
 }
 
 // P = X^T Y over 64 rows (16 tiles of 16x16, tile t = 4 ti + tj, row-major).
@@ -720,7 +741,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
 
   // ---- Panel 1 ----
   gram_partial<64>(Ab1, part, lane, wave);
-  blk_allreduce<kGramElems>(a, 0, true, g, G, a.gsum, tid, &slot);
+  blk_allreduce<kGramElems>(a, 0, true, G, g, G, a.gsum, tmpb, tid, &slot);
   load_gram(a.gsum, M1, tid);
   __syncthreads();
   if (!small64::chol64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, gdiag, &okf, tid)) {
@@ -731,7 +752,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
     for (int e = tid; e < NB * NB; e += 256) sc[kBR1a + e] = M1[e / NB][e % NB];
   times_upper<64>(Ab1, M2, lane, wave);
   gram_partial<64>(Ab1, part, lane, wave);
-  blk_allreduce<kGramElems>(a, 1, true, g, G, a.gsum + kCross, tid, &slot);
+  blk_allreduce<kGramElems>(a, 1, true, G, g, G, a.gsum + kCross, tmpb, tid, &slot);
   load_gram(a.gsum + kCross, M1, tid);
   __syncthreads();
   double dev = dev_from_identity(M1, red4, tid);
@@ -777,7 +798,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
 
   // ---- Panel 1's reflectors applied to panel 2: A2 -= V1 (T1^T (V1^T A2)) ----
   cross_partial(Ab1, Ab2, part, lane, wave);
-  blk_contribute<kCross>(a, 2, g, G, a.gsum + 2 * kCross, tid, &slot);
+  blk_contribute<kCross>(a, 2, true, G, g - 1, G - 1, a.gsum + 2 * kCross, tmpb, tid, &slot);  // g0: T1
   if (g == 0) {
     // T1 while the reduction completes; the LU is still in M1 and rowbuf.
     f64x4 tacc[4];
@@ -792,7 +813,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
       }
     publish(&a.sync[kBlkPubT1]);
   }
-  blk_wait(a, 2);
+  blk_wait(a, 2, G - 1);
   wait_flag(&a.sync[kBlkPubT1], a.abortf);
   load_tiles16(a.gsum + 2 * kCross, M2, tid);                                     // W
   load64(sc + kBT1, M1, tid);  // T1
@@ -815,7 +836,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
   const bool p2 = (g >= 1);
   const int gi = g - 1, G2 = G - 1;
   if (p2) gram_partial<64>(Ab2, a.part + (size_t)gi * kCross, lane, wave);
-  blk_allreduce<kGramElems>(a, 3, p2, gi, G2, a.gsum + 3 * kCross, tid, &slot);
+  blk_allreduce<kGramElems>(a, 3, p2, G2, g, G, a.gsum + 3 * kCross, tmpb, tid, &slot);
   load_gram(a.gsum + 3 * kCross, M1, tid);
   __syncthreads();
   if (!small64::chol64<LDA_S>(&M1[0][0], &M2[0][0], tmpb, gdiag, &okf, tid)) {
@@ -827,7 +848,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
   if (p2) times_upper<64>(Ab2, M2, lane, wave);
   else __syncthreads();
   if (p2) gram_partial<64>(Ab2, a.part + (size_t)gi * kCross, lane, wave);
-  blk_allreduce<kGramElems>(a, 4, p2, gi, G2, a.gsum + 4 * kCross, tid, &slot);
+  blk_allreduce<kGramElems>(a, 4, p2, G2, g, G, a.gsum + 4 * kCross, tmpb, tid, &slot);
   load_gram(a.gsum + 4 * kCross, M1, tid);
   __syncthreads();
   dev = dev_from_identity(M1, red4, tid);
@@ -914,7 +935,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
   // ---- Merge: T12 = -T1 (V1^T V2) T2, by workgroup 1 (which holds LU2) ----
   if (!p2) return;
   cross_partial(Ab1, Ab2, a.part + (size_t)gi * kCross, lane, wave);
-  blk_contribute<kCross>(a, 5, gi, G2, a.gsum + 5 * kCross, tid, &slot);
+  blk_contribute<kCross>(a, 5, true, G2, g - 2, G - 2, a.gsum + 5 * kCross, tmpb, tid, &slot);  // g1: T2
   if (g != 1) return;
   {
     f64x4 tacc[4];
@@ -928,7 +949,7 @@ __global__ __launch_bounds__(256) void cqr_block_kernel(CqrBlockArgs a) {
       }
     store_acc(tacc, Ab2, 1.0, lane, wave);  // Ab2 was last read before the arrival's barrier
   }
-  blk_wait(a, 5);
+  blk_wait(a, 5, G - 2);
   load_tiles16(a.gsum + 5 * kCross, M2, tid);                                     // S
   load64(sc + kBT1, M1, tid);  // T1
   __syncthreads();

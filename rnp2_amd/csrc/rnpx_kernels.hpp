@@ -82,6 +82,24 @@ __device__ __forceinline__ void wait_flag(unsigned* flag, unsigned* abortf) {
   __syncthreads();
 }
 
+// Bounded wait until an agent-scope counter reaches target (abort flag and
+// ~2 s wall-clock limit), then acquire for the whole workgroup.
+__device__ __forceinline__ void wait_count(unsigned* counter, unsigned target, unsigned* abortf) {
+  if (threadIdx.x == 0) {
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+      if (wall_clock64() - t0 > kBarrierTimeoutTicks) {
+        __hip_atomic_store(abortf, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
 // LU panel (rnpx_lu.hip): one cooperative launch factors an mp x jb panel
 // (jb <= 64) with partial pivoting.
 struct LuPanelArgs {
