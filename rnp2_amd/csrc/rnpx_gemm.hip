@@ -237,8 +237,7 @@ __global__ void gemm_reduce_kernel(int M, int N, int ns, const T* __restrict__ p
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)M * N) return;
   const int i = (int)(e % M), j = (int)(e / M);
-  T s = S<T>::zero();
-  for (int q = 0; q < ns; ++q) s = s + part[(size_t)q * M * N + (size_t)e];
+  const T s = sum_strided(part + e, ns, (size_t)M * N);
   T* cp = C + (size_t)i + (size_t)j * ldc;
   *cp = S<T>::is_zero(beta) ? alpha * s : alpha * s + beta * (*cp);
 }

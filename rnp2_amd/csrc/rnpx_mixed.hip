@@ -163,8 +163,7 @@ __global__ void reduce64_kernel(int M, int N, int ns, const double* __restrict__
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)M * N) return;
   const int i = (int)(e % M), j = (int)(e / M);
-  double s = 0.0;
-  for (int q = 0; q < ns; ++q) s += part[(size_t)q * M * N + (size_t)e];
+  const double s = sum_strided(part + e, ns, (size_t)M * N);
   double* cp = C + (size_t)i + (size_t)j * ldc;
   *cp = (beta == 0.0) ? alpha * s : alpha * s + beta * (*cp);
 }

@@ -126,7 +126,7 @@ __global__ void gram_final_kernel(int k, int nb, int nchunks, const T* __restric
   const int j = e % nb, c = e / nb;
   T s = S<T>::zero();
   if (c < k && j < c) {
-    for (int q = 0; q < nchunks; ++q) s = s + part[(size_t)q * nb * nb + (size_t)c * nb + j];
+    s = sum_strided(part + (size_t)c * nb + j, nchunks, (size_t)nb * nb);
   } else if (c < k && j == c) {
     s = tau[c];
   }
