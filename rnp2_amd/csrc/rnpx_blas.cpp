@@ -1,7 +1,10 @@
 // rnpx_blas.cpp -- the vendor DGEMM (rocBLAS) for the plain trailing-update
 // product C -= V Z of real factorizations, where it beats the library's own
-// MFMA GEMM at the update's shapes (K = 128).  One rocBLAS handle per
-// library handle, created on first use, bound to the calling stream per call.
+// MFMA GEMM at the update's shapes (K = 128).  Two rocBLAS handles per
+// library handle, created on first use: one for the look-ahead's panel stream
+// and one for every other stream, so that products running concurrently on
+// the panel and update streams never share a handle's workspace.  Each
+// is bound to the calling stream per call.
 #include "rnpx_internal.hpp"
 
 #include <rocblas/rocblas.h>
@@ -28,12 +31,13 @@ int vendor_dgemm_nn(Handle* h, hipStream_t st, int m, int n, int k, double alpha
 
 int vendor_dgemm(Handle* h, hipStream_t st, char ta, char tb, int m, int n, int k, double alpha, const double* A,
                  long lda, const double* B, long ldb, double beta, double* C, long ldc) {
-  if (!h->blas) {
+  void*& slot = (st && (st == h->pstream || st == h->part_ps)) ? h->blas_side : h->blas;
+  if (!slot) {
     rocblas_handle rh = nullptr;
     if (rocblas_create_handle(&rh) != rocblas_status_success) return RNPX_ERR_HIP;
-    h->blas = rh;
+    slot = rh;
   }
-  rocblas_handle rh = (rocblas_handle)h->blas;
+  rocblas_handle rh = (rocblas_handle)slot;
   if (rocblas_set_stream(rh, st) != rocblas_status_success) return RNPX_ERR_HIP;
   auto op = [](char t) { return t == 'N' ? rocblas_operation_none : rocblas_operation_transpose; };
   const rocblas_status s = rocblas_dgemm(rh, op(ta), op(tb), m, n, k, &alpha, A, (rocblas_int)lda, B,
@@ -42,8 +46,10 @@ int vendor_dgemm(Handle* h, hipStream_t st, char ta, char tb, int m, int n, int 
 }
 
 void vendor_release(Handle* h) {
-  if (h->blas) rocblas_destroy_handle((rocblas_handle)h->blas);
-  h->blas = nullptr;
+  for (void** p : {&h->blas, &h->blas_side}) {
+    if (*p) rocblas_destroy_handle((rocblas_handle)*p);
+    *p = nullptr;
+  }
 }
 
 }  // namespace rnpx

@@ -69,6 +69,7 @@ struct Handle {
   void (*batch_free)(void*) = nullptr;
   int update_prec = 0;                 // 1: trailing-update products on FP32 MFMA
   void* blas = nullptr;                // rocBLAS handle of the vendor update product (lazy)
+  void* blas_side = nullptr;           // a second one for the panel stream (a handle's workspace is not shared)
   bool prof = false;
   unsigned prof_mask = ~0u;  // kernel ids (bit per KernelId) that are timed
   std::vector<ProfRec> recs;

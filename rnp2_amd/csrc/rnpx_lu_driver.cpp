@@ -48,9 +48,25 @@ int getrf_dev(Handle* h, int m, int n, T* A, long lda, int* piv_host, int* info)
   unsigned* sync = (unsigned*)(rowc + 6 * nb);
   int* zl = (int*)(sync + 2 * nb);
   int* prel = zl + npan;
-  for (int pi = 0; pi < npan; ++pi) {
+  // Panel pi factors columns [j, j+jb).  Its update of the columns to the
+  // right is split: the next panel's columns are brought up to date on the
+  // panel stream right behind it (so the chain of panels crosses no stream),
+  // the rest on the handle's stream, overlapped with the next panel.  The
+  // panel stream waits only for the previous panel's rest-of-update, which
+  // covered the next panel's columns; the interchanges of columns left of
+  // the panel go to the handle's stream in panel order.
+  hipStream_t st = h->stream, ps = st;
+  const bool look = npan > 1;
+  if (look) {
+    const int e = h->ensure_pipeline();
+    if (e) return e;
+    ps = h->pstream;
+    RNPX_TRY(hipEventRecord(h->ev_start, st));
+    RNPX_TRY(hipStreamWaitEvent(ps, h->ev_start, 0));
+  }
+  auto panel = [&](int pi) -> int {
     const int j = pi * nb, jb = std::min(nb, k - j), mp = m - j;
-    RNPX_TRY(launch_zero_u32(sync, 2 * nb, h->stream));
+    RNPX_TRY(launch_zero_u32(sync, 2 * nb, ps));
     LuPanelArgs pa;
     pa.mp = mp;
     pa.jb = jb;
@@ -62,32 +78,45 @@ int getrf_dev(Handle* h, int m, int n, T* A, long lda, int* piv_host, int* info)
     pa.rowc = rowc;
     pa.sync = sync;
     pa.abortf = (unsigned*)h->flag.p;
-    RNPX_TRY(launch_lu_panel<T>(pa, lu_panel_rows(mp, maxg, S<T>::cplx ? 1 : 0), h->stream));
-    if (j > 0) RNPX_TRY(launch_lu_laswp<T>(j, jb, prel + j, A, lda, 0, j, h->stream));
-    if (j + jb < n) {
-      RNPX_TRY(launch_lu_laswp<T>(j, jb, prel + j, A, lda, j + jb, n - j - jb, h->stream));
-      RNPX_TRY(launch_lu_trsm<T>(jb, A + (size_t)j + (size_t)j * lda, lda, A + (size_t)j + (size_t)(j + jb) * lda, lda,
-                                 n - j - jb, h->stream));
-      if (j + jb < m) {
-        T* L21 = A + (size_t)(j + jb) + (size_t)j * lda;
-        T* U12 = A + (size_t)j + (size_t)(j + jb) * lda;
-        T* A22 = A + (size_t)(j + jb) + (size_t)(j + jb) * lda;
-        bool done = false;
-        if constexpr (!S<T>::cplx) {
-          if (vendor_update_enabled()) {
-            const int s = vendor_dgemm_nn(h, h->stream, m - j - jb, n - j - jb, jb, -1.0, L21, lda, U12, lda, 1.0,
-                                          A22, lda);
-            if (s) return s;
-            done = true;
-          }
-        }
-        if (!done) {
-          const T one = S<T>::one(), mone = S<T>::zero() - S<T>::one();
-          RNPX_TRY(gemm<T>('N', 'N', m - j - jb, n - j - jb, jb, mone, L21, lda, U12, lda, one, A22, lda, 1, nullptr,
-                           h->stream));
-        }
+    RNPX_TRY(launch_lu_panel<T>(pa, lu_panel_rows(mp, maxg, S<T>::cplx ? 1 : 0), ps));
+    if (look) RNPX_TRY(hipEventRecord(h->ev_pan[pi & 1], ps));
+    return RNPX_OK;
+  };
+  // Panel pi's interchanges, U12 and A22 update for columns [c0, c0+nc).
+  auto right = [&](int pi, int c0, int nc, hipStream_t s) -> int {
+    if (nc <= 0) return RNPX_OK;
+    const int j = pi * nb, jb = std::min(nb, k - j);
+    RNPX_TRY(launch_lu_laswp<T>(j, jb, prel + j, A, lda, c0, nc, s));
+    RNPX_TRY(launch_lu_trsm<T>(jb, A + (size_t)j + (size_t)j * lda, lda, A + (size_t)j + (size_t)c0 * lda, lda, nc, s));
+    if (j + jb < m) {
+      T* L21 = A + (size_t)(j + jb) + (size_t)j * lda;
+      T* U12 = A + (size_t)j + (size_t)c0 * lda;
+      T* A22 = A + (size_t)(j + jb) + (size_t)c0 * lda;
+      if constexpr (!S<T>::cplx) {
+        if (vendor_update_enabled()) return vendor_dgemm_nn(h, s, m - j - jb, nc, jb, -1.0, L21, lda, U12, lda, 1.0, A22, lda);
       }
+      const T one = S<T>::one(), mone = S<T>::zero() - S<T>::one();
+      RNPX_TRY(gemm<T>('N', 'N', m - j - jb, nc, jb, mone, L21, lda, U12, lda, one, A22, lda, 1, nullptr, s));
     }
+    return RNPX_OK;
+  };
+  int e = panel(0);
+  if (e) return e;
+  for (int pi = 0; pi < npan; ++pi) {
+    const int j = pi * nb, jb = std::min(nb, k - j);
+    const int c1 = j + jb;  // first column right of the panel
+    if (look) RNPX_TRY(hipStreamWaitEvent(st, h->ev_pan[pi & 1], 0));
+    int cr = c1;  // first column of the handle stream's part
+    if (pi + 1 < npan) {
+      const int jb2 = std::min(nb, k - c1);
+      if (pi > 0) RNPX_TRY(hipStreamWaitEvent(ps, h->ev_rest, 0));
+      if ((e = right(pi, c1, jb2, ps))) return e;
+      if ((e = panel(pi + 1))) return e;
+      cr = c1 + jb2;
+    }
+    if (j > 0) RNPX_TRY(launch_lu_laswp<T>(j, jb, prel + j, A, lda, 0, j, st));
+    if ((e = right(pi, cr, n - cr, st))) return e;
+    if (look) RNPX_TRY(hipEventRecord(h->ev_rest, st));
   }
   std::vector<int> z(npan), pr(k);
   RNPX_TRY(hipMemcpyAsync(z.data(), zl, sizeof(int) * npan, hipMemcpyDeviceToHost, h->stream));
