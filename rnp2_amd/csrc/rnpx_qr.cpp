@@ -617,7 +617,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   } cap_reset{h};
   static const int gate_rows = [] {
     const char* e = getenv("RNPX_GATE_ROWS");
-    return e ? atoi(e) : 6144;
+    return e ? atoi(e) : 4096;
   }();
   const GemmWork panel_work = {ps, &h->wbuf2, &h->zbuf2, &h->part2, false};
   const GemmWork upd_work = {us, &h->wbuf, &h->zbuf, &h->part, true};

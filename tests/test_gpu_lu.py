@@ -34,7 +34,7 @@ def check_lu(a, tag):
 
 
 SHAPES = [(5, 5), (64, 64), (65, 65), (300, 200), (200, 300), (1, 1), (1, 9), (9, 1), (1000, 1000), (2048, 2048),
-          (4096, 300)]
+          (4096, 300), (513, 1200)]
 
 
 @pytest.mark.parametrize("cplx", [False, True])
