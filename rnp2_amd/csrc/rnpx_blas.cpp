@@ -35,6 +35,18 @@ int vendor_dgemm(Handle* h, hipStream_t st, char ta, char tb, int m, int n, int 
   if (!slot) {
     rocblas_handle rh = nullptr;
     if (rocblas_create_handle(&rh) != rocblas_status_success) return RNPX_ERR_HIP;
+    // Atomics would let rocBLAS sum split products in arrival order; without
+    // them a factorization is reproducible bit for bit (RNPX_BLAS_ATOMICS=1
+    // allows them).
+    static const bool atomics = [] {
+      const char* e = getenv("RNPX_BLAS_ATOMICS");
+      return e && atoi(e) == 1;
+    }();
+    if (rocblas_set_atomics_mode(rh, atomics ? rocblas_atomics_allowed : rocblas_atomics_not_allowed) !=
+        rocblas_status_success) {
+      rocblas_destroy_handle(rh);
+      return RNPX_ERR_HIP;
+    }
     slot = rh;
   }
   rocblas_handle rh = (rocblas_handle)slot;

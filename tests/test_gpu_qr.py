@@ -233,3 +233,29 @@ def test_profile_families():
     assert h.profile_read("update")["launches"] == upd["launches"]
     h.profile(False)
     assert h.profile_read("update")["launches"] == 0
+
+
+@pytest.mark.parametrize("shape", [(4096, 4096), (8192, 2048)])
+def test_factor_deterministic(shape):
+    """Every reduction on the path sums in a fixed order, so repeated
+    factorizations of one matrix on one handle (look-ahead streams, split-K
+    products, the block kernel's grid reductions) agree bit for bit."""
+    import torch
+    m, n = shape
+    print(f"[case] deterministic {shape}", flush=True)
+    h = R.Handle()
+    a0 = R.colmajor_empty(m, n, torch.float64)
+    h.larnv(2, [5, 3, 2, 9], a0)
+    h.synchronize()  # larnv runs on the handle's stream, the copies below on torch's
+    a = R.colmajor_empty(m, n, torch.float64)
+    tau = torch.empty(min(m, n), dtype=torch.float64, device="cuda")
+    outs = []
+    for _ in range(3):
+        a.copy_(a0)
+        torch.cuda.synchronize()
+        h.geqrf(a, tau)
+        h.synchronize()
+        outs.append((a.clone(), tau.clone()))
+    for af, tf in outs[1:]:
+        assert torch.equal(af, outs[0][0]) and torch.equal(tf, outs[0][1])
+    assert bool(torch.isfinite(outs[0][1]).all())
