@@ -187,7 +187,7 @@ __device__ __forceinline__ T* as_global(T* p) {
 #define RNPX_CQR_GLOBAL(a)                                                                        \
   do {                                                                                            \
     (a).A = as_global((a).A); (a).tau = as_global((a).tau); (a).V = as_global((a).V);              \
-    (a).T = as_global((a).T); (a).part = as_global((a).part); (a).gpart = as_global((a).gpart);    \
+    (a).T = as_global((a).T); (a).part = as_global((a).part);                                      \
     (a).gsum = as_global((a).gsum); (a).scratch = as_global((a).scratch);                         \
     (a).sync = as_global((a).sync); (a).abortf = as_global((a).abortf); (a).fail = as_global((a).fail); \
   } while (0)
@@ -228,8 +228,7 @@ struct CqrBlockArgs {
   double* tau;          // out: 128 scalars
   double* V; long ldv;  // out: clean V, mp x 128
   double* T; long ldt;  // out: 128 x 128 T
-  double* part;         // scratch: G * 4096
-  double* gpart;        // scratch: 16 * 4096 (group sums)
+  double* part;         // scratch: G * 4096 (each workgroup's partial products)
   double* gsum;         // scratch: 6 * 4096 (reduced products of each pass)
   double* scratch;      // scratch: kCqrBlockScratch doubles
   unsigned* sync;       // kCqrBlockSyncWords words, zero at launch

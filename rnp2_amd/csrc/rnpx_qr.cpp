@@ -464,7 +464,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   static_assert(2 * kCqrSyncWords >= kCqrBlockSyncWords, "a block uses its two panels' sync words");
   if (use_block) {
     RNPX_TRY(h->cqr_part.ensure(sizeof(double) * (size_t)cqr_block_groups(m, std::min(256, h->num_cus)) * 4096, h));
-    RNPX_TRY(h->cqr_gsum.ensure(sizeof(double) * (6 + 16) * 4096, h));
+    RNPX_TRY(h->cqr_gsum.ensure(sizeof(double) * 6 * 4096, h));
     RNPX_TRY(h->cqr_scratch.ensure(sizeof(double) * kCqrBlockScratch, h));
   }
   {
@@ -511,7 +511,6 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
         ba.ldt = ldt;
         ba.part = (double*)h->cqr_part.p;
         ba.gsum = (double*)h->cqr_gsum.p;
-        ba.gpart = ba.gsum + 6 * 4096;
         ba.scratch = (double*)h->cqr_scratch.p;
         ba.sync = (unsigned*)h->sync.p + 1 + 2 * npan + (size_t)kCqrSyncWords * p1;  // panels p1 and p1+1
         ba.abortf = (unsigned*)h->flag.p;
