@@ -410,8 +410,10 @@ hipError_t launch_lu_axpy(int n, const T* x, T* y, hipStream_t st) {
 }
 
 int lu_panel_rows(int mp, int max_groups, int cplx) {
-  // The smallest row block that fits (measured at 8192^2: 64 rows 77.9 ms,
-  // 128 rows 78.1 ms, 256 rows 87.4 ms); RNPX_LU_PANEL_RW overrides.
+  // 128-row blocks, or 256 when 128 would need too many workgroups (measured
+  // at 8192^2 with the panels overlapped by the trailing update: 64 rows
+  // 72.4 ms, 128 rows 69.9 ms; fewer workgroups contend less with the
+  // update's GEMM for CUs); RNPX_LU_PANEL_RW overrides.
   static const int pref = [] {
     const char* e = getenv("RNPX_LU_PANEL_RW");
     return e ? atoi(e) : 0;
@@ -420,7 +422,7 @@ int lu_panel_rows(int mp, int max_groups, int cplx) {
     const int G = (mp + pref - 1) / pref;
     if (G <= max_groups) return pref;
   }
-  for (int rw : {64, 128, 256}) {
+  for (int rw : {128, 256}) {
     if (cplx && rw > 128) break;
     const int G = (mp + rw - 1) / rw;
     if (G <= max_groups) return rw;
