@@ -1,0 +1,213 @@
+"""Correctness at the benchmarked sizes, on the GPU.
+
+Factor at 8192 x 8192 (real), 32768 x 4096 (real, tall) and 4096 x 4096
+(complex), on N(0,1) input generated on the device:
+  * ||A - Q R||_F / (n eps ||A||_F) <= 10 and ||Q^H Q - I||_F / (n eps) <= 10,
+    with Q from this library's GenerateQ and the products in FP64 on the
+    device;
+  * Householder QR is column-causal (column j of R, V and tau depends only on
+    columns 0..j of A), so the leading 256 columns of R, V and tau are
+    compared elementwise with the oracle's Factor of A(:, :256), column-scaled
+    error <= 1e-11.
+The mixed-precision least-squares configuration at its full size
+(16384 x 2048, condition 1e6) must reach eta <= 1e-13, and the request server
+must serve requests up to 8192 rows at eta <= 1e-13 with the same bits alone
+and in the mix.  The trailing-update product C -= V Z (FP64 N,N with C in the
+accumulators) is checked against an FP64 product on the host.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import rnp2_amd as R
+from metrics import rel_err_colscaled
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(np.float64).eps
+LEAD = 256
+
+
+def _device_factor_checks(m, n, cplx, iseed):
+    import torch
+    dt = torch.complex128 if cplx else torch.float64
+    print(f"[case] headline Factor {m}x{n} {'z' if cplx else 'd'} iseed={iseed}", flush=True)
+    h = R.Handle()
+    a0 = R.colmajor_empty(m, n, dt)
+    h.larnv(O.NORMAL01, iseed, a0)
+    h.synchronize()
+    a = R.colmajor_empty(m, n, dt)
+    a.copy_(a0)
+    tau = torch.empty(min(m, n), dtype=dt, device="cuda")
+    torch.cuda.synchronize()
+    h.geqrf(a, tau)
+    h.synchronize()
+    q = R.colmajor_empty(m, n, dt)
+    q.copy_(a)
+    torch.cuda.synchronize()
+    h.orgqr(q, tau)
+    h.synchronize()
+    r = torch.triu(a[:n, :])
+    res = float(torch.linalg.norm(a0 - q @ r) / (n * EPS * torch.linalg.norm(a0)))
+    eye = torch.eye(n, dtype=dt, device="cuda")
+    orth = float(torch.linalg.norm(q.mH @ q - eye) / (n * EPS))
+    del q, r, eye
+    # leading columns against the oracle
+    a_lead = np.asfortranarray(a0[:, :LEAD].cpu().numpy())
+    af_o, tau_o = O.factor(a_lead)
+    af_g = a[:, :LEAD].cpu().numpy()
+    tau_g = tau[:LEAD].cpu().numpy()
+    e_r = rel_err_colscaled(np.triu(af_g[:LEAD]), np.triu(af_o[:LEAD]))
+    e_v = float(np.max(np.abs(np.tril(af_g, -1) - np.tril(af_o, -1))))
+    e_t = float(np.max(np.abs(tau_g - tau_o)))
+    out = {"res": res, "orth": orth, "R_lead": e_r, "V_lead": e_v, "tau_lead": e_t}
+    print(f"        {out}", flush=True)
+    assert np.isfinite(res) and res <= 10 and orth <= 10, out
+    assert e_r <= 1e-11 and e_v <= 1e-11 and e_t <= 1e-11, out
+
+
+def test_factor_8192_square():
+    _device_factor_checks(8192, 8192, False, [1, 2, 3, 5])
+
+
+def test_factor_32768x4096_tall():
+    _device_factor_checks(32768, 4096, False, [1, 2, 3, 5])
+
+
+def test_factor_4096_complex():
+    _device_factor_checks(4096, 4096, True, [1, 2, 3, 5])
+
+
+def _q_of(torch, h, rows, cols, sd):
+    g = R.colmajor_empty(rows, cols, torch.float64)
+    h.larnv(O.NORMAL01, [sd % 4096, 5, 7, 11], g)
+    tau = torch.empty(cols, dtype=torch.float64, device="cuda")
+    h.geqrf(g, tau)
+    h.orgqr(g, tau)
+    return g
+
+
+def conditioned_dev(torch, h, m, n, cond, seed):
+    """A = U diag(s) V^T on the device, U and V orthonormal (this library's
+    QR of Gaussian matrices), s log-spaced from 1 to 1/cond."""
+    u = _q_of(torch, h, m, n, seed)
+    v = _q_of(torch, h, n, n, seed + 1)
+    h.synchronize()
+    s = torch.logspace(0, -np.log10(cond), n, dtype=torch.float64, device="cuda")
+    a = R.colmajor_empty(m, n, torch.float64)
+    a.copy_((u * s) @ v.t())
+    torch.cuda.synchronize()
+    return a
+
+
+def eta_dev(torch, a, b, x):
+    """eta = ||A^T r|| / (||A||_F (||A||_F ||x|| + ||b||)), worst column, in FP64 on the device."""
+    r = b - a @ x
+    an = torch.linalg.norm(a)
+    num = torch.linalg.norm(a.t() @ r, dim=0)
+    den = an * (an * torch.linalg.norm(x, dim=0) + torch.linalg.norm(b, dim=0))
+    return float((num / den).max())
+
+
+def test_mixed_ls_full_size():
+    """Config 4 at its full size: 16384 x 2048, s log-spaced from 1 to 1e-6,
+    b = A x_true + 1e-8 N(0,1)."""
+    import torch
+    m, n, cond = 16384, 2048, 1e6
+    print(f"[case] mixed LS {m}x{n} cond={cond:g}", flush=True)
+    h = R.Handle()
+    a = conditioned_dev(torch, h, m, n, cond, 21)
+    gen = torch.Generator("cuda").manual_seed(1)
+    xt = torch.randn(n, 1, dtype=torch.float64, device="cuda", generator=gen)
+    b = R.colmajor_empty(m, 1, torch.float64)
+    b.copy_(a @ xt + 1e-8 * torch.randn(m, 1, dtype=torch.float64, device="cuda", generator=gen))
+    x = R.colmajor_empty(n, 1, torch.float64)
+    torch.cuda.synchronize()
+    st, iters, be = h.gels_mixed(a, b, x, max_iter=30, tol=1e-13)
+    h.synchronize()
+    e_mixed = eta_dev(torch, a, b, x)
+    # the plain FP64 path on the same data
+    af = R.colmajor_empty(m, n, torch.float64)
+    af.copy_(a)
+    bf = R.colmajor_empty(m, 1, torch.float64)
+    bf.copy_(b)
+    tau = torch.empty(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    h.gels(af, tau, bf)
+    h.synchronize()
+    x64 = bf[:n].clone()
+    e_plain = eta_dev(torch, a, b, x64)
+    diff = float((x - x64).abs().max() / x64.abs().max())
+    print(f"        status={st} iters={iters} eta(dev)={be:.2e} eta(mixed)={e_mixed:.2e} eta(plain)={e_plain:.2e} "
+          f"|x - x_fp64|/|x_fp64|={diff:.2e}", flush=True)
+    assert st == 0 and be <= 1e-13 and e_mixed <= 1e-13 and e_plain <= 1e-13
+    # both are backward stable solutions of a kappa = 1e6 problem
+    assert diff <= 1e-6
+
+
+def test_server_up_to_8192_rows():
+    """Config 5's size range: requests up to 8192 rows (n in [m/8, m/2],
+    1-16 right-hand sides, condition up to 1e8) served concurrently, each at
+    eta <= 1e-13, and the same bits when served alone."""
+    import torch
+    shapes = [(8192, 4096, 16, 1e8), (8192, 1100, 1, 1e2), (6000, 2500, 8, 1e6), (4000, 600, 3, 1e4),
+              (2000, 300, 1, 1e7), (700, 200, 5, 1e3), (130, 40, 2, 1e5)]
+    print(f"[case] server large requests {[(s[0], s[1]) for s in shapes]}", flush=True)
+    h = R.Handle()
+    rng = np.random.default_rng(8)
+    reqs = []
+    for i, (m, n, q, cond) in enumerate(shapes):
+        a = conditioned_dev(torch, h, m, n, cond, 300 + 3 * i)
+        xt = torch.from_numpy(rng.standard_normal((n, q))).cuda()
+        b = a @ xt + 1e-8 * torch.from_numpy(rng.standard_normal((m, q))).cuda()
+        reqs.append((a.cpu().numpy(), b.cpu().numpy(), cond))
+    del h
+    srv = R.Server(hbm_budget=64 << 30, workers=4, mode=1, max_iter=30, tol=1e-13)
+    ids = [srv.submit(a, b)[1] for a, b, _ in reqs]
+    mix = [srv.poll(i) for i in ids]
+    srv.close()
+    for (a, b, cond), (x, info) in zip(reqs, mix):
+        assert info.status == 0, (a.shape, info.status)
+        r = b - a @ x
+        an = np.linalg.norm(a)
+        e = float(np.max(np.linalg.norm(a.T @ r, axis=0) /
+                         (an * (an * np.linalg.norm(x, axis=0) + np.linalg.norm(b, axis=0)))))
+        print(f"        {a.shape} nrhs={b.shape[1]} cond={cond:g} path={R.PATHS[info.path]} eta={e:.2e}", flush=True)
+        assert e <= 1e-13, (a.shape, cond, e)
+    srv1 = R.Server(hbm_budget=64 << 30, workers=1, mode=1, max_iter=30, tol=1e-13)
+    for k in (0, 2):
+        a, b, _ = reqs[k]
+        x1, _ = srv1.poll(srv1.submit(a, b)[1])
+        assert np.array_equal(x1, mix[k][0]), k
+    srv1.close()
+
+
+@pytest.mark.parametrize("mc,nc,k", [(8064, 7936, 128), (4000, 3000, 128), (777, 513, 64), (8192, 128, 128),
+                                     (129, 4097, 128), (2048, 2048, 256), (33, 17, 5)])
+def test_update_product_nn(mc, nc, k):
+    """C -= V Z in FP64 (the trailing update's N,N product, C held in the
+    accumulators) against an exact-order-independent bound: every entry
+    within k * eps * (|V| |Z|)_ij + eps |C_ij| of the host FP64 product."""
+    import torch
+    print(f"[case] update product {mc}x{nc} k={k}", flush=True)
+    rng = np.random.default_rng(mc + nc + k)
+    v = rng.standard_normal((mc, k))
+    z = rng.standard_normal((k, nc))
+    c = rng.standard_normal((mc, nc))
+    ref = c - v @ z
+    bound = (k + 2) * EPS * (np.abs(v) @ np.abs(z) + np.abs(c))
+    h = R.Handle()
+
+    def dev(x):
+        t = R.colmajor_empty(x.shape[0], x.shape[1], torch.float64)
+        t.copy_(torch.from_numpy(np.ascontiguousarray(x)))
+        return t
+
+    V, Z, C = dev(v), dev(z), dev(c)
+    torch.cuda.synchronize()
+    h.gemm("N", "N", -1.0, V, Z, 1.0, C)
+    h.synchronize()
+    out = C.cpu().numpy()
+    err = np.abs(out - ref)
+    print(f"        max err / bound = {float(np.max(err / bound)):.3f}", flush=True)
+    assert np.all(err <= bound)
