@@ -298,6 +298,17 @@ int rnpx_zgeqrf_batched_plan(rnpx_handle h, int count, const rnpx_qr_desc *desc,
                              rnpx_batched_plan *plan);
 int rnpx_batched_plan_run(rnpx_batched_plan plan);
 int rnpx_batched_plan_destroy(rnpx_batched_plan plan);
+/* rnpx_dgeqrf_batched without any device allocation (no counterpart in the
+ * reference; for callers that cannot allocate: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as rnpx_dgeqrf_batched; both tiers run on the handle's stream.  The
+ * descriptor table, problem order and queue counters live in a static device
+ * array sized for at most 16384 problems: a larger count returns
+ * RNPX_ERR_UNSUPPORTED before anything is touched.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (different streams or
+ * threads) race; calls on one stream are ordered by it. */
+int rnpx_dgeqrf_batched_noalloc(rnpx_handle h, int count, const rnpx_qr_desc *desc, double *A,
+                                double *tau, double *B);
 
 /* Least squares min ||A X - B||, m >= n, device pointers: Factor, then
  * MultQ("L","C") on B, then back substitution with R (the composition of

@@ -130,6 +130,7 @@ SIGNATURES = {
                                   ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
     "rnpx_dls_berr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p, _sz, _p, _sz, ctypes.POINTER(ctypes.c_double)]),
     "rnpx_dgeqrf_batched": (_i, [_p, _i, _p, _p, _p, _p]),
+    "rnpx_dgeqrf_batched_noalloc": (_i, [_p, _i, _p, _p, _p, _p]),
     "rnpx_zgeqrf_batched": (_i, [_p, _i, _p, _p, _p, _p]),
     "rnpx_dgeqrf_batched_plan": (_i, [_p, _i, _p, _p, _p, _p, ctypes.POINTER(_p)]),
     "rnpx_zgeqrf_batched_plan": (_i, [_p, _i, _p, _p, _p, _p, ctypes.POINTER(_p)]),
@@ -682,6 +683,14 @@ class Handle:
         fn = "rnpx_dgeqrf_batched" if _tpre(A) == "d" else "rnpx_zgeqrf_batched"
         _check(getattr(lib(), fn)(self.h, len(desc), desc.ctypes.data_as(_p), _dptr(A), _dptr(tau),
                                   None if B is None else _dptr(B)), "geqrf_batched")
+
+    def geqrf_batched_noalloc(self, desc, A, tau, B=None):
+        """rnpx_dgeqrf_batched_noalloc: the batched QR / LS with no device
+        allocation (scratch in a static device array, at most 16384
+        problems); returns the status code."""
+        desc = np.ascontiguousarray(desc, dtype=QR_DESC)
+        return lib().rnpx_dgeqrf_batched_noalloc(self.h, len(desc), desc.ctypes.data_as(_p), _dptr(A), _dptr(tau),
+                                                 None if B is None else _dptr(B))
 
     def batched_plan(self, desc, A, tau, B=None):
         desc = np.ascontiguousarray(desc, dtype=QR_DESC)

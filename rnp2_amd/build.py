@@ -26,6 +26,7 @@ SOURCES = [
     "rnpx_aux.hip",
     "rnpx_batched.hip",
     "rnpx_mixed.hip",
+    "rnpx_noalloc.hip",
     "rnpx_qr.cpp", "rnpx_blas.cpp", "rnpx_qlrq.cpp", "rnpx_hess.cpp", "rnpx_lu_driver.cpp",
     "rnpx_lq.cpp",
     "rnpx_ls.cpp",

@@ -108,6 +108,19 @@ int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda,
 int ls_berr_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb,
                 const double* X, long ldx, double* berr);
 template <typename T> int geqrf_batched(Handle* h, int count, const rnpx_qr_desc* desc, T* A, T* tau, T* B);
+// Batched set-up shared by the batched entry points: problems of at most
+// kBatchLdsBytes run from LDS in one-wave teams (the small tier), the rest in
+// place in 512-thread teams (the large tier); each tier's list is sorted by
+// cost, largest first.  Returns -3 on a bad descriptor.
+constexpr int kBatchLdsBytes = 40 * 1024;  // per one-wave team: four teams per CU
+struct BatchSetup {
+  std::vector<int> small, large;
+};
+template <typename T> int batch_setup(int count, const rnpx_qr_desc* desc, BatchSetup* out);
+// Allocation-free batched entry point's set-up: resolves this device's copy of
+// the static scratch array once per device (called by create_handle with the
+// device current).
+int noalloc_setup(int device);
 struct BatchedPlan;
 template <typename T>
 int geqrf_batched_plan(Handle* h, int count, const rnpx_qr_desc* desc, T* A, T* tau, T* B, BatchedPlan** out);

@@ -205,14 +205,8 @@ template int gels_dev<zd>(Handle*, int, int, int, zd*, long, zd*, zd*, long);
 // ---------------------------------------------------------------------------
 // Batched
 // ---------------------------------------------------------------------------
-constexpr int kBatchLdsBytes = 40 * 1024;  // per one-wave team: four teams per CU
-
-struct BatchSetup {
-  std::vector<int> small, large;  // cost-sorted problem indices per tier
-};
-
 template <typename T>
-static int batch_setup(int count, const rnpx_qr_desc* desc, BatchSetup* out) {
+int batch_setup(int count, const rnpx_qr_desc* desc, BatchSetup* out) {
   std::vector<double> cost(count);
   for (int i = 0; i < count; ++i) {
     const rnpx_qr_desc& d = desc[i];
@@ -358,6 +352,8 @@ int batched_plan_destroy(BatchedPlan* p) {
   return RNPX_OK;
 }
 
+template int batch_setup<double>(int, const rnpx_qr_desc*, BatchSetup*);
+template int batch_setup<zd>(int, const rnpx_qr_desc*, BatchSetup*);
 template int geqrf_batched<double>(Handle*, int, const rnpx_qr_desc*, double*, double*, double*);
 template int geqrf_batched<zd>(Handle*, int, const rnpx_qr_desc*, zd*, zd*, zd*);
 template int geqrf_batched_plan<double>(Handle*, int, const rnpx_qr_desc*, double*, double*, double*, BatchedPlan**);

@@ -188,6 +188,10 @@ int create_handle(Handle** out, size_t budget) {
     delete h;
     return hip_status(e);
   }
+  // Library set-up for the allocation-free batched entry point: this
+  // device's address of its static scratch (once per device; a failure is
+  // recorded and reported by that entry point only).
+  (void)noalloc_setup(h->device);
   *out = h;
   return RNPX_OK;
 }
