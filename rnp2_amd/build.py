@@ -23,11 +23,12 @@ SOURCES = [
     "rnpx_cqr.hip", "rnpx_trsm.hip", "rnpx_lu.hip",
     "rnpx_tfactor.hip",
     "rnpx_gemm.hip",
+    "rnpx_dgemm.hip",
     "rnpx_aux.hip",
     "rnpx_batched.hip",
     "rnpx_mixed.hip",
     "rnpx_noalloc.hip",
-    "rnpx_qr.cpp", "rnpx_blas.cpp", "rnpx_qlrq.cpp", "rnpx_hess.cpp", "rnpx_lu_driver.cpp",
+    "rnpx_qr.cpp", "rnpx_qlrq.cpp", "rnpx_hess.cpp", "rnpx_lu_driver.cpp",
     "rnpx_lq.cpp",
     "rnpx_ls.cpp",
     "rnpx_serve.cpp",
@@ -77,7 +78,7 @@ def build(verbose=False, jobs=None):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocblas"]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -76,7 +76,8 @@ template <typename T>
 int dropin_geqrf(size_t m, size_t n, T* a, size_t lda, T* tau, size_t* lwork) {
   if (lda < m || lda == 0) return -4;
   if (!lwork) return -6;
-  if (!(*lwork == 0 || *lwork >= m)) return -6;
+  // 0 < *lwork < m is accepted: the reference asserts *lwork >= m (print and
+  // continue) while its own workspace query returns nb n, and factors anyway.
   const size_t k = m < n ? m : n;
   if (k == 0) return RNPX_OK;
   if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
@@ -155,7 +156,8 @@ template <typename T>
 int dropin_gelqf(size_t m, size_t n, T* a, size_t lda, T* tau, size_t* lwork) {
   if (lda < m || lda == 0) return -4;
   if (!lwork) return -6;
-  if (!(*lwork == 0 || *lwork >= m)) return -6;
+  // 0 < *lwork < m is accepted: the reference asserts *lwork >= m (print and
+  // continue) while its own workspace query returns nb n, and factors anyway.
   const size_t k = m < n ? m : n;
   if (k == 0) return RNPX_OK;
   if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
@@ -238,7 +240,7 @@ template <typename T>
 int dropin_gexxf(Fam f, size_t m, size_t n, T* a, size_t lda, T* tau, size_t* lwork) {
   if (lda < m || lda == 0) return -4;
   if (!lwork) return -6;
-  if (f == Fam::RQ && !(*lwork == 0 || *lwork >= m)) return -6;
+  // RQ: 0 < *lwork < m accepted as in QR (the reference prints and continues)
   const size_t k = m < n ? m : n;
   if (k == 0) return RNPX_OK;
   if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;

@@ -291,6 +291,17 @@ static void launch_gemm_t(int M, int N, int K, T alpha, const T* A, long lda, co
   }
 }
 
+// RNPX_DGEMM (read once): -1 keeps every product on gemm_kernel (a tuning
+// and comparison knob); 0 (default) lets dgemm_mfma pick its tile; 1..3 force
+// one of its tiles.
+static int dgemm_cfg_override() {
+  static const int v = [] {
+    const char* e = getenv("RNPX_DGEMM");
+    return e ? atoi(e) : 0;
+  }();
+  return v;
+}
+
 template <typename T>
 hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta,
                 T* C, long ldc, int ksplit, T* part, hipStream_t st) {
@@ -318,6 +329,19 @@ hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long
   }
   if (K == 0) kslice = 0;
   T* pp = (ns > 1) ? part : nullptr;
+  bool done = false;
+  if constexpr (!S<T>::cplx) {
+    // Real op(A) N/T/C times op(B) N: the double-buffered 128 x 128 kernel.
+    const int cfg = dgemm_cfg_override();
+    if (cfg >= 0 && tb == 'N') {
+      const hipError_t e = dgemm_mfma(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, pp, cfg, st);
+      if (e != hipErrorNotSupported) {
+        if (e != hipSuccess) return e;
+        done = true;
+      }
+    }
+  }
+  if (!done) {
 #define RNPX_GEMM_CASE(A_, B_)                                                                          \
   if (ta == A_ && tb == B_) {                                                                            \
     launch_gemm_t<T, A_, B_>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, pp, st);          \
@@ -326,6 +350,7 @@ hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long
   else RNPX_GEMM_CASE('T', 'N') else RNPX_GEMM_CASE('T', 'T') else RNPX_GEMM_CASE('T', 'C')
   else RNPX_GEMM_CASE('C', 'N') else RNPX_GEMM_CASE('C', 'T') else RNPX_GEMM_CASE('C', 'C')
 #undef RNPX_GEMM_CASE
+  }
   if (ns > 1) {
     const long tot = (long)M * N;
     hipLaunchKernelGGL((gemm_reduce_kernel<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, M, N, ns,

@@ -65,11 +65,10 @@ struct Handle {
             &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
             &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf};
   }
+  hipError_t sync_all_streams();       // the handle's stream, panel stream and partition streams
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
   int update_prec = 0;                 // 1: trailing-update products on FP32 MFMA
-  void* blas = nullptr;                // rocBLAS handle of the vendor update product (lazy)
-  void* blas_side = nullptr;           // a second one for the panel stream (a handle's workspace is not shared)
   bool prof = false;
   unsigned prof_mask = ~0u;  // kernel ids (bit per KernelId) that are timed
   std::vector<ProfRec> recs;
@@ -86,12 +85,6 @@ struct Handle {
 };
 
 int hip_status(hipError_t e);
-bool vendor_update_enabled();
-int vendor_dgemm_nn(Handle* h, hipStream_t st, int m, int n, int k, double alpha, const double* A, long lda,
-                    const double* B, long ldb, double beta, double* C, long ldc);
-int vendor_dgemm(Handle* h, hipStream_t st, char ta, char tb, int m, int n, int k, double alpha, const double* A,
-                 long lda, const double* B, long ldb, double beta, double* C, long ldc);
-void vendor_release(Handle* h);
 int create_handle(Handle** out, size_t budget);
 Handle* default_handle(int* status);
 int sync_handle(Handle* h);

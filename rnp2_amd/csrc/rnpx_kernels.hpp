@@ -14,8 +14,9 @@
 
 namespace rnpx {
 
-// Panel kernel: at most this many workgroups (each holds 256*RPT rows).
-constexpr int kMaxPanelGroups = 128;
+// Panel kernel: at most this many workgroups (each holds 256*RPT rows; one
+// 256-thread workgroup per CU at these register counts, all co-resident).
+constexpr int kMaxPanelGroups = 256;
 // Grid-barrier timeout: 2 s of the 100 MHz wall clock.
 constexpr uint64_t kBarrierTimeoutTicks = 200000000ull;
 
@@ -270,6 +271,16 @@ template <typename T>
 hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long lda,
                 const T* B, long ldb, T beta, T* C, long ldc, int ksplit, T* part,
                 hipStream_t st);
+
+// The real trailing-update product on FP64 MFMA with LDS double-buffered
+// 128 x 128 tiles (rnpx_dgemm.hip): op(A) 'N', 'T' or 'C', op(B) 'N'; ns > 1
+// writes the raw products of K slices of kslice (a multiple of 16) to part
+// (M x N each, leading dimension M); cfg 0 picks the tile (1: 128 x 128,
+// 2: 128 x 64, 3: 64 x 64).  hipErrorNotSupported for other operand forms.
+hipError_t dgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const double* A, long lda, const double* B,
+                      long ldb, double beta, double* C, long ldc, int kslice, int ns, double* part, int cfg,
+                      hipStream_t st);
+int dgemm_tile_cfg(int M, int N, int K, int ksplit, int num_cus);
 
 // Small helpers.
 template <typename T>

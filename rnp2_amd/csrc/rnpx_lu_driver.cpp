@@ -92,9 +92,6 @@ int getrf_dev(Handle* h, int m, int n, T* A, long lda, int* piv_host, int* info)
       T* L21 = A + (size_t)(j + jb) + (size_t)j * lda;
       T* U12 = A + (size_t)j + (size_t)c0 * lda;
       T* A22 = A + (size_t)(j + jb) + (size_t)c0 * lda;
-      if constexpr (!S<T>::cplx) {
-        if (vendor_update_enabled()) return vendor_dgemm_nn(h, s, m - j - jb, nc, jb, -1.0, L21, lda, U12, lda, 1.0, A22, lda);
-      }
       const T one = S<T>::one(), mone = S<T>::zero() - S<T>::one();
       RNPX_TRY(gemm<T>('N', 'N', m - j - jb, nc, jb, mone, L21, lda, U12, lda, one, A22, lda, 1, nullptr, s));
     }

@@ -359,6 +359,7 @@ hipError_t launch_panel(const PanelArgs<T>& a, int nb, int rpt, hipStream_t st) 
   if (a.nbp > nb || a.nbp < 1) return hipErrorInvalidValue;
   if constexpr (S<T>::cplx) {
     if (nb == 32 && rpt == 1) return launch_panel_t<T, 32, 1>(a, G, st);
+    if (nb == 32 && rpt == 2) return launch_panel_t<T, 32, 2>(a, G, st);
     if (nb == 64 && rpt == 1) return launch_panel_t<T, 64, 1>(a, G, st);
   } else {
     if (nb == 32 && rpt == 1) return launch_panel_t<T, 32, 1>(a, G, st);

@@ -19,7 +19,10 @@ hipError_t Buffer::ensure(size_t b, Handle* h) {
   if (b <= bytes) return hipSuccess;
   if (h && h->budget && h->held_bytes() - bytes + b > h->budget) return hipErrorOutOfMemory;
   if (p) {
-    hipError_t e = hipStreamSynchronize(h ? h->stream : nullptr);
+    // The buffer may be in use on any stream of the handle (the look-ahead's
+    // panel stream grows its GEMM scratch while earlier panel work is queued):
+    // every one of them is drained before the old allocation goes.
+    hipError_t e = h ? h->sync_all_streams() : hipDeviceSynchronize();
     if (e != hipSuccess) return e;
     hipFree(p);
     p = nullptr;
@@ -38,6 +41,15 @@ void Buffer::release() {
   if (p) hipFree(p);
   p = nullptr;
   bytes = 0;
+}
+
+hipError_t Handle::sync_all_streams() {
+  for (hipStream_t s : {stream, pstream, part_ps, part_us}) {
+    if (!s) continue;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 size_t Handle::held_bytes() const {
@@ -125,7 +137,6 @@ int Handle::ensure_partition(int cus) {
 
 Handle::~Handle() {
   if (stream) hipStreamSynchronize(stream);
-  vendor_release(this);
   if (batch && batch_free) batch_free(batch);
   for (auto& r : recs) {
     hipEventDestroy(r.a);
@@ -230,23 +241,26 @@ static int pick_nb(const Handle* h, int m, int n) {
 static int pick_rpt(const Handle* h, int mp, int nb, bool cplx) {
   int rpt = h->tune.panel_rpt;
   // Built register-resident configurations (no spills): real NB=32 with up
-  // to 4 rows per thread, NB=64 with 1; complex NB=32/64 with 1.
-  const int maxrpt = cplx ? 1 : (nb <= 32 ? 4 : 1);
+  // to 4 rows per thread, NB=64 with 1; complex NB=32 with up to 2, NB=64
+  // with 1.
+  const int maxrpt = nb <= 32 ? (cplx ? 2 : 4) : 1;
   if (rpt < 1 || rpt > maxrpt || (rpt & (rpt - 1))) {
     // default: keep the workgroup count near 16-32
     rpt = 1;
     while (rpt < maxrpt && (mp + 256 * rpt - 1) / (256 * rpt) > 32) rpt *= 2;
   }
-  while ((mp + 256 * rpt - 1) / (256 * rpt) > kMaxPanelGroups && rpt < maxrpt) rpt *= 2;
+  const int cap = std::min(kMaxPanelGroups, h->num_cus);
+  while ((mp + 256 * rpt - 1) / (256 * rpt) > cap && rpt < maxrpt) rpt *= 2;
   return rpt;
 }
 
 static int pick_ksplit(const Handle* h, int rows, int cols_tiles) {
   if (h->tune.ksplit > 0) return h->tune.ksplit;
-  // About 1024 workgroup tiles in all.  K slices of at least 512, or 128
-  // for the narrow products (one or two column tiles) that sit on the chain
-  // of panels, where latency matters more than the partial sums' traffic.
-  int want = 1024 / std::max(1, cols_tiles);
+  // About 512 workgroup tiles of 128 x 128 in all (two per CU; measured best
+  // for W = V^H C at 4096-8192 rows).  K slices of at least 512, or 128 for
+  // the narrow products (one or two column tiles) that sit on the chain of
+  // panels, where latency matters more than the partial sums' traffic.
+  int want = (cols_tiles <= 2 ? 1024 : 512) / std::max(1, cols_tiles);
   int maxs = std::max(1, rows / (cols_tiles <= 2 ? 128 : 512));
   return std::max(1, std::min(want, std::min(maxs, 32)));
 }
@@ -305,17 +319,7 @@ static int apply_block_dev(Handle* h, char side, char trans, int mc, int nc, int
     if constexpr (!S<T>::cplx) {
       if (f32) RNPX_TRY(gemm32('N', 'N', mc, nc, k, -1.0, V, ldv, Z, k, 1.0, C, ldc, 1, nullptr, st));
     }
-    if (!f32) {
-      bool done = false;
-      if constexpr (!S<T>::cplx) {
-        if (vendor_update_enabled()) {
-          const int vs = vendor_dgemm_nn(h, st, mc, nc, k, -1.0, V, ldv, Z, k, 1.0, C, ldc);
-          if (vs) return vs;
-          done = true;
-        }
-      }
-      if (!done) RNPX_TRY(gemm<T>('N', 'N', mc, nc, k, mone, V, ldv, Z, k, one, C, ldc, 1, nullptr, st));
-    }
+    if (!f32) RNPX_TRY(gemm<T>('N', 'N', mc, nc, k, mone, V, ldv, Z, k, one, C, ldc, 1, nullptr, st));
     if (gw->prof) h->prof_end(K_UPDATE, ev, fl, esz * (2.0 * mc * nc + (double)mc * k + (double)k * nc), st);
   } else {
     // W = C V (mc x k), Z = W op(T), C -= Z V^H
@@ -439,15 +443,23 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   const int k = std::min(m, n);
   if (k == 0) return RNPX_OK;
   hipStream_t st = h->stream;
-  const int nb = pick_nb<T>(h, m, n);
+  int nb = pick_nb<T>(h, m, n);
+  // Every panel workgroup must be resident at once (they meet at grid
+  // barriers): one 256-thread workgroup per CU at these register counts.
+  const int gcap = std::min(kMaxPanelGroups, h->num_cus);
+  int rpt0 = pick_rpt(h, m, nb, S<T>::cplx);
+  int G0 = (m + 256 * rpt0 - 1) / (256 * rpt0);
+  if (G0 > gcap && nb > 32) {
+    // Taller than a 64-column panel holds: 32-column panels, whose
+    // workgroups keep up to 4 (real) or 2 (complex) rows per thread.
+    nb = 32;
+    rpt0 = pick_rpt(h, m, nb, S<T>::cplx);
+    G0 = (m + 256 * rpt0 - 1) / (256 * rpt0);
+  }
+  if (G0 > gcap) return RNPX_ERR_UNSUPPORTED;
   const int nbo = (h->tune.nb == 0 || h->tune.nb == 128) ? 2 * nb : nb;
   const int npan = (k + nb - 1) / nb;     // inner panels
   const int nob = (k + nbo - 1) / nbo;    // outer blocks
-  const int rpt0 = pick_rpt(h, m, nb, S<T>::cplx);
-  const int G0 = (m + 256 * rpt0 - 1) / (256 * rpt0);
-  // Every panel workgroup must be resident at once (they meet at grid
-  // barriers): one 256-thread workgroup per CU at these register counts.
-  if (G0 > std::min(kMaxPanelGroups, h->num_cus)) return RNPX_ERR_UNSUPPORTED;
   const bool look = (h->tune.lookahead >= 0) && nob > 1;
   const long ldv = m, ldt = nbo;
   Buffer* vb[2] = {&h->vbuf, &h->vbuf2};
@@ -648,8 +660,11 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
       if constexpr (!S<T>::cplx) {
         // Once the panels are the longer chain, the rest of the update waits
         // until every workgroup of block b+1's kernel has a CU.
+        // Not under a panel stream shared with other handles (the server's):
+        // there block b+1's kernel can queue behind other requests' work and
+        // the gate would hold this stream for nothing.
         const int Gb = use_block ? cqr_block_groups(m - c0, std::min(256, h->num_cus)) : 0;
-        if (wn == 2 * nb && Gb > 0 && m - c0 <= gate_rows && c0 + wn < n)
+        if (wn == 2 * nb && Gb > 0 && m - c0 <= gate_rows && c0 + wn < n && !h->ext_pstream)
           RNPX_TRY(launch_cqr_gate((const unsigned*)h->sync.p + 1 + 2 * npan + (size_t)kCqrSyncWords * (c0 / nb), Gb,
                                    us));
       }

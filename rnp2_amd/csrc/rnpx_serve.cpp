@@ -242,6 +242,7 @@ static void worker_main(Server* s, int w) {
     int status = RNPX_OK;
     if (work.size() > 1 || work[0]->m <= kSmallM) {
       status = solve_small_batch(h, work, dA, dB, dT);
+      std::lock_guard<std::mutex> lk(s->mu);  // stats are read and written under the server lock
       s->stats.batches += 1;
     } else {
       status = solve_dense(s, h, work[0], dA, dA2, dB, dX);

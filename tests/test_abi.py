@@ -85,10 +85,14 @@ def test_argument_errors_lapack_style():
     sz = ctypes.c_size_t
     lw = sz(0)
     assert L.rnpx_dgeqrf(10, 5, None, 9, None, ctypes.byref(lw), None) == -4       # lda < m
+    # 0 < lwork < m is not an argument error: the reference asserts lwork >= m
+    # but prints, continues and factors (its own query returns nb n)
     lw = sz(5)
     a = np.zeros(50)
-    assert L.rnpx_dgeqrf(10, 5, a.ctypes.data_as(ctypes.c_void_p), 10, None, ctypes.byref(lw),
-                         a.ctypes.data_as(ctypes.c_void_p)) == -6                     # 0 < lwork < m
+    t = np.zeros(5)
+    st = L.rnpx_dgeqrf(10, 5, a.ctypes.data_as(ctypes.c_void_p), 10, t.ctypes.data_as(ctypes.c_void_p),
+                       ctypes.byref(lw), a.ctypes.data_as(ctypes.c_void_p))
+    assert st in (0, R.RNPX_ERR_NOGPU)
     assert L.rnpx_dormqr(b"X", b"N", 10, 5, 3, None, 10, None, None, 10, ctypes.byref(sz(0)), None) == -1
     assert L.rnpx_dormqr(b"L", b"Q", 10, 5, 3, None, 10, None, None, 10, ctypes.byref(sz(0)), None) == -2
     assert L.rnpx_dormqr(b"L", b"N", 10, 5, 11, None, 10, None, None, 10, ctypes.byref(sz(0)), None) == -5
@@ -263,8 +267,10 @@ def test_ql_rq_workspace_queries_match_reference():
     # argument errors, LAPACK style
     assert L.rnpx_dgeqlf(10, 5, None, 9, None, ctypes.byref(sz(0)), None) == -4
     a = np.zeros(50)
-    assert L.rnpx_dgerqf(10, 5, a.ctypes.data_as(ctypes.c_void_p), 10, None, ctypes.byref(sz(5)),
-                         a.ctypes.data_as(ctypes.c_void_p)) == -6                   # 0 < lwork < m
+    t = np.zeros(5)
+    st = L.rnpx_dgerqf(10, 5, a.ctypes.data_as(ctypes.c_void_p), 10, t.ctypes.data_as(ctypes.c_void_p),
+                       ctypes.byref(sz(5)), a.ctypes.data_as(ctypes.c_void_p))  # 0 < lwork < m: accepted
+    assert st in (0, R.RNPX_ERR_NOGPU)
     assert L.rnpx_dormql(b"L", b"N", 10, 5, 3, None, 2, None, None, 10, ctypes.byref(sz(0)), None) == -7
     assert L.rnpx_dorgql(10, 5, 6, None, 10, None, ctypes.byref(sz(0)), None) == -3   # k > n
     assert L.rnpx_dorgrq(5, 10, 6, None, 5, None, ctypes.byref(sz(0)), None) == -3   # k > m
