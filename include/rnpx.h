@@ -40,7 +40,8 @@ enum {
 	RNPX_ERR_NOMEM = -1002,     /* device allocation failed / over budget */
 	RNPX_ERR_SYNC = -1003,      /* a grid barrier timed out (results invalid) */
 	RNPX_ERR_UNSUPPORTED = -1004, /* shape/option outside what is built */
-	RNPX_ERR_NOTCONV = -1005    /* iterative refinement did not reach its target */
+	RNPX_ERR_NOTCONV = -1005,   /* iterative refinement did not reach its target */
+	RNPX_ERR_ARG = -1006        /* an invalid argument where -i would read as a result (rnpx_*getri) */
 };
 
 /* One problem of a batched call: A is m-by-n at A + a_off (leading dimension
@@ -207,6 +208,18 @@ int rnpx_dgetrs(const char *trans, size_t n, size_t nrhs, const double *a, size_
 /* LU::Solve<std::complex<double>> -- RNP/LA/LU.hpp:117-144 (zgetrs). */
 int rnpx_zgetrs(const char *trans, size_t n, size_t nrhs, const rnpx_zcomplex *a, size_t lda,
                 const size_t *ipiv, rnpx_zcomplex *b, size_t ldb);
+/* LU::Invert<double> -- RNP/LA/LU.hpp:146-214 (dgetri): inv(A) in place from
+ * its LU factors and pivots.  Returns the reference's value: 0, or -(i+1)
+ * for the first exactly-zero U_ii (a left unchanged), or RNPX_ERR_* (an
+ * invalid argument is RNPX_ERR_ARG, not -i, which would read as a zero U_ii).
+ * Workspace query as the reference's (work == NULL or *lwork == 0 writes
+ * 64 n); work is not used.  The reference's blocked branch (n > 64 with a
+ * full workspace) mis-indexes L and returns a wrong matrix; this computes the
+ * inverse both of its branches intend (its unblocked branch's). */
+int rnpx_dgetri(size_t n, double *a, size_t lda, const size_t *ipiv, size_t *lwork, double *work);
+/* LU::Invert<std::complex<double>> -- RNP/LA/LU.hpp:146-214 (zgetri). */
+int rnpx_zgetri(size_t n, rnpx_zcomplex *a, size_t lda, const size_t *ipiv, size_t *lwork,
+                rnpx_zcomplex *work);
 /* LU::RefineSolution<double> -- RNP/LA/LU.hpp:216-366 (dgerfs): refines x
  * (n x nrhs) for op(A) x = b from the factors af/ipiv, and returns the
  * componentwise backward error berr and the estimated forward error ferr

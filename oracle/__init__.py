@@ -88,6 +88,8 @@ def _declare(L):
             f.restype = ctypes.c_int
             f.argtypes = [_sz, _sz, _p, _sz, _p]
         getattr(L, f"orc_{pre}_lu_solve").argtypes = [ctypes.c_char_p, _sz, _sz, _p, _sz, _p, _p, _sz]
+        getattr(L, f"orc_{pre}_lu_invert").restype = ctypes.c_int
+        getattr(L, f"orc_{pre}_lu_invert").argtypes = [_sz, _p, _sz, _p, _p]
         getattr(L, f"orc_{pre}_lu_refine").argtypes = [ctypes.c_char_p, _sz, _sz, _p, _sz, _p, _sz, _p, _p, _sz,
                                                        _p, _sz, _p, _p]
         for fam in ("ql", "rq"):
@@ -563,6 +565,18 @@ def lu_factor(a, unblocked=False):
     f = getattr(lib(), f"orc_{pre}_lu_factor_unblocked" if unblocked else f"orc_{pre}_lu_factor")
     info = f(m, n, _ptr(a), max(m, 1), piv.ctypes.data_as(ctypes.c_void_p))
     return a, piv[:min(m, n)], info
+
+
+def lu_invert(lu, piv):
+    """LU::Invert (LU.hpp:146-214): returns (inv(A), the reference's return
+    value: 0, or -(i+1) for a zero U_ii with the input left unchanged)."""
+    a = _fortran(lu, np.result_type(lu, np.float64)).copy(order="F")
+    n = a.shape[0]
+    piv = np.ascontiguousarray(piv, dtype=np.uint64)
+    work = np.zeros(max(n, 1), dtype=a.dtype)
+    info = getattr(lib(), f"orc_{_pre(a.dtype)}_lu_invert")(n, _ptr(a), max(n, 1), piv.ctypes.data_as(ctypes.c_void_p),
+                                                            _ptr(work))
+    return a, info
 
 
 def lu_solve(trans, lu, piv, b):

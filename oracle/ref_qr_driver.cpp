@@ -84,15 +84,18 @@ int refqr_d_lu_factor(size_t m, size_t n, double* a, size_t lda, size_t* ipiv) {
 int refqr_z_lu_factor(size_t m, size_t n, double* a, size_t lda, size_t* ipiv) {
   return RNP::LA::LU::Factor(m, n, reinterpret_cast<zc*>(a), lda, ipiv);
 }
-int refqr_d_lu_invert(size_t n, double* a, size_t lda, const size_t* ipiv) {
+// lwork_cap > 0 shortens the workspace (lwork = n takes the unblocked branch).
+int refqr_d_lu_invert(size_t n, double* a, size_t lda, const size_t* ipiv, size_t lwork_cap) {
   size_t lwork = 0;
   RNP::LA::LU::Invert(n, a, lda, ipiv, &lwork, (double*)NULL);
+  if (lwork_cap && lwork_cap < lwork) lwork = lwork_cap;
   std::vector<double> work(lwork ? lwork : 1);
   return RNP::LA::LU::Invert(n, a, lda, ipiv, &lwork, &work[0]);
 }
-int refqr_z_lu_invert(size_t n, double* a, size_t lda, const size_t* ipiv) {
+int refqr_z_lu_invert(size_t n, double* a, size_t lda, const size_t* ipiv, size_t lwork_cap) {
   size_t lwork = 0;
   RNP::LA::LU::Invert(n, reinterpret_cast<zc*>(a), lda, ipiv, &lwork, (zc*)NULL);
+  if (lwork_cap && lwork_cap < lwork) lwork = lwork_cap;
   std::vector<zc> work(lwork ? lwork : 1);
   return RNP::LA::LU::Invert(n, reinterpret_cast<zc*>(a), lda, ipiv, &lwork, &work[0]);
 }

@@ -55,7 +55,7 @@ def _load():
         getattr(L, f"refqr_{pre}_generateq").argtypes = [_sz, _sz, _sz, _p, _sz, _p]
         getattr(L, f"refqr_{pre}_lu_factor").argtypes = [_sz, _sz, _p, _sz, _p]
         getattr(L, f"refqr_{pre}_lu_factor").restype = ctypes.c_int
-        getattr(L, f"refqr_{pre}_lu_invert").argtypes = [_sz, _p, _sz, _p]
+        getattr(L, f"refqr_{pre}_lu_invert").argtypes = [_sz, _p, _sz, _p, _sz]
         getattr(L, f"refqr_{pre}_lu_invert").restype = ctypes.c_int
     L.refqr_d_random_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         _sz, _sz, _p, _sz, _p, _p]
@@ -156,10 +156,11 @@ def lu_factor(a):
     return a, piv[:min(m, n)], info
 
 
-def lu_invert(lu, piv):
+def lu_invert(lu, piv, lwork_cap=0):
+    """LU::Invert; lwork_cap = n takes the reference's unblocked branch."""
     L = _load()
     a = _f(lu).copy(order="F")
     n = a.shape[0]
     p = np.ascontiguousarray(piv, dtype=np.uint64)
-    info = getattr(L, f"refqr_{_pre(a.dtype)}_lu_invert")(n, a.ctypes.data, max(n, 1), p.ctypes.data)
+    info = getattr(L, f"refqr_{_pre(a.dtype)}_lu_invert")(n, a.ctypes.data, max(n, 1), p.ctypes.data, lwork_cap)
     return a, info

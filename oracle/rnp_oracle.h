@@ -189,10 +189,12 @@ void orc_z_hess_generateq(size_t n, size_t ilo, size_t ihi, orc_z *a, size_t lda
  *   BLAS::MaximumIndex BLAS_mix.hpp:798-810, ApplyPermutations Permutation.hpp:33-52 */
 int orc_d_lu_factor_unblocked(size_t m, size_t n, double *a, size_t lda, size_t *piv);
 int orc_d_lu_factor(size_t m, size_t n, double *a, size_t lda, size_t *piv);
+int orc_d_lu_invert(size_t n, double *a, size_t lda, const size_t *ipiv, double *work);
 void orc_d_lu_solve(const char *trans, size_t n, size_t nrhs, const double *a, size_t lda, const size_t *ipiv,
                     double *b, size_t ldb);
 int orc_z_lu_factor_unblocked(size_t m, size_t n, orc_z *a, size_t lda, size_t *piv);
 int orc_z_lu_factor(size_t m, size_t n, orc_z *a, size_t lda, size_t *piv);
+int orc_z_lu_invert(size_t n, orc_z *a, size_t lda, const size_t *ipiv, orc_z *work);
 void orc_z_lu_solve(const char *trans, size_t n, size_t nrhs, const orc_z *a, size_t lda, const size_t *ipiv,
                     orc_z *b, size_t ldb);
 

@@ -1194,6 +1194,66 @@ int FN(lu_factor)(size_t m, size_t n, T *a, size_t lda, size_t *piv){
 	return info ? info+1 : 0;
 }
 
+/* LU::Invert, RNP/LA/LU.hpp:146-214: inv(A) in place from its LU factors.
+ *  - Triangular::Invert("U","N") (RNP/LA/Triangular.hpp:207-253): returns
+ *    -(i+1) for the first zero U_ii and leaves a untouched; otherwise inv(U)
+ *    in the upper triangle.  Restated in its unblocked form
+ *    (Triangular.hpp:150-172: U_jj <- 1/U_jj, then column j above the
+ *    diagonal <- -U_jj * triu(U(0:j,0:j)) U(0:j,j), a triangular
+ *    matrix-vector product); the blocked form (Triangular.hpp:227-236) is the
+ *    same product reassociated.
+ *  - inv(A) L = inv(U) for inv(A), columns from the right (LU.hpp:172-185):
+ *    save and zero L's column j, then a(:,j) -= a(:,j+1:n) l(j+1:n).
+ *  - the column interchanges backward (LU.hpp:210, ApplyPermutations "R" "B",
+ *    RNP/LA/Permutation.hpp:60-66).
+ * The reference's blocked branch (LU.hpp:186-207, taken for n > 64 with a
+ * full workspace) saves columns jj instead of j+jj of L and so is wrong for
+ * every block but the first; this restatement computes the inverse both
+ * branches intend (LAPACK dgetri's), which the unblocked branch delivers.
+ * work holds n elements. */
+int FN(lu_invert)(size_t n, T *a, size_t lda, const size_t *ipiv, T *work){
+	if(0 == n){ return 0; }
+	for(size_t i = 0; i < n; ++i){
+		if((T)0 == a[i+i*lda]){ return -(int)(i+1); }
+	}
+	for(size_t j = 0; j < n; ++j){
+		a[j+j*lda] = (T)1 / a[j+j*lda];
+		const T ajj = -a[j+j*lda];
+		/* x = triu(U(0:j,0:j)) x in place, x = a(0:j, j): row i uses x_k, k >= i */
+		for(size_t i = 0; i < j; ++i){
+			T acc = 0;
+			for(size_t k = i; k < j; ++k){ acc += a[i+k*lda] * a[k+j*lda]; }
+			a[i+j*lda] = acc;
+		}
+		for(size_t i = 0; i < j; ++i){ a[i+j*lda] *= ajj; }
+	}
+	size_t j = n;
+	while(j-- > 0){
+		for(size_t i = j+1; i < n; ++i){
+			work[i] = a[i+j*lda];
+			a[i+j*lda] = 0;
+		}
+		if(j+1 < n){
+			for(size_t i = 0; i < n; ++i){
+				T acc = 0;
+				for(size_t k = j+1; k < n; ++k){ acc += a[i+k*lda] * work[k]; }
+				a[i+j*lda] -= acc;
+			}
+		}
+	}
+	j = n;
+	while(j-- > 0){
+		const size_t jp = ipiv[j];
+		if(jp == j){ continue; }
+		for(size_t i = 0; i < n; ++i){
+			const T t = a[i+j*lda];
+			a[i+j*lda] = a[i+jp*lda];
+			a[i+jp*lda] = t;
+		}
+	}
+	return 0;
+}
+
 /* One triangular solve of LU::Solve (LU.hpp:117-144) on every column of b:
  * op(M) x = b with M the unit lower (lower) or upper part of a. */
 static void FN(lu_trsv)(int lower, int unit, char op, size_t n, size_t nrhs, const T *a, size_t lda, T *b,

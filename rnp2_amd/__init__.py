@@ -25,6 +25,7 @@ RNPX_ERR_NOMEM = -1002
 RNPX_ERR_SYNC = -1003
 RNPX_ERR_UNSUPPORTED = -1004
 RNPX_ERR_NOTCONV = -1005
+RNPX_ERR_ARG = -1006
 
 _sz = ctypes.c_size_t
 _p = ctypes.c_void_p
@@ -109,6 +110,8 @@ SIGNATURES = {
     "rnpx_dgetrf": (_i, [_sz, _sz, _p, _sz, _p]),
     "rnpx_zgetrf": (_i, [_sz, _sz, _p, _sz, _p]),
     "rnpx_dgetrs": (_i, [_s, _sz, _sz, _p, _sz, _p, _p, _sz]),
+    "rnpx_dgetri": (_i, [_sz, _p, _sz, _p, _p, _p]),
+    "rnpx_zgetri": (_i, [_sz, _p, _sz, _p, _p, _p]),
     "rnpx_zgetrs": (_i, [_s, _sz, _sz, _p, _sz, _p, _p, _sz]),
     "rnpx_dormhr": (_i, [_s, _s, _sz, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
     "rnpx_zunmhr": (_i, [_s, _s, _sz, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
@@ -462,6 +465,24 @@ def lu_solve(trans, lu, piv, b):
     _check(fn(trans.encode(), n, nrhs, _ptr(lu), lu.shape[0], piv.ctypes.data_as(ctypes.c_void_p), _ptr(b),
               max(n, 1)), "LU::Solve")
     return b
+
+
+def lu_invert(lu, piv):
+    """LU::Invert (LU.hpp:146-214) on the GPU: returns (inv(A), the
+    reference's return value: 0, or -(i+1) for a zero U_ii with the factors
+    returned unchanged)."""
+    a = np.array(lu, dtype=np.result_type(lu, np.float64), order="F", copy=True)
+    n = a.shape[0]
+    piv = np.ascontiguousarray(piv, dtype=np.uint64)
+    lw = np.zeros(1, dtype=np.uint64)
+    fn = getattr(lib(), f"rnpx_{_pre(a.dtype)}getri")
+    _check(fn(n, None, max(n, 1), None, lw.ctypes.data_as(ctypes.c_void_p), None), "LU::Invert query")
+    work = np.zeros(max(int(lw[0]), 1), dtype=a.dtype)
+    info = fn(n, _ptr(a), max(n, 1), piv.ctypes.data_as(ctypes.c_void_p), lw.ctypes.data_as(ctypes.c_void_p),
+              _ptr(work))
+    if info <= -1000:
+        _check(info, "LU::Invert")
+    return a, info
 
 
 def lu_refine(trans, a, lu, piv, b, x):

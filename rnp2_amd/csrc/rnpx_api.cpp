@@ -445,6 +445,34 @@ int dropin_getrs(const char* trans, size_t n, size_t nrhs, const T* a, size_t ld
 }
 
 template <typename T>
+int dropin_getri(size_t n, T* a, size_t lda, const size_t* ipiv, size_t* lwork, bool query) {
+  if (!lwork) return RNPX_ERR_ARG;
+  if (n == 0) return RNPX_OK;        // LU.hpp:149, before the query
+  if (query) {
+    *lwork = 64 * n;                 // LU::Tuning<T>::invert_block_size_opt(n) * n, LU.hpp:151-154
+    return RNPX_OK;
+  }
+  if (lda < n || !a || !ipiv) return RNPX_ERR_ARG;
+  if (!fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  std::vector<int> piv(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (ipiv[i] >= n) return RNPX_ERR_ARG;
+    piv[i] = (int)ipiv[i];
+  }
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A;
+  if ((st = A.in(h, h->stage1, a, n, n, lda, true))) return st;
+  int info = 0;
+  if ((st = getri_dev<T>(h, (int)n, A.dev, A.ld, piv.data(), &info))) return st > -1000 ? RNPX_ERR_ARG : st;
+  if (info != 0) return info;        // a left as it was
+  if ((st = A.out(h, lda))) return st;
+  if ((st = sync_handle(h))) return st;
+  return info;
+}
+
+template <typename T>
 int dropin_gerfs(const char* trans, size_t n, size_t nrhs, const T* a, size_t lda, const T* af, size_t ldaf,
                  const size_t* ipiv, const T* b, size_t ldb, T* x, size_t ldx, double* ferr, double* berr) {
   if (!trans || (trans[0] != 'N' && trans[0] != 'T' && trans[0] != 'C')) return -1;
@@ -629,6 +657,12 @@ int rnpx_dgetrf(size_t m, size_t n, double* a, size_t lda, size_t* pivots) {
 }
 int rnpx_zgetrf(size_t m, size_t n, rnpx_zcomplex* a, size_t lda, size_t* pivots) {
   return dropin_getrf<zd>(m, n, reinterpret_cast<zd*>(a), lda, pivots);
+}
+int rnpx_dgetri(size_t n, double* a, size_t lda, const size_t* ipiv, size_t* lwork, double* work) {
+  return dropin_getri<double>(n, a, lda, ipiv, lwork, lwork && (work == nullptr || *lwork == 0));
+}
+int rnpx_zgetri(size_t n, rnpx_zcomplex* a, size_t lda, const size_t* ipiv, size_t* lwork, rnpx_zcomplex* work) {
+  return dropin_getri<zd>(n, reinterpret_cast<zd*>(a), lda, ipiv, lwork, lwork && (work == nullptr || *lwork == 0));
 }
 int rnpx_dgetrs(const char* trans, size_t n, size_t nrhs, const double* a, size_t lda, const size_t* ipiv, double* b,
                 size_t ldb) {

@@ -58,12 +58,13 @@ struct Handle {
   Buffer qlbuf;                            // reversed reflectors and tau (QL, RQ)
   Buffer lubuf;                            // LU panel scratch, Solve's permutation and gather
   Buffer rfbuf;                            // RefineSolution's residual and |op(A)||x| + |b|
+  Buffer invbuf;                           // LU::Invert's n x n right-hand side and U's diagonal
   unsigned trsm_epoch = 0;                 // value the flags take in the current solve
   // Every buffer above, for accounting and release.
   std::vector<Buffer*> buffers() {
     return {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc,
             &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
-            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf};
+            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf, &invbuf};
   }
   hipError_t sync_all_streams();       // the handle's stream, panel stream and partition streams
   void* batch = nullptr;               // batched-call descriptor buffers
@@ -152,6 +153,9 @@ template <typename T> int orghr_dev(Handle* h, int n, int ilo, int ihi, T* A, lo
 template <typename T> int getrf_dev(Handle* h, int m, int n, T* A, long lda, int* piv_host, int* info);
 template <typename T>
 int getrs_dev(Handle* h, char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb);
+// LU::Invert: inv(A) in place from the factors; *info = 0, or -(i+1) for a
+// zero U_ii (A untouched).  Returns a status.
+template <typename T> int getri_dev(Handle* h, int n, T* A, long lda, const int* ipiv, int* info);
 // LU::RefineSolution (rnpx_lu_driver.cpp); A, AF, B, X on the device, ipiv,
 // ferr and berr on the host.
 template <typename T>

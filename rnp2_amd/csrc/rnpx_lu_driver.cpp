@@ -349,6 +349,40 @@ template int gerfs_dev<zd>(Handle*, char, int, int, const zd*, long, const zd*, 
 
 template int getrf_dev<double>(Handle*, int, int, double*, long, int*, int*);
 template int getrf_dev<zd>(Handle*, int, int, zd*, long, int*, int*);
+// LU::Invert (RNP/LA/LU.hpp:146-214).  The reference first inverts U in
+// place (Triangular::Invert, which returns -(i+1) for the first zero U_ii
+// before touching anything), then solves inv(A) L = inv(U) and undoes the
+// column interchanges.  Here the same inverse comes from the factors through
+// the device LU solve: X = A^{-1} I (the interchanges gathered into I, L then
+// U solved by the blocked triangular kernels), copied into A.  The zero test
+// reads U's diagonal to the host first, so A is untouched when it fails.
+template <typename T>
+int getri_dev(Handle* h, int n, T* A, long lda, const int* ipiv, int* info) {
+  *info = 0;
+  if (n < 0) return -1;
+  if (lda < std::max(1, n)) return -3;
+  if (n == 0) return RNPX_OK;
+  RNPX_TRY(h->invbuf.ensure(sizeof(T) * (size_t)n * n, h));
+  T* X = (T*)h->invbuf.p;
+  std::vector<T> diag(n);
+  RNPX_TRY(hipMemcpy2DAsync(diag.data(), sizeof(T), A, sizeof(T) * (size_t)(lda + 1), sizeof(T), n,
+                            hipMemcpyDeviceToHost, h->stream));
+  RNPX_TRY(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < n; ++i) {
+    if (S<T>::is_zero(diag[i])) {
+      *info = -(i + 1);
+      return RNPX_OK;
+    }
+  }
+  RNPX_TRY(launch_set<T>(n, n, S<T>::zero(), S<T>::one(), X, n, h->stream));
+  int s = getrs_dev<T>(h, 'N', n, n, A, lda, ipiv, X, n);
+  if (s) return s;
+  RNPX_TRY(launch_copy<T>(n, n, X, n, A, lda, h->stream));
+  return RNPX_OK;
+}
+template int getri_dev<double>(Handle*, int, double*, long, const int*, int*);
+template int getri_dev<zd>(Handle*, int, zd*, long, const int*, int*);
+
 template int getrs_dev<double>(Handle*, char, int, int, const double*, long, const int*, double*, long);
 template int getrs_dev<zd>(Handle*, char, int, int, const zd*, long, const int*, zd*, long);
 

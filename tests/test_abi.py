@@ -351,3 +351,24 @@ def test_lu_argument_errors_and_empty():
     assert L.rnpx_dgetrs(b"N", 4, 1, None, 3, pp, None, 4) == -5       # lda < n
     bad = np.array([0, 9, 2, 3], dtype=np.uint64)
     assert L.rnpx_dgetrs(b"N", 4, 1, None, 4, bad.ctypes.data_as(ctypes.c_void_p), None, 4) == -6  # pivot out of range
+
+
+def test_getri_query_and_arguments():
+    """LU::Invert's workspace query (LU.hpp:149-155): n = 0 returns before it,
+    then 64 n; invalid arguments are RNPX_ERR_ARG (-i would read as the
+    reference's -(i+1) zero-pivot result)."""
+    L = R.lib()
+    lw = ctypes.c_size_t(9)
+    assert L.rnpx_dgetri(0, None, 1, None, ctypes.byref(lw), None) == 0 and lw.value == 9
+    lw = ctypes.c_size_t(0)
+    assert L.rnpx_dgetri(100, None, 100, None, ctypes.byref(lw), None) == 0 and lw.value == 6400
+    lw = ctypes.c_size_t(0)
+    assert L.rnpx_zgetri(30, None, 30, None, ctypes.byref(lw), None) == 0 and lw.value == 64 * 30
+    a = np.zeros(16)
+    piv = np.zeros(4, dtype=np.uint64)
+    lw = ctypes.c_size_t(4)
+    assert L.rnpx_dgetri(4, a.ctypes.data_as(ctypes.c_void_p), 3, piv.ctypes.data_as(ctypes.c_void_p),
+                         ctypes.byref(lw), a.ctypes.data_as(ctypes.c_void_p)) == R.RNPX_ERR_ARG
+    piv[2] = 9
+    assert L.rnpx_dgetri(4, a.ctypes.data_as(ctypes.c_void_p), 4, piv.ctypes.data_as(ctypes.c_void_p),
+                         ctypes.byref(lw), a.ctypes.data_as(ctypes.c_void_p)) == R.RNPX_ERR_ARG

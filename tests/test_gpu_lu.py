@@ -105,3 +105,37 @@ def test_lu_refine_matches_oracle(trans, cplx):
         print(f"        n={n}: x {ex:.2e} ferr {ef:.2e} berr {bg} (oracle {bo})", flush=True)
         assert ex <= 1e-12 and ef <= 0.05
         assert np.all(bg <= 4 * np.finfo(float).eps) and np.all(bo <= 4 * np.finfo(float).eps)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 300, 1000, 2048])
+def test_lu_invert_matches_oracle(n, cplx):
+    """LU::Invert (LU.hpp:146-214) through the drop-in ABI against the
+    oracle's restatement of the same factors: entries within 1e-11 times
+    kappa-scaled rounding of the largest entry, and inv(A) A = I."""
+    a = seeded(n, n, cplx, 31 + n)
+    print(f"[case] LU::Invert n={n} cplx={cplx}", flush=True)
+    lu, piv, info = O.lu_factor(a)
+    assert info == 0
+    inv_o, st_o = O.lu_invert(lu, piv)
+    inv_g, st_g = R.lu_invert(lu, piv)
+    scale = np.abs(inv_o).max()
+    e = float(np.abs(inv_g - inv_o).max() / scale)
+    resid = float(np.abs(inv_g @ a - np.eye(n)).max())
+    kappa = float(np.linalg.cond(a)) if n <= 1000 else 1e4
+    print(f"        status {st_g} (oracle {st_o}), |inv - inv_o| / |inv_o| {e:.2e}, |inv A - I| {resid:.2e}, "
+          f"kappa {kappa:.1e}", flush=True)
+    assert st_g == st_o == 0
+    assert e <= 1e-13 * n * kappa and resid <= 1e-13 * n * kappa
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_lu_invert_zero_pivot(cplx):
+    """A zero U_ii returns -(i+1) and leaves the factors as they were, as
+    Triangular::Invert does (Triangular.hpp:217-223)."""
+    print(f"[case] LU::Invert zero pivot cplx={cplx}", flush=True)
+    a = seeded(50, 50, cplx, 5)
+    lu, piv, _ = O.lu_factor(a)
+    lu[17, 17] = 0.0
+    out, st = R.lu_invert(lu, piv)
+    assert st == -18 and np.array_equal(out, lu)

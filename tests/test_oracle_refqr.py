@@ -215,3 +215,69 @@ def test_factor_panels_equals_factor_leading_panels():
         full = O.multq("L", "C", np.asfortranarray(ap[:, :c]), taup[:c], np.asfortranarray(a[:, c:]))
         assert np.max(np.abs(full - ap[:, c:])) <= 1e-12 * np.max(np.abs(a)), P
         assert qp.shape == (m, c)
+
+
+# -- LU::Invert (RNP/LA/LU.hpp:146-214) ---------------------------------------
+def _lu_case(n, cplx, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+    lu, piv, info = O.lu_factor(a)
+    assert info == 0
+    return a, lu, piv
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 65, 130, 257])
+def test_lu_invert_is_the_inverse(n, cplx):
+    """The oracle's LU::Invert equals the inverse of A (numpy's, to the
+    conditioning) and inv(A) A = I."""
+    a, lu, piv = _lu_case(n, cplx, 100 + n)
+    inv, st = O.lu_invert(lu, piv)
+    assert st == 0
+    ref = np.linalg.inv(a)
+    kappa = np.linalg.cond(a)
+    assert np.max(np.abs(inv - ref)) <= 50 * n * kappa * 2.2e-16 * np.max(np.abs(ref))
+    assert np.max(np.abs(inv @ a - np.eye(n))) <= 50 * n * kappa * 2.2e-16
+
+
+def test_lu_invert_zero_pivot_returns_minus_index():
+    """Triangular::Invert's zero test (Triangular.hpp:217-223): the first
+    exactly-zero U_ii returns -(i+1) and nothing is modified."""
+    rng = np.random.default_rng(4)
+    lu = np.triu(rng.standard_normal((6, 6))) + np.tril(rng.standard_normal((6, 6)), -1)
+    lu[3, 3] = 0.0
+    piv = np.array([2, 1, 5, 3, 4, 5], dtype=np.uint64)
+    out, st = O.lu_invert(lu, piv)
+    assert st == -4 and np.array_equal(out, lu)
+
+
+@live
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("n", [5, 64, 65, 130, 200])
+def test_lu_invert_live_unblocked_branch(n, cplx):
+    """Against the reference's own LU::Invert: its unblocked branch (a
+    workspace of n, LU.hpp:170-185) at every n, and its default workspace where
+    n <= 64 (the same branch)."""
+    refqr.set_threads(1)
+    a, lu, piv = _lu_case(n, cplx, n)
+    inv, st = O.lu_invert(lu, piv)
+    r_inv, r_st = refqr.lu_invert(lu, piv, lwork_cap=n)
+    scale = np.max(np.abs(r_inv))
+    assert st == r_st == 0
+    assert np.max(np.abs(inv - r_inv)) <= 1e-12 * scale
+    if n <= 64:
+        d_inv, _ = refqr.lu_invert(lu, piv)
+        assert np.max(np.abs(inv - d_inv)) <= 1e-12 * scale
+
+
+@live
+def test_reference_blocked_lu_invert_defect():
+    """The defect documented for the reference's blocked branch (n > 64 with
+    the default workspace: LU.hpp:192-197 saves columns jj, not j + jj, of L):
+    its result is not the inverse, while its unblocked branch is."""
+    a, lu, piv = _lu_case(100, False, 7)
+    d_inv, _ = refqr.lu_invert(lu, piv)
+    u_inv, _ = refqr.lu_invert(lu, piv, lwork_cap=100)
+    ref = np.linalg.inv(a)
+    assert np.max(np.abs(u_inv - ref)) <= 1e-10 * np.max(np.abs(ref))
+    assert np.max(np.abs(d_inv - ref)) > 1e-3 * np.max(np.abs(ref))
