@@ -59,7 +59,8 @@ typedef struct rnpx_batched_plan_s *rnpx_batched_plan;
 /* Tuning hook: the counterpart of specialising QR::Tuning<T>
  * (RNP/LA/QR.hpp:41-51).  Zero fields keep the built-in choice. */
 typedef struct rnpx_tuning {
-	int nb;           /* panel width: 32, 64 (128 real) */
+	int nb;           /* outer block width: 32, 64, 128 (0 = the autotuned table's
+	                     entry for the shape, else 128 with 64-column panels) */
 	int panel_rpt;    /* panel rows per thread: 1, 2, 4 */
 	int ksplit;       /* split-K slices for W = V^H C (0 = automatic) */
 	int lookahead;    /* 1 = overlap the next panel with the trailing update */
@@ -246,6 +247,26 @@ int rnpx_set_stream(rnpx_handle h, void *hip_stream);
 void *rnpx_get_stream(rnpx_handle h);
 int rnpx_set_tuning(rnpx_handle h, const rnpx_tuning *t);
 int rnpx_get_tuning(rnpx_handle h, rnpx_tuning *t);
+
+/* The autotuning table behind the automatic tuning (the device counterpart
+ * of RNP's QR::Tuning hooks, RNP/LA/QR.hpp:41-51, which fix nb = 32 for
+ * every shape).  rnp2_amd/autotune.py times candidate configurations on the
+ * device and writes it as JSON; it is read once per process on first use,
+ * from RNPX_TUNING_FILE or rnpx_tuning.json beside librnpx.so.  A
+ * factorization whose handle tuning has nb = 0 takes the entry nearest its
+ * shape (element type equal, m and n within a factor of two): the outer
+ * block width (32, 64 or 128) and the W = V^H C split.  Batched calls take
+ * the small tier's LDS per team and the blocked-path column threshold. */
+/* Replace the table from path (NULL: the default location, where no file
+ * means an empty table); returns the entry count, RNPX_ERR_UNSUPPORTED for an
+ * unreadable path, RNPX_ERR_ARG for a malformed file (the previous table is
+ * kept in both cases). */
+int rnpx_tuning_load(const char *path);
+/* 1 and the entry's nb and W split when the table has an entry for an
+ * m x n factorization of element type dtype ('d' or 'z'), else 0. */
+int rnpx_tuning_lookup_factor(char dtype, size_t m, size_t n, int *nb, int *ksplit);
+/* 1 and the batched parameters for element type dtype, else 0. */
+int rnpx_tuning_lookup_batched(char dtype, int *lds_bytes, int *blocked_min);
 /* Wait for the handle's stream; reports RNPX_ERR_SYNC if a panel barrier
  * timed out in any call since the last check. */
 int rnpx_synchronize(rnpx_handle h);

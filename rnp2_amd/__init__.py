@@ -134,6 +134,9 @@ SIGNATURES = {
     "rnpx_dls_berr_dev": (_i, [_p, _sz, _sz, _sz, _p, _sz, _p, _sz, _p, _sz, ctypes.POINTER(ctypes.c_double)]),
     "rnpx_dgeqrf_batched": (_i, [_p, _i, _p, _p, _p, _p]),
     "rnpx_dgeqrf_batched_noalloc": (_i, [_p, _i, _p, _p, _p, _p]),
+    "rnpx_tuning_load": (_i, [ctypes.c_char_p]),
+    "rnpx_tuning_lookup_factor": (_i, [ctypes.c_char, _sz, _sz, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "rnpx_tuning_lookup_batched": (_i, [ctypes.c_char, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "rnpx_zgeqrf_batched": (_i, [_p, _i, _p, _p, _p, _p]),
     "rnpx_dgeqrf_batched_plan": (_i, [_p, _i, _p, _p, _p, _p, ctypes.POINTER(_p)]),
     "rnpx_zgeqrf_batched_plan": (_i, [_p, _i, _p, _p, _p, _p, ctypes.POINTER(_p)]),
@@ -535,6 +538,27 @@ def hess_generateq(afac, tau, ilo, ihi):
     lw = _sz(work.size)
     _check(fn(n, ilo, ihi, _ptr(a), max(n, 1), _ptr(tau), ctypes.byref(lw), _ptr(work)), "Hessenberg::GenerateQ")
     return a
+
+
+def tuning_load(path=None):
+    """Replace the autotuning table (rnpx_tuning_load); returns the entry count."""
+    st = lib().rnpx_tuning_load(None if path is None else str(path).encode())
+    if st < 0:
+        raise RnpxError(st, "rnpx_tuning_load")
+    return st
+
+
+def tuning_lookup_factor(dtype, m, n):
+    """(nb, ksplit) of the table's entry for an m x n factorization, or None."""
+    nb, ks = ctypes.c_int(), ctypes.c_int()
+    found = lib().rnpx_tuning_lookup_factor(dtype.encode(), m, n, ctypes.byref(nb), ctypes.byref(ks))
+    return (nb.value, ks.value) if found == 1 else None
+
+
+def tuning_lookup_batched(dtype):
+    lds, bm = ctypes.c_int(), ctypes.c_int()
+    found = lib().rnpx_tuning_lookup_batched(dtype.encode(), ctypes.byref(lds), ctypes.byref(bm))
+    return (lds.value, bm.value) if found == 1 else None
 
 
 def version():

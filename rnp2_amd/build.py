@@ -33,6 +33,7 @@ SOURCES = [
     "rnpx_ls.cpp",
     "rnpx_serve.cpp",
     "rnpx_api.cpp",
+    "rnpx_tune.cpp",
 ]
 
 HIPFLAGS = [
@@ -78,7 +79,7 @@ def build(verbose=False, jobs=None):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-ldl"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -302,21 +302,24 @@ __global__ __launch_bounds__(kLargeThreads) void batched_large_kernel(int count,
 
 template <typename T>
 hipError_t launch_batched(int tier, int count, const int* order, const rnpx_qr_desc* desc, T* A, T* tau, T* B,
-                          unsigned* counter, int teams, int lds_bytes, hipStream_t st) {
+                          unsigned* counter, int teams, int param, hipStream_t st) {
   if (count <= 0) return hipSuccess;
   if (teams < 1) return hipErrorInvalidValue;
   RNPX_LAUNCH_CHECK(launch_zero_u32(counter, 1, st));
   if (tier == 0) {
+    const int lds_bytes = param;
     if (lds_bytes < 1024 || lds_bytes > 160 * 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL((batched_small_kernel<T>), dim3(teams), dim3(64), lds_bytes, st, count, order, desc, A, tau, B,
                        counter, lds_bytes / (int)sizeof(T));
   } else {
     // Real problems of at least bmin columns take the blocked path even when
-    // they would fit in LDS whole (RNPX_BATCH_BLOCKED_MIN, read once).
-    static const int bmin = [] {
+    // they would fit in LDS whole (RNPX_BATCH_BLOCKED_MIN, read once,
+    // overrides the caller's value).
+    static const int env_bmin = [] {
       const char* e = getenv("RNPX_BATCH_BLOCKED_MIN");
-      return e ? atoi(e) : kBlockedMinCols;
+      return e ? atoi(e) : 0;
     }();
+    const int bmin = env_bmin > 0 ? env_bmin : (param > 0 ? param : kBlockedMinCols);
     hipLaunchKernelGGL((batched_large_kernel<T>), dim3(teams), dim3(kLargeThreads), 0, st, count, order, desc, A, tau, B,
                        counter, bmin);
   }

@@ -106,11 +106,24 @@ template <typename T> int geqrf_batched(Handle* h, int count, const rnpx_qr_desc
 // kBatchLdsBytes run from LDS in one-wave teams (the small tier), the rest in
 // place in 512-thread teams (the large tier); each tier's list is sorted by
 // cost, largest first.  Returns -3 on a bad descriptor.
-constexpr int kBatchLdsBytes = 40 * 1024;  // per one-wave team: four teams per CU
+constexpr int kBatchLdsBytes = 40 * 1024;  // per one-wave team: four teams per CU (default)
+constexpr int kBatchBlockedMin = 33;       // real problems of at least this many columns: blocked (default)
 struct BatchSetup {
   std::vector<int> small, large;
+  int lds_bytes = kBatchLdsBytes;       // per one-wave team of the small tier (also the tier threshold)
+  int blocked_min = kBatchBlockedMin;   // the large tier's blocked-path threshold
 };
+// Fills the tiers with bs->lds_bytes as the threshold; call batch_params
+// first to take the autotuned values.
 template <typename T> int batch_setup(int count, const rnpx_qr_desc* desc, BatchSetup* out);
+template <typename T> void batch_params(BatchSetup* out);
+// The persisted autotuning table (rnpx_tune.cpp): the entry nearest to an
+// m x n factorization of element type dtype ('d' or 'z'), and the batched
+// tiers' parameters.  False: no entry (defaults apply).
+bool tuning_factor(char dtype, int m, int n, int* nb, int* ksplit);
+bool tuning_batched(char dtype, int* lds, int* blocked_min);
+int tuning_load(const char* path);
+
 // Allocation-free batched entry point's set-up: resolves this device's copy of
 // the static scratch array once per device (called by create_handle with the
 // device current).

@@ -20,13 +20,27 @@ constexpr int kMaxPanelGroups = 256;
 // Grid-barrier timeout: 2 s of the 100 MHz wall clock.
 constexpr uint64_t kBarrierTimeoutTicks = 200000000ull;
 
+// Agent-scope release of the global stores made before it (this wave's, and
+// after a workgroup barrier every wave's): the fence writes back this XCD's
+// L2, then a hard wait for that write-back, before the caller's relaxed flag
+// or counter update.  The wait must be explicit: with an earlier
+// s_waitcnt vmcnt(0) in the source, hipcc (gfx950) merges the release's own
+// wait into that earlier one, so the write-back was still in flight when the
+// flag became visible and a reader on another XCD could acquire stale data.
+__device__ __forceinline__ void release_agent() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+}
+
 // Bounded grid barrier: arrive with release, spin on a relaxed load, then an
 // acquire fence.  Gives up (and raises the abort flag, which every spinner
 // also watches) after ~2 s of wall clock so that no wave can spin forever.
 __device__ inline void grid_barrier(unsigned* counter, unsigned* abortf, unsigned target) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // this wave's stores done
   __syncthreads();
   if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    release_agent();
+    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t t0 = wall_clock64();
     while (true) {
       unsigned v = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -53,7 +67,11 @@ __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F7
 __device__ __forceinline__ unsigned arrive(unsigned* counter, unsigned* slot) {
   wait_stores();
   __syncthreads();
-  if (threadIdx.x == 0) *slot = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) {
+    release_agent();
+    *slot = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
   __syncthreads();
   return *slot;
 }
@@ -62,7 +80,10 @@ __device__ __forceinline__ unsigned arrive(unsigned* counter, unsigned* slot) {
 __device__ __forceinline__ void publish(unsigned* flag) {
   wait_stores();
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) {
+    release_agent();
+    __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Bounded wait for a flag (abort flag and ~2 s wall-clock limit), then
@@ -346,10 +367,12 @@ hipError_t launch_colnorm2(int m, int n, const double* X, long ldx, double* out,
 // Y += X
 hipError_t launch_axpy_mat(int m, int n, const double* X, long ldx, double* Y, long ldy, hipStream_t st);
 
-// Batched QR / LS: tier 0 = one wave per problem in LDS (lds_bytes of
-// dynamic LDS per team), tier 1 = 256 threads per problem in place.
+// Batched QR / LS: tier 0 = one wave per problem in LDS (param: bytes of
+// dynamic LDS per team), tier 1 = 512 threads per problem in place (param:
+// the column count from which real problems take the blocked path, 0 = the
+// default).
 template <typename T>
 hipError_t launch_batched(int tier, int count, const int* order, const rnpx_qr_desc* desc, T* A, T* tau, T* B,
-                          unsigned* counter, int teams, int lds_bytes, hipStream_t st);
+                          unsigned* counter, int teams, int param, hipStream_t st);
 
 }  // namespace rnpx

@@ -216,7 +216,7 @@ int batch_setup(int count, const rnpx_qr_desc* desc, BatchSetup* out) {
     const double k = std::min(d.m, d.n);
     cost[i] = (double)d.m * d.n * k + (double)d.nrhs * d.m * d.n;
     const long need = (long)(d.m | 1) * d.n * (long)sizeof(T);
-    (need <= kBatchLdsBytes ? out->small : out->large).push_back(i);
+    (need <= out->lds_bytes ? out->small : out->large).push_back(i);
   }
   auto by_cost = [&](int a, int b) { return cost[a] != cost[b] ? cost[a] > cost[b] : a < b; };
   std::sort(out->small.begin(), out->small.end(), by_cost);
@@ -255,11 +255,12 @@ static int batch_launch(Handle* h, const BatchSetup& bs, BatchBuffers* bb, T* A,
   if (nl > 0) {
     RNPX_TRY(hipEventRecord(h->ev_start, h->stream));
     RNPX_TRY(hipStreamWaitEvent(h->pstream, h->ev_start, 0));
-    RNPX_TRY(launch_batched<T>(1, nl, order, d, A, tau, B, ctr, std::min(nl, h->num_cus), 0, h->pstream));
+    RNPX_TRY(launch_batched<T>(1, nl, order, d, A, tau, B, ctr, std::min(nl, h->num_cus), bs.blocked_min, h->pstream));
   }
   if (ns > 0) {
-    const int teams = std::min(ns, 4 * h->num_cus);
-    RNPX_TRY(launch_batched<T>(0, ns, order + nl, d, A, tau, B, ctr + 1, teams, kBatchLdsBytes, h->stream));
+    const int per_cu = std::max(1, (160 * 1024) / bs.lds_bytes);
+    const int teams = std::min(ns, per_cu * h->num_cus);
+    RNPX_TRY(launch_batched<T>(0, ns, order + nl, d, A, tau, B, ctr + 1, teams, bs.lds_bytes, h->stream));
   }
   if (nl > 0) {
     RNPX_TRY(hipEventRecord(h->ev_next, h->pstream));
@@ -276,6 +277,7 @@ int geqrf_batched(Handle* h, int count, const rnpx_qr_desc* desc, T* A, T* tau, 
   if (!A) return -4;
   if (!tau) return -5;
   BatchSetup bs;
+  batch_params<T>(&bs);
   int s = batch_setup<T>(count, desc, &bs);
   if (s) return s;
   // one set of descriptor buffers per handle
@@ -308,6 +310,7 @@ int geqrf_batched_plan(Handle* h, int count, const rnpx_qr_desc* desc, T* A, T* 
   if (count <= 0) return -2;
   if (!desc) return -3;
   BatchSetup bs;
+  batch_params<T>(&bs);
   int s = batch_setup<T>(count, desc, &bs);
   if (s) return s;
   BatchedPlan* p = new BatchedPlan();
@@ -352,6 +355,16 @@ int batched_plan_destroy(BatchedPlan* p) {
   return RNPX_OK;
 }
 
+template <typename T>
+void batch_params(BatchSetup* out) {
+  int lds = 0, bmin = 0;
+  if (tuning_batched(S<T>::cplx ? 'z' : 'd', &lds, &bmin)) {
+    out->lds_bytes = lds;
+    out->blocked_min = bmin;
+  }
+}
+template void batch_params<double>(BatchSetup*);
+template void batch_params<zd>(BatchSetup*);
 template int batch_setup<double>(int, const rnpx_qr_desc*, BatchSetup*);
 template int batch_setup<zd>(int, const rnpx_qr_desc*, BatchSetup*);
 template int geqrf_batched<double>(Handle*, int, const rnpx_qr_desc*, double*, double*, double*);

@@ -140,7 +140,8 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuPanelArgs a) {
     wait_stores();
     __syncthreads();
     if (tid == 0) {
-      __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      release_agent();
+      __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const uint64_t t0 = wall_clock64();
       while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (unsigned)G) {
         if (__hip_atomic_load(a.abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;

@@ -80,6 +80,7 @@ static int geqrf_batched_noalloc(Handle* h, int count, const rnpx_qr_desc* desc,
   if (!tau) return -5;
   if (count > kNoAllocMaxProblems) return RNPX_ERR_UNSUPPORTED;
   BatchSetup bs;
+  batch_params<T>(&bs);
   int s = batch_setup<T>(count, desc, &bs);
   if (s) return s;
   unsigned char* base = (unsigned char*)scratch_of(h->device);
@@ -96,10 +97,13 @@ static int geqrf_batched_noalloc(Handle* h, int count, const rnpx_qr_desc* desc,
   RNPX_TRY_HIP(hipStreamSynchronize(st));
   const int nl = (int)bs.large.size(), ns = (int)bs.small.size();
   if (nl > 0)
-    RNPX_TRY_HIP(launch_batched<T>(1, nl, d_order, d_desc, A, tau, B, d_ctr, std::min(nl, h->num_cus), 0, st));
-  if (ns > 0)
-    RNPX_TRY_HIP(launch_batched<T>(0, ns, d_order + nl, d_desc, A, tau, B, d_ctr + 1, std::min(ns, 4 * h->num_cus),
-                                   kBatchLdsBytes, st));
+    RNPX_TRY_HIP(launch_batched<T>(1, nl, d_order, d_desc, A, tau, B, d_ctr, std::min(nl, h->num_cus), bs.blocked_min,
+                                   st));
+  if (ns > 0) {
+    const int per_cu = std::max(1, (160 * 1024) / bs.lds_bytes);
+    RNPX_TRY_HIP(launch_batched<T>(0, ns, d_order + nl, d_desc, A, tau, B, d_ctr + 1, std::min(ns, per_cu * h->num_cus),
+                                   bs.lds_bytes, st));
+  }
   return RNPX_OK;
 }
 

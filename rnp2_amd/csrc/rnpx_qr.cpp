@@ -442,6 +442,20 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   if (nrhs > 0 && ldb < std::max(1, m)) return -9;
   const int k = std::min(m, n);
   if (k == 0) return RNPX_OK;
+  // Automatic tuning (nb = 0): the autotuned table's entry nearest this
+  // shape, for this call only.
+  struct TuneRestore {
+    Handle* h;
+    rnpx_tuning t;
+    ~TuneRestore() { h->tune = t; }
+  } tune_restore{h, h->tune};
+  if (h->tune.nb == 0) {
+    int tnb = 0, tks = 0;
+    if (tuning_factor(S<T>::cplx ? 'z' : 'd', m, n, &tnb, &tks)) {
+      h->tune.nb = tnb;
+      if (h->tune.ksplit == 0) h->tune.ksplit = tks;
+    }
+  }
   hipStream_t st = h->stream;
   int nb = pick_nb<T>(h, m, n);
   // Every panel workgroup must be resident at once (they meet at grid

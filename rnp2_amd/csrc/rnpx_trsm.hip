@@ -135,7 +135,10 @@ __global__ __launch_bounds__(256) void trsm_tri_kernel(int n, const T* __restric
   }
   if (lane < bs) b[i0 + lane] = y;
   wait_vm_stores();
-  if (lane == 0) __hip_atomic_store(&fl[p], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0) {
+    release_agent();
+    __hip_atomic_store(&fl[p], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 }  // namespace
