@@ -81,6 +81,19 @@ int rnpx_dgeqrf(size_t m, size_t n, double *a, size_t lda, double *tau,
 int rnpx_zgeqrf(size_t m, size_t n, rnpx_zcomplex *a, size_t lda, rnpx_zcomplex *tau,
                 size_t *lwork, rnpx_zcomplex *work);
 
+/* Reflector::LastNonzeroColumnLimit / LastNonzeroRowLimit -- RNP/LA/Reflector.hpp:
+ * 36-75: one plus the index of the last column (row) of the m x n matrix a (host or
+ * device memory, lda >= m) that holds an entry other than zero (NaN counts, -0 does
+ * not), 0 when there is none.  The limits ApplyBlock and Apply trim to. */
+int rnpx_dlast_nonzero_column_limit(size_t m, size_t n, const double *a, size_t lda,
+                                    size_t *limit);
+int rnpx_zlast_nonzero_column_limit(size_t m, size_t n, const rnpx_zcomplex *a, size_t lda,
+                                    size_t *limit);
+int rnpx_dlast_nonzero_row_limit(size_t m, size_t n, const double *a, size_t lda,
+                                 size_t *limit);
+int rnpx_zlast_nonzero_row_limit(size_t m, size_t n, const rnpx_zcomplex *a, size_t lda,
+                                 size_t *limit);
+
 /* QR::MultQ<double> -- RNP/LA/QR.hpp:337-429 (dormqr). */
 int rnpx_dormqr(const char *side, const char *trans, size_t m, size_t n, size_t k,
                 const double *a, size_t lda, const double *tau, double *c, size_t ldc,
@@ -284,6 +297,12 @@ int rnpx_profile_read(rnpx_handle h, int kernel, long *launches, double *ms, dou
                       double *bytes);
 
 /* Device-pointer forms of Factor / MultQ / GenerateQ (same semantics). */
+/* Factorizations on h (NULL: the drop-in calls' handle) whose matrix held Inf or NaN:
+ * those follow the reference's own blocking (nb 32, crossover 128, RNP/LA/QR.hpp:
+ * 141-196) and every LastNonzero* trim it makes, since which 0 * Inf products the
+ * reference forms depends on them; finite matrices take the blocked fast path. */
+int rnpx_exact_path_count(rnpx_handle h, long *count);
+
 int rnpx_dgeqrf_dev(rnpx_handle h, size_t m, size_t n, double *a, size_t lda, double *tau);
 int rnpx_zgeqrf_dev(rnpx_handle h, size_t m, size_t n, rnpx_zcomplex *a, size_t lda,
                     rnpx_zcomplex *tau);

@@ -68,6 +68,10 @@ class ServerStats(ctypes.Structure):
 SIGNATURES = {
     "rnpx_dgeqrf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
     "rnpx_zgeqrf": (_i, [_sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
+    "rnpx_dlast_nonzero_column_limit": (_i, [_sz, _sz, _p, _sz, ctypes.POINTER(_sz)]),
+    "rnpx_zlast_nonzero_column_limit": (_i, [_sz, _sz, _p, _sz, ctypes.POINTER(_sz)]),
+    "rnpx_dlast_nonzero_row_limit": (_i, [_sz, _sz, _p, _sz, ctypes.POINTER(_sz)]),
+    "rnpx_zlast_nonzero_row_limit": (_i, [_sz, _sz, _p, _sz, ctypes.POINTER(_sz)]),
     "rnpx_dormqr": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
     "rnpx_zunmqr": (_i, [_s, _s, _sz, _sz, _sz, _p, _sz, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
     "rnpx_dorgqr": (_i, [_sz, _sz, _sz, _p, _sz, _p, ctypes.POINTER(_sz), _p]),
@@ -101,6 +105,7 @@ SIGNATURES = {
     "rnpx_profile_enable": (_i, [_p, _i]),
     "rnpx_profile_read": (_i, [_p, _i, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "rnpx_exact_path_count": (_i, [_p, ctypes.POINTER(ctypes.c_long)]),
     "rnpx_dgeqrf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_zgeqrf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
     "rnpx_dgelqf_dev": (_i, [_p, _sz, _sz, _p, _sz, _p]),
@@ -277,6 +282,30 @@ def factor(a):
     work = np.zeros(max(int(lw[0]), 1), dtype=a.dtype)
     _check(Factor(m, n, a, max(m, 1), tau, lw, work), "Factor")
     return a, tau[:k]
+
+
+def last_nonzero_limits(a):
+    """(LastNonzeroColumnLimit, LastNonzeroRowLimit) of a numpy matrix, on
+    the device -- Reflector.hpp:36-75."""
+    a = np.asfortranarray(a, dtype=np.result_type(a, np.float64))
+    m, n = a.shape
+    pre = _pre(a.dtype)
+    out = []
+    for which in ("column", "row"):
+        lim = _sz(0)
+        _check(getattr(lib(), f"rnpx_{pre}last_nonzero_{which}_limit")(m, n, _ptr(a), max(m, 1), ctypes.byref(lim)),
+               f"last_nonzero_{which}_limit")
+        out.append(lim.value)
+    return tuple(out)
+
+
+def exact_path_count(handle=None):
+    """Factorizations on the handle (None: the drop-in calls') whose matrix
+    held Inf or NaN and so took the reference's blocking and zero trims."""
+    c = ctypes.c_long(0)
+    _check(lib().rnpx_exact_path_count(handle.h if handle is not None else None, ctypes.byref(c)),
+           "exact_path_count")
+    return c.value
 
 
 def multq(side, trans, afac, tau, c, k=None):

@@ -28,6 +28,7 @@ SOURCES = [
     "rnpx_batched.hip",
     "rnpx_mixed.hip",
     "rnpx_noalloc.hip",
+    "rnpx_exact.hip",
     "rnpx_qr.cpp", "rnpx_qlrq.cpp", "rnpx_hess.cpp", "rnpx_lu_driver.cpp",
     "rnpx_lq.cpp",
     "rnpx_ls.cpp",

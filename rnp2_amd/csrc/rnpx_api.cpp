@@ -87,11 +87,36 @@ int dropin_geqrf(size_t m, size_t n, T* a, size_t lda, T* tau, size_t* lwork) {
   Arg<T> A, TAU;
   if ((st = A.in(h, h->stage1, a, m, n, lda, true))) return st;
   if ((st = TAU.in(h, h->stage2, tau, k, 1, k, false))) return st;
+  // the workspace decides the reference's block width, which the path for
+  // matrices holding Inf or NaN follows
+  h->exact_lwork = *lwork;
   st = geqrf_dev<T>(h, (int)m, (int)n, A.dev, A.ld, TAU.dev);
+  h->exact_lwork = 0;
   if (st) return st;
   if ((st = A.out(h, lda))) return st;
   if ((st = TAU.out(h, k))) return st;
   return sync_handle(h);
+}
+
+// Reflector::LastNonzeroColumnLimit (which 0) / LastNonzeroRowLimit (which 1),
+// RNP/LA/Reflector.hpp:36-75.
+template <typename T>
+int dropin_last_nonzero(int which, size_t m, size_t n, const T* a, size_t lda, size_t* limit) {
+  if (!limit) return -5;
+  *limit = 0;
+  if (m == 0 || n == 0) return RNPX_OK;
+  if (lda < m) return -4;
+  if (!a) return -3;
+  if (!fits_int(m) || !fits_int(n)) return RNPX_ERR_UNSUPPORTED;
+  int st = 0;
+  Handle* h = default_handle(&st);
+  if (!h) return st;
+  Arg<T> A;
+  if ((st = A.in(h, h->stage1, const_cast<T*>(a), m, n, lda, true))) return st;
+  int out = 0;
+  if ((st = last_nonzero_limit<T>(h, which, (int)m, (int)n, A.dev, A.ld, &out))) return st;
+  *limit = (size_t)out;
+  return RNPX_OK;
 }
 
 template <typename T>
@@ -553,6 +578,19 @@ int rnpx_zgeqrf(size_t m, size_t n, rnpx_zcomplex* a, size_t lda, rnpx_zcomplex*
   return dropin_geqrf<zd>(m, n, reinterpret_cast<zd*>(a), lda, reinterpret_cast<zd*>(tau), lwork);
 }
 
+int rnpx_dlast_nonzero_column_limit(size_t m, size_t n, const double* a, size_t lda, size_t* limit) {
+  return dropin_last_nonzero<double>(0, m, n, a, lda, limit);
+}
+int rnpx_zlast_nonzero_column_limit(size_t m, size_t n, const rnpx_zcomplex* a, size_t lda, size_t* limit) {
+  return dropin_last_nonzero<zd>(0, m, n, reinterpret_cast<const zd*>(a), lda, limit);
+}
+int rnpx_dlast_nonzero_row_limit(size_t m, size_t n, const double* a, size_t lda, size_t* limit) {
+  return dropin_last_nonzero<double>(1, m, n, a, lda, limit);
+}
+int rnpx_zlast_nonzero_row_limit(size_t m, size_t n, const rnpx_zcomplex* a, size_t lda, size_t* limit) {
+  return dropin_last_nonzero<zd>(1, m, n, reinterpret_cast<const zd*>(a), lda, limit);
+}
+
 int rnpx_dormqr(const char* side, const char* trans, size_t m, size_t n, size_t k, const double* a, size_t lda,
                 const double* tau, double* c, size_t ldc, size_t* lwork, double* work) {
   return dropin_ormqr<double>(side, trans, m, n, k, a, lda, tau, c, ldc, lwork,
@@ -762,6 +800,14 @@ int rnpx_synchronize(rnpx_handle h) {
   return sync_handle(H(h));
 }
 size_t rnpx_workspace_bytes(rnpx_handle h) { return h ? H(h)->held_bytes() : 0; }
+int rnpx_exact_path_count(rnpx_handle h, long* count) {
+  if (!count) return -2;
+  int st = 0;
+  Handle* hh = h ? H(h) : default_handle(&st);
+  if (!hh) return st;
+  *count = hh->exact_calls;
+  return RNPX_OK;
+}
 int rnpx_profile_enable(rnpx_handle h, int on) {
   if (!h) return -1;
   prof_reset(H(h));

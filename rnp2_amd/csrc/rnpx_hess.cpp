@@ -17,12 +17,6 @@
 
 namespace rnpx {
 
-#define RNPX_TRY(x)                               \
-  do {                                            \
-    hipError_t _e = (x);                          \
-    if (_e != hipSuccess) return hip_status(_e);  \
-  } while (0)
-
 template <typename T>
 int ormhr_dev(Handle* h, char side, char trans, int m, int n, int ilo, int ihi, const T* A, long lda, const T* tau,
               T* C, long ldc) {

@@ -59,17 +59,21 @@ struct Handle {
   Buffer lubuf;                            // LU panel scratch, Solve's permutation and gather
   Buffer rfbuf;                            // RefineSolution's residual and |op(A)||x| + |b|
   Buffer invbuf;                           // LU::Invert's n x n right-hand side and U's diagonal
+  Buffer exbuf;                            // non-finite flag, trim limits, T and work of the exact path
+  Buffer exsave;                           // the first block's columns while the rest is checked
   unsigned trsm_epoch = 0;                 // value the flags take in the current solve
   // Every buffer above, for accounting and release.
   std::vector<Buffer*> buffers() {
     return {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc,
             &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
-            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf, &invbuf};
+            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf, &invbuf, &exbuf, &exsave};
   }
   hipError_t sync_all_streams();       // the handle's stream, panel stream and partition streams
   void* batch = nullptr;               // batched-call descriptor buffers
   void (*batch_free)(void*) = nullptr;
   int update_prec = 0;                 // 1: trailing-update products on FP32 MFMA
+  size_t exact_lwork = 0;              // the workspace the reference's Factor was given (0: optimal)
+  long exact_calls = 0;                // factorizations that took the non-finite-input path
   bool prof = false;
   unsigned prof_mask = ~0u;  // kernel ids (bit per KernelId) that are timed
   std::vector<ProfRec> recs;
@@ -86,6 +90,19 @@ struct Handle {
 };
 
 int hip_status(hipError_t e);
+
+// Return from the enclosing int-status function on a HIP error (RNPX_TRY) or
+// on a nonzero status (RNPX_CHECK).
+#define RNPX_TRY(x)                               \
+  do {                                            \
+    hipError_t _e = (x);                          \
+    if (_e != hipSuccess) return hip_status(_e);  \
+  } while (0)
+#define RNPX_CHECK(x)        \
+  do {                       \
+    const int _s = (x);      \
+    if (_s != 0) return _s;  \
+  } while (0)
 int create_handle(Handle** out, size_t budget);
 Handle* default_handle(int* status);
 int sync_handle(Handle* h);
@@ -96,6 +113,15 @@ void prof_reset(Handle* h);
 // applied to B right after its trailing update (the least-squares drivers).
 template <typename T>
 int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B = nullptr, int nrhs = 0, long ldb = 0);
+// Inf/NaN anywhere in the m x n matrix: checked on stream st, which is then
+// synchronized.
+template <typename T> int any_nonfinite(Handle* h, int m, int n, const T* A, long lda, bool* out, hipStream_t st);
+// Reflector::LastNonzeroColumnLimit (which 0) / LastNonzeroRowLimit (which 1)
+// of a device matrix, read back (synchronous).
+template <typename T> int last_nonzero_limit(Handle* h, int which, int m, int n, const T* a, long lda, int* out);
+// QR::Factor with the reference's blocking and zero trims (input holding
+// Inf or NaN); lwork as given to the reference (0: optimal).
+template <typename T> int geqrf_exact(Handle* h, int m, int n, T* A, long lda, T* tau, size_t lwork);
 template <typename T> int gels_dev(Handle* h, int m, int n, int nrhs, T* A, long lda, T* tau, T* B, long ldb);
 int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb, double* X,
                    long ldx, int max_iter, double tol, int* iters, double* berr);

@@ -22,12 +22,6 @@
 
 namespace rnpx {
 
-#define RNPX_TRY(x)                               \
-  do {                                            \
-    hipError_t _e = (x);                          \
-    if (_e != hipSuccess) return hip_status(_e);  \
-  } while (0)
-
 // LQ::Factor (LQ.hpp:126-190).
 template <typename T>
 int gelqf_dev(Handle* h, int m, int n, T* A, long lda, T* tau) {

@@ -18,12 +18,6 @@
 
 namespace rnpx {
 
-#define RNPX_TRY(x)                               \
-  do {                                            \
-    hipError_t _e = (x);                          \
-    if (_e != hipSuccess) return hip_status(_e);  \
-  } while (0)
-
 // Triangular solve op(M) X = B in place on B, M the upper ('U') or lower
 // ('L') triangle of R (n x n), op 'N', 'T' or 'C', unit diagonal for diag
 // 'U'.  Each call takes a new flag epoch, so a solve is not replayable from

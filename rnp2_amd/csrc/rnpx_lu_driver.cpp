@@ -19,12 +19,6 @@
 
 namespace rnpx {
 
-#define RNPX_TRY(x)                               \
-  do {                                            \
-    hipError_t _e = (x);                          \
-    if (_e != hipSuccess) return hip_status(_e);  \
-  } while (0)
-
 template <typename T>
 int getrf_dev(Handle* h, int m, int n, T* A, long lda, int* piv_host, int* info) {
   if (m < 0) return -1;

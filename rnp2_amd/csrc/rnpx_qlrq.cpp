@@ -29,12 +29,6 @@
 
 namespace rnpx {
 
-#define RNPX_TRY(x)                               \
-  do {                                            \
-    hipError_t _e = (x);                          \
-    if (_e != hipSuccess) return hip_status(_e);  \
-  } while (0)
-
 namespace {
 
 // The k reflectors of a MultQ call reversed into the handle's qlbuf (rows x

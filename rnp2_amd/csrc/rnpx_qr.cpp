@@ -165,12 +165,6 @@ int hip_status(hipError_t e) {
   return RNPX_ERR_HIP;
 }
 
-#define RNPX_TRY(x)                             \
-  do {                                          \
-    hipError_t _e = (x);                        \
-    if (_e != hipSuccess) return hip_status(_e); \
-  } while (0)
-
 int Handle::check_gpu() {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return RNPX_ERR_NOGPU;
@@ -476,6 +470,23 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   const int nob = (k + nbo - 1) / nbo;    // outer blocks
   const bool look = (h->tune.lookahead >= 0) && nob > 1;
   const long ldv = m, ldt = nbo;
+  // Inf or NaN in A: the reference's blocking and zero trims decide which
+  // 0 * Inf products it forms, so such a matrix is factored their way.  With
+  // look-ahead only the first block's columns are checked up front; the
+  // rest is checked while the first panel runs, from a copy of those
+  // columns that is put back if the factorization has to start over.
+  auto factor_nonfinite = [&]() -> int {
+    RNPX_CHECK(geqrf_exact<T>(h, m, n, A, lda, tau, h->exact_lwork));
+    if (nrhs > 0) RNPX_CHECK(ormqr_dev<T>(h, 'L', 'C', m, nrhs, k, A, lda, tau, B, ldb));
+    return RNPX_OK;
+  };
+  const int w0 = std::min(nbo, k);
+  const bool split_scan = look && n > w0;
+  {
+    bool bad = false;
+    RNPX_CHECK(any_nonfinite<T>(h, m, split_scan ? w0 : n, A, lda, &bad, st));
+    if (bad) return factor_nonfinite();
+  }
   Buffer* vb[2] = {&h->vbuf, &h->vbuf2};
   Buffer* tb[2] = {&h->tbuf, &h->tbuf2};
   for (int q = 0; q < (look ? 2 : 1); ++q) {
@@ -650,11 +661,28 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   }();
   const GemmWork panel_work = {ps, &h->wbuf2, &h->zbuf2, &h->part2, false};
   const GemmWork upd_work = {us, &h->wbuf, &h->zbuf, &h->part, true};
+  T* save = nullptr;
+  if (split_scan) {
+    RNPX_TRY(h->exsave.ensure(sizeof(T) * (size_t)m * w0, h));
+    save = (T*)h->exsave.p;
+    RNPX_TRY(hipMemcpy2DAsync(save, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, w0, hipMemcpyDeviceToDevice, st));
+  }
   RNPX_TRY(hipEventRecord(h->ev_start, st));
   RNPX_TRY(hipStreamWaitEvent(ps, h->ev_start, 0));
   if (us != st) RNPX_TRY(hipStreamWaitEvent(us, h->ev_start, 0));
   if ((s = panel_phase(0, ps, &panel_work))) return s;
   RNPX_TRY(hipEventRecord(h->ev_pan[0], ps));
+  if (split_scan) {
+    // the columns right of the first block (untouched until its update)
+    bool bad = false;
+    RNPX_CHECK(any_nonfinite<T>(h, m, n - w0, A + (size_t)w0 * lda, lda, &bad, st));
+    if (bad) {
+      RNPX_TRY(hipStreamSynchronize(ps));
+      RNPX_TRY(hipMemcpy2DAsync(A, sizeof(T) * lda, save, sizeof(T) * m, sizeof(T) * m, w0, hipMemcpyDeviceToDevice,
+                                st));
+      return factor_nonfinite();
+    }
+  }
   for (int b = 0; b < nob; ++b) {
     RNPX_TRY(hipStreamWaitEvent(us, h->ev_pan[b & 1], 0));
     const int c0 = b * nbo + std::min(nbo, k - b * nbo);  // first column after block b

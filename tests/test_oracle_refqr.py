@@ -281,3 +281,24 @@ def test_reference_blocked_lu_invert_defect():
     ref = np.linalg.inv(a)
     assert np.max(np.abs(u_inv - ref)) <= 1e-10 * np.max(np.abs(ref))
     assert np.max(np.abs(d_inv - ref)) > 1e-3 * np.max(np.abs(ref))
+
+
+# ---------------------------------------------------------------------------
+# Inf / NaN in the rows the reference's zero trims skip: the oracle's
+# restatement of those trims (Reflector.hpp:36-75, 196-221, 581-623, 742-744)
+# against the reference built from its sources.
+# ---------------------------------------------------------------------------
+import nonfinite_cases as NF  # noqa: E402
+
+
+@live
+@pytest.mark.parametrize("case,seed", NF.cases(), ids=lambda c: c[0] if isinstance(c, tuple) else str(c))
+def test_factor_nonfinite_live(case, seed):
+    tag, m, n, k0, specials, zc, cplx, lwn = case
+    a = NF.build(m, n, k0, specials, zc, cplx, seed)
+    refqr.set_threads(1)
+    lw = lwn * n if lwn else 0
+    af_r, tau_r = refqr.factor(a, lwork_cap=lw) if lw else refqr.factor(a)
+    af_o, tau_o = O.factor(a, lwork=lw) if lw else O.factor(a)
+    assert not np.isfinite(af_r).all(), "the case must reach a non-finite result"
+    NF.compare(af_o, tau_o, af_r, tau_r)
