@@ -59,8 +59,8 @@ typedef struct rnpx_batched_plan_s *rnpx_batched_plan;
 /* Tuning hook: the counterpart of specialising QR::Tuning<T>
  * (RNP/LA/QR.hpp:41-51).  Zero fields keep the built-in choice. */
 typedef struct rnpx_tuning {
-	int nb;           /* outer block width: 32, 64, 128 (0 = the autotuned table's
-	                     entry for the shape, else 128 with 64-column panels) */
+	int nb;           /* outer block width: 32, 64, 128, 256 (0 = the autotuned
+	                     table's entry for the shape, else 128 with 64-column panels) */
 	int panel_rpt;    /* panel rows per thread: 1, 2, 4 */
 	int ksplit;       /* split-K slices for W = V^H C (0 = automatic) */
 	int lookahead;    /* 1 = overlap the next panel with the trailing update */

@@ -4,7 +4,7 @@ the path given), the device counterpart of RNP's QR::Tuning hooks
 (RNP/LA/QR.hpp:41-51), which fix nb = 32 for every shape.
 
 Factor: for each shape and element type, every outer block width in
-{32, 64, 128} (the width of the compact-WY block the trailing matrix sees)
+{32, 64, 128, 256} (the width of the compact-WY block the trailing matrix sees)
 and W = V^H C split factor in the candidates, on RNP Normal01 input resident
 in HBM, best of `reps` timed runs after a warm-up; the fastest is recorded.
 Batched: for the batched configuration's distribution (m log-uniform in
@@ -25,7 +25,7 @@ from . import lib, Handle, colmajor_empty, make_batch, device_name
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PATH = os.path.join(HERE, "rnpx_tuning.json")
-WIDTHS = (32, 64, 128)
+WIDTHS = (32, 64, 128, 256)
 KSPLITS = (0, 4, 8, 16)
 BATCH_LDS = (20480, 40960, 81920)
 BATCH_BLOCKED_MIN = (17, 33, 65)

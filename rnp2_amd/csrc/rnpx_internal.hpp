@@ -61,12 +61,13 @@ struct Handle {
   Buffer invbuf;                           // LU::Invert's n x n right-hand side and U's diagonal
   Buffer exbuf;                            // non-finite flag, trim limits, T and work of the exact path
   Buffer exsave;                           // the first block's columns while the rest is checked
+  Buffer rinv;                             // mixed LS: R^{-1}, its recursion's products and step vectors
   unsigned trsm_epoch = 0;                 // value the flags take in the current solve
   // Every buffer above, for accounting and release.
   std::vector<Buffer*> buffers() {
     return {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc,
             &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
-            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf, &invbuf, &exbuf, &exsave};
+            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf, &invbuf, &exbuf, &exsave, &rinv};
   }
   hipError_t sync_all_streams();       // the handle's stream, panel stream and partition streams
   void* batch = nullptr;               // batched-call descriptor buffers

@@ -354,6 +354,8 @@ hipError_t launch_trsm_blocked(char trans, int n, int nrhs, const T* R, long ldr
 
 template <typename T>
 hipError_t launch_trsm_un(int n, int nrhs, const T* R, long ldr, T* B, long ldb, hipStream_t st);
+// Inverses of the 64 x 64 diagonal blocks of upper-triangular R into X.
+hipError_t launch_trtri_diag(int n, const double* R, long ldr, double* X, long ldx, hipStream_t st);
 
 // Mixed precision: FP64 operands, FP32 MFMA product (FP32 accumulation),
 // FP64 epilogue; split-K slices reduced in FP64.

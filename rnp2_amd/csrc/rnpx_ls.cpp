@@ -13,6 +13,7 @@
 #include "rnpx_internal.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <numeric>
 #include <vector>
 
@@ -138,6 +139,30 @@ int ls_berr_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, co
 // every column), or ||d|| stops halving, or max_iter corrections; then eta
 // of the final x is checked against tol.  A and B are not modified; X
 // (n x nrhs) receives the solution.
+// X = R^{-1} for the upper-triangular n x n R (X upper triangular, zero
+// below): the 64 x 64 diagonal blocks inverted in LDS, then bottom-up
+// [X11 X12; 0 X22] with X12 = -X11 (R12 X22) on the MFMA GEMM.  tmp holds
+// (n/2 + 64)^2 doubles.
+static int trtri_upper(Handle* h, int n, const double* R, long ldr, double* X, long ldx, double* tmp) {
+  hipStream_t st = h->stream;
+  RNPX_TRY(launch_set<double>(n, n, 0.0, 0.0, X, ldx, st));
+  RNPX_TRY(launch_trtri_diag(n, R, ldr, X, ldx, st));
+  std::function<int(int, int)> join = [&](int i0, int sz) -> int {
+    if (sz <= 64) return RNPX_OK;
+    const int s1 = ((sz / 2 + 63) / 64) * 64, s2 = sz - s1;
+    RNPX_CHECK(join(i0, s1));
+    RNPX_CHECK(join(i0 + s1, s2));
+    const double* R12 = R + (size_t)i0 + (size_t)(i0 + s1) * ldr;
+    const double* X11 = X + (size_t)i0 + (size_t)i0 * ldx;
+    const double* X22 = X + (size_t)(i0 + s1) + (size_t)(i0 + s1) * ldx;
+    double* X12 = X + (size_t)i0 + (size_t)(i0 + s1) * ldx;
+    RNPX_TRY(gemm<double>('N', 'N', s1, s2, s2, 1.0, R12, ldr, X22, ldx, 0.0, tmp, s1, 1, nullptr, st));
+    RNPX_TRY(gemm<double>('N', 'N', s1, s2, s1, -1.0, X11, ldx, tmp, s1, 0.0, X12, ldx, 1, nullptr, st));
+    return RNPX_OK;
+  };
+  return join(0, n);
+}
+
 int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb, double* X,
                    long ldx, int max_iter, double tol, int* iters, double* berr) {
   if (m < 0) return -2;
@@ -161,8 +186,17 @@ int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda,
   int s = geqrf_dev<double>(h, m, n, Af, m, tau, Bq, nrhs, m);
   h->update_prec = 0;
   if (s) return s;
-  if ((s = trsm_dev<double>(h, 'N', n, nrhs, Af, m, Bq, m))) return s;
-  RNPX_TRY(launch_copy<double>(n, nrhs, Bq, m, X, ldx, st));
+  // R^{-1} once: the refinement's two triangular solves per step become two
+  // products (its error, kappa(R) eps, is far below the FP32 updates' that
+  // the refinement corrects)
+  const size_t half = (size_t)(n / 2 + 64);
+  RNPX_TRY(h->rinv.ensure(sizeof(double) * ((size_t)n * n + half * half + (size_t)n * nrhs), h));
+  double* Ri = (double*)h->rinv.p;
+  double* tmp = Ri + (size_t)n * n;
+  double* Tn = tmp + half * half;
+  RNPX_CHECK(trtri_upper(h, n, Af, m, Ri, n, tmp));
+  // x0 = R^{-1} (Q^T b)(0:n)
+  RNPX_TRY(gemm<double>('N', 'N', n, nrhs, n, 1.0, Ri, n, Bq, m, 0.0, X, ldx, 1, nullptr, st));
   LsScratch sc;
   if ((s = ls_scratch(h, m, n, nrhs, &sc))) return s;
   double a2 = -1.0, eta = 0.0, prev = 0.0;
@@ -172,8 +206,8 @@ int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda,
     if ((s = ls_residual(h, m, n, nrhs, A, lda, B, ldb, X, ldx, sc, &a2, &eta))) return s;
     if (it >= max_iter) break;
     // d = R^{-1} R^{-T} s  (s is in sc.S, n x nrhs), x += d
-    if ((s = trsm_dev<double>(h, 'C', n, nrhs, Af, m, sc.S, n))) return s;
-    if ((s = trsm_dev<double>(h, 'N', n, nrhs, Af, m, sc.S, n))) return s;
+    RNPX_TRY(gemm<double>('T', 'N', n, nrhs, n, 1.0, Ri, n, sc.S, n, 0.0, Tn, n, 1, nullptr, st));
+    RNPX_TRY(gemm<double>('N', 'N', n, nrhs, n, 1.0, Ri, n, Tn, n, 0.0, sc.S, n, 1, nullptr, st));
     RNPX_TRY(launch_axpy_mat(n, nrhs, sc.S, n, X, ldx, st));
     RNPX_TRY(launch_colnorm2(n, nrhs, sc.S, n, sc.nrm, st));
     RNPX_TRY(launch_colnorm2(n, nrhs, X, ldx, sc.nrm + nrhs, st));

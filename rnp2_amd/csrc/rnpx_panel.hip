@@ -147,7 +147,10 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
   };
   auto barrier = [&]() {
     ++epoch;
-    grid_barrier(p.counter, p.abortf, epoch * (unsigned)G);
+    if (G == 1)
+      __syncthreads();  // one workgroup: its own partials, no grid hand-off
+    else
+      grid_barrier(p.counter, p.abortf, epoch * (unsigned)G);
     --epoch;  // gather() reads the slot of the epoch just completed
   };
 

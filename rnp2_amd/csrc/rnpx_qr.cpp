@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <vector>
 
@@ -352,7 +353,9 @@ static bool use_cqr(const Handle* h, int mp, int ib, int nb, int* g, int* rw) {
   if constexpr (S<T>::cplx) {
     return false;
   } else {
-    if (h->tune.panel == 1 || nb != 64 || ib != 64) return false;
+    // Panels of at most 256 rows: one workgroup of the column-wise kernel
+    // (no grid hand-offs) is faster than the CholeskyQR2 passes.
+    if (h->tune.panel == 1 || nb != 64 || ib != 64 || (mp <= 256 && h->tune.panel != 2)) return false;
     *g = cqr_panel_groups(mp, std::min(kCqrMaxGroups, h->panel_cap > 0 ? h->panel_cap : h->num_cus), rw);
     return *g > 0;
   }
@@ -421,6 +424,10 @@ static int panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, 
 //   V = [V1 | (0; V2)],  T = [T1, -T1 (V1^H V2) T2; 0, T2],
 // so that the trailing matrix sees one rank-nbo update (K = 128 GEMMs run
 // half as many passes over it at higher arithmetic intensity than K = 64).
+// With nb = 256 tuning, blocks are 256 columns (two 128-column units merged
+// the same way, K = 256 updates) while the remaining height exceeds
+// RNPX_WIDE_ROWS (4096), and 128 columns below it, where the chain of panels
+// is the longer one.
 // With look-ahead (the default) the panel phase of block b+1 runs on a
 // second stream while the trailing update of block b runs: the update of
 // block b is split into block b+1's columns, applied on the panel stream
@@ -465,9 +472,26 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
     G0 = (m + 256 * rpt0 - 1) / (256 * rpt0);
   }
   if (G0 > gcap) return RNPX_ERR_UNSUPPORTED;
-  const int nbo = (h->tune.nb == 0 || h->tune.nb == 128) ? 2 * nb : nb;
+  const int nbo = (h->tune.nb == 256 ? 4 : (h->tune.nb == 0 || h->tune.nb == 128) ? 2 : 1) * nb;
+  // Outer blocks: nbo columns while the remaining height exceeds wide_rows
+  // (where the trailing products are the longer chain and K = nbo runs them
+  // faster), 2 nb columns below it (where the panels are the longer chain).
+  static const int wide_rows = [] {
+    const char* e = getenv("RNPX_WIDE_ROWS");
+    return e ? atoi(e) : 4096;
+  }();
+  std::vector<int> bstart;
+  {
+    const int tailw = std::min(nbo, 2 * nb);
+    for (int i = 0; i < k;) {
+      bstart.push_back(i);
+      i += std::min(k - i, (m - i > wide_rows) ? nbo : tailw);
+    }
+    bstart.push_back(k);
+  }
+  auto bw = [&](int b) { return bstart[b + 1] - bstart[b]; };
   const int npan = (k + nb - 1) / nb;     // inner panels
-  const int nob = (k + nbo - 1) / nbo;    // outer blocks
+  const int nob = (int)bstart.size() - 1;  // outer blocks
   const bool look = (h->tune.lookahead >= 0) && nob > 1;
   const long ldv = m, ldt = nbo;
   // Inf or NaN in A: the reference's blocking and zero trims decide which
@@ -480,7 +504,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
     if (nrhs > 0) RNPX_CHECK(ormqr_dev<T>(h, 'L', 'C', m, nrhs, k, A, lda, tau, B, ldb));
     return RNPX_OK;
   };
-  const int w0 = std::min(nbo, k);
+  const int w0 = bw(0);
   const bool split_scan = look && n > w0;
   {
     bool bad = false;
@@ -500,7 +524,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   RNPX_TRY(h->sync.ensure(sizeof(unsigned) * (size_t)sync_words, h));
   // Outer blocks in one launch (real, 64-column panels, 128-column blocks,
   // tuning panel 0): buffers for the block kernel.
-  const bool use_block = !S<T>::cplx && h->tune.panel == 0 && nb == 64 && nbo == 2 * nb &&
+  const bool use_block = !S<T>::cplx && h->tune.panel == 0 && nb == 64 && nbo % (2 * nb) == 0 &&
                          cqr_block_groups(m, std::min(256, h->num_cus)) > 0;
   static_assert(2 * kCqrSyncWords >= kCqrBlockSyncWords, "a block uses its two panels' sync words");
   if (use_block) {
@@ -517,28 +541,41 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
       RNPX_TRY(h->cqr_scratch.ensure(sizeof(double) * kCqrScratch, h));
     }
   }
-  RNPX_TRY(h->misc2.ensure(sizeof(T) * (size_t)3 * nb * nb, h));
+  RNPX_TRY(h->misc2.ensure(sizeof(T) * (size_t)std::max(3 * nb * nb, 2 * (nbo / 2) * (nbo / 2)), h));
   RNPX_TRY(launch_zero_u32((unsigned*)h->sync.p, sync_words, st));
-  if (nbo > nb && k > nb) {
-    // Parts of the merged V and T no panel writes: V's rows above the second
-    // panel, T's block below the diagonal (m >= k > nb here).
-    for (int q = 0; q < (look ? 2 : 1); ++q) {
-      RNPX_TRY(launch_set<T>(nb, nb, S<T>::zero(), S<T>::zero(), (T*)vb[q]->p + (size_t)nb * ldv, ldv, st));
-      RNPX_TRY(launch_set<T>(nb, nb, S<T>::zero(), S<T>::zero(), (T*)tb[q]->p + nb, ldt, st));
-    }
+  // An outer block of w columns is split into w1 = 2 nb (a block-kernel
+  // unit) or nb columns and the rest.  The parts of the merged V and T no
+  // panel writes -- V's rows above the second part, T's block below the
+  // diagonal -- are zero; they are cleared once, for the full width and the
+  // last block's, in both buffers.
+  auto split_of = [&](int w) { return w > 2 * nb ? 2 * nb : nb; };
+  std::function<int(int, long, T*, T*)> zero_parts = [&](int w, long off, T* Vq, T* Tq) -> int {
+    if (w <= nb) return RNPX_OK;
+    const int w1 = split_of(w), w2 = w - w1;
+    RNPX_TRY(launch_set<T>(w1, w2, S<T>::zero(), S<T>::zero(), Vq + off + (off + w1) * ldv, ldv, st));
+    RNPX_TRY(launch_set<T>(w2, w1, S<T>::zero(), S<T>::zero(), Tq + (off + w1) + off * ldt, ldt, st));
+    RNPX_CHECK(zero_parts(w1, off, Vq, Tq));
+    return zero_parts(w2, off + w1, Vq, Tq);
+  };
+  if (k > nb) {
+    std::vector<int> widths;
+    for (int b = 0; b < nob; ++b)
+      if (std::find(widths.begin(), widths.end(), bw(b)) == widths.end()) widths.push_back(bw(b));
+    for (int q = 0; q < (look ? 2 : 1); ++q)
+      for (int w : widths) RNPX_CHECK(zero_parts(w, 0, (T*)vb[q]->p, (T*)tb[q]->p));
   }
 
-  // The panel phase of outer block b (columns [i, i + wo)) on stream ps.
-  auto panel_phase = [&](int b, hipStream_t ps, const GemmWork* gw) -> int {
-    const int i = b * nbo, wo = std::min(nbo, k - i);
-    const int q = look ? (b & 1) : 0;
-    T* Vo = (T*)vb[q]->p;
-    T* To = (T*)tb[q]->p;
-    hipEvent_t ev;
-    h->prof_begin(K_PANEL, &ev, ps);
+  // Columns [i, i + w) (rows [i, m)) factored into V (row 0 = row i,
+  // leading dimension ldv) and T (ldt) on stream ps: one block-kernel launch
+  // for a real 2 nb-column unit, one panel for nb columns or fewer, else the
+  // first part, its reflectors applied to the rest's columns, the rest, and
+  // the two compact-WY forms merged,
+  //   V = [V1 | (0; V2)],  T = [T1, -T1 (V1^H V2) T2; 0, T2].
+  std::function<int(int, int, T*, T*, hipStream_t, const GemmWork*)> phase;
+  phase = [&](int i, int w, T* Vp, T* Tp, hipStream_t ps, const GemmWork* gw) -> int {
     if constexpr (!S<T>::cplx) {
-      const int Gb = use_block ? cqr_block_groups(m - i, std::min(256, h->num_cus)) : 0;
-      if (wo == 2 * nb && Gb > 0) {
+      const int Gb = (use_block && w == 2 * nb) ? cqr_block_groups(m - i, std::min(256, h->num_cus)) : 0;
+      if (Gb > 0) {
         // Both panels, the apply between them and the merge in one launch.
         const int p1 = i / nb;
         CqrBlockArgs ba;
@@ -546,9 +583,9 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
         ba.A = A + (size_t)i + (size_t)i * lda;
         ba.lda = lda;
         ba.tau = tau + i;
-        ba.V = Vo;
+        ba.V = Vp;
         ba.ldv = ldv;
-        ba.T = To;
+        ba.T = Tp;
         ba.ldt = ldt;
         ba.part = (double*)h->cqr_part.p;
         ba.gsum = (double*)h->cqr_gsum.p;
@@ -556,47 +593,47 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
         ba.sync = (unsigned*)h->sync.p + 1 + 2 * npan + (size_t)kCqrSyncWords * p1;  // panels p1 and p1+1
         ba.abortf = (unsigned*)h->flag.p;
         ba.fail = (unsigned*)h->sync.p + 1 + npan + p1;
-        RNPX_TRY(launch_cqr_block(ba, Gb, ps));
-        const double mpd = m - i;
-        const double fl = 2.0 * mpd * wo * wo - 2.0 / 3.0 * wo * wo * wo;
-        h->prof_end(K_PANEL, ev, fl, sizeof(T) * 3.0 * mpd * wo, ps);
-        return RNPX_OK;
+        return hip_status(launch_cqr_block(ba, Gb, ps));
       }
     }
-    const int ib1 = std::min(nb, wo);
-    int s = panel_step<T>(h, i / nb, npan, i, ib1, m, nb, A, lda, tau, Vo, ldv, To, ldt, ps);
-    if (s) return s;
-    double fl = (S<T>::cplx ? 4.0 : 1.0) * (2.0 * (m - i) * ib1 * ib1 - 2.0 / 3.0 * ib1 * ib1 * ib1);
-    if (wo > nb) {
-      const int i2 = i + nb, ib2 = wo - nb;
-      // panel 1's reflectors applied to panel 2's columns
-      if ((s = apply_block_dev<T>(h, 'L', 'C', m - i, ib2, ib1, Vo, ldv, To, ldt, A + (size_t)i + (size_t)i2 * lda,
-                                  lda, gw)))
-        return s;
-      T* V2 = Vo + (size_t)nb + (size_t)nb * ldv;
-      T* T2 = To + (size_t)nb + (size_t)nb * ldt;
-      if ((s = panel_step<T>(h, i2 / nb, npan, i2, ib2, m, nb, A, lda, tau, V2, ldv, T2, ldt, ps))) return s;
-      // T12 = -T1 (V1^H V2) T2, V1^H V2 over the rows of V2
-      const T one = S<T>::one(), zero = S<T>::zero(), mone = S<T>::zero() - S<T>::one();
-      T* Sm = (T*)h->misc2.p;
-      T* Xm = Sm + (size_t)nb * nb;
-      const int rows2 = m - i2;
-      const int ks = std::max(1, std::min(32, rows2 / 256));
-      RNPX_TRY(gw->part->ensure(sizeof(T) * (size_t)ks * nb * nb, h));
-      RNPX_TRY(gemm<T>('C', 'N', ib1, ib2, rows2, one, Vo + nb, ldv, V2, ldv, zero, Sm, nb, ks, (T*)gw->part->p, ps));
-      RNPX_TRY(gemm<T>('N', 'N', ib1, ib2, ib1, one, To, ldt, Sm, nb, zero, Xm, nb, 1, nullptr, ps));
-      RNPX_TRY(gemm<T>('N', 'N', ib1, ib2, ib2, mone, Xm, nb, T2, ldt, zero, To + (size_t)nb * ldt, ldt, 1, nullptr,
-                       ps));
-      fl += (S<T>::cplx ? 4.0 : 1.0) * (2.0 * rows2 * ib2 * ib2 - 2.0 / 3.0 * ib2 * ib2 * ib2);
-    }
-    h->prof_end(K_PANEL, ev, fl, sizeof(T) * 3.0 * (m - i) * wo, ps);
+    if (w <= nb) return panel_step<T>(h, i / nb, npan, i, w, m, nb, A, lda, tau, Vp, ldv, Tp, ldt, ps);
+    const int w1 = split_of(w), w2 = w - w1;
+    RNPX_CHECK(phase(i, w1, Vp, Tp, ps, gw));
+    // the first part's reflectors applied to the rest's columns
+    RNPX_CHECK(apply_block_dev<T>(h, 'L', 'C', m - i, w2, w1, Vp, ldv, Tp, ldt,
+                                  A + (size_t)i + (size_t)(i + w1) * lda, lda, gw));
+    T* V2 = Vp + (size_t)w1 + (size_t)w1 * ldv;
+    T* T2 = Tp + (size_t)w1 + (size_t)w1 * ldt;
+    RNPX_CHECK(phase(i + w1, w2, V2, T2, ps, gw));
+    // T12 = -T1 (V1^H V2) T2, V1^H V2 over the rows of V2
+    const T one = S<T>::one(), zero = S<T>::zero(), mone = S<T>::zero() - S<T>::one();
+    T* Sm = (T*)h->misc2.p;
+    T* Xm = Sm + (size_t)w1 * w2;
+    const int rows2 = m - i - w1;
+    const int ks = std::max(1, std::min(32, rows2 / 256));
+    RNPX_TRY(gw->part->ensure(sizeof(T) * (size_t)ks * w1 * w2, h));
+    RNPX_TRY(gemm<T>('C', 'N', w1, w2, rows2, one, Vp + w1, ldv, V2, ldv, zero, Sm, w1, ks, (T*)gw->part->p, ps));
+    RNPX_TRY(gemm<T>('N', 'N', w1, w2, w1, one, Tp, ldt, Sm, w1, zero, Xm, w1, 1, nullptr, ps));
+    RNPX_TRY(gemm<T>('N', 'N', w1, w2, w2, mone, Xm, w1, T2, ldt, zero, Tp + (size_t)w1 * ldt, ldt, 1, nullptr, ps));
+    return RNPX_OK;
+  };
+  // The panel phase of outer block b (columns [i, i + wo)) on stream ps.
+  auto panel_phase = [&](int b, hipStream_t ps, const GemmWork* gw) -> int {
+    const int i = bstart[b], wo = bw(b);
+    const int q = look ? (b & 1) : 0;
+    hipEvent_t ev;
+    h->prof_begin(K_PANEL, &ev, ps);
+    RNPX_CHECK(phase(i, wo, (T*)vb[q]->p, (T*)tb[q]->p, ps, gw));
+    const double mpd = m - i;
+    const double fl = (S<T>::cplx ? 4.0 : 1.0) * (2.0 * mpd * wo * wo - 2.0 / 3.0 * wo * wo * wo);
+    h->prof_end(K_PANEL, ev, fl, sizeof(T) * 3.0 * mpd * wo, ps);
     return RNPX_OK;
   };
   // Trailing update by outer block b of columns [c0, c1) (main stream unless
   // gw names another).
   auto update = [&](int b, int c0, int c1, const GemmWork* gw) -> int {
     if (c1 <= c0) return RNPX_OK;
-    const int i = b * nbo, wo = std::min(nbo, k - i);
+    const int i = bstart[b], wo = bw(b);
     const int q = look ? (b & 1) : 0;
     return apply_block_dev<T>(h, 'L', 'C', m - i, c1 - c0, wo, (const T*)vb[q]->p, ldv, (const T*)tb[q]->p, ldt,
                               A + (size_t)i + (size_t)c0 * lda, lda, gw);
@@ -605,7 +642,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   // Q^H B: block b's reflectors applied to B (rows [i, m)).
   auto update_rhs = [&](int b, const GemmWork* gw) -> int {
     if (nrhs == 0) return RNPX_OK;
-    const int i = b * nbo, wo = std::min(nbo, k - i);
+    const int i = bstart[b], wo = bw(b);
     const int q = look ? (b & 1) : 0;
     return apply_block_dev<T>(h, 'L', 'C', m - i, nrhs, wo, (const T*)vb[q]->p, ldv, (const T*)tb[q]->p, ldt,
                               B + (size_t)i, ldb, gw);
@@ -626,7 +663,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
     for (int b = 0; b < nob; ++b) {
       int s = panel_phase(b, st, &main_work);
       if (s) return s;
-      const int c0 = b * nbo + std::min(nbo, k - b * nbo);
+      const int c0 = bstart[b + 1];
       if ((s = update(b, c0, n, nullptr))) return s;
       if ((s = update_rhs(b, nullptr))) return s;
     }
@@ -685,13 +722,13 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   }
   for (int b = 0; b < nob; ++b) {
     RNPX_TRY(hipStreamWaitEvent(us, h->ev_pan[b & 1], 0));
-    const int c0 = b * nbo + std::min(nbo, k - b * nbo);  // first column after block b
+    const int c0 = bstart[b + 1];  // first column after block b
     if (c0 >= n) {
       if ((s = update_rhs(b, &upd_work))) return s;
       break;
     }
     if (b + 1 < nob) {
-      const int wn = std::min(nbo, k - c0);  // block b+1's columns
+      const int wn = bw(b + 1);  // block b+1's columns
       // Block b+1's columns are brought up to date on the panel stream, right
       // behind panel b: the rest of block b-1's update (which covered them)
       // is the only work of the other stream it waits for.
@@ -706,7 +743,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
         // there block b+1's kernel can queue behind other requests' work and
         // the gate would hold this stream for nothing.
         const int Gb = use_block ? cqr_block_groups(m - c0, std::min(256, h->num_cus)) : 0;
-        if (wn == 2 * nb && Gb > 0 && m - c0 <= gate_rows && c0 + wn < n && !h->ext_pstream)
+        if (wn >= 2 * nb && Gb > 0 && m - c0 <= gate_rows && c0 + wn < n && !h->ext_pstream)
           RNPX_TRY(launch_cqr_gate((const unsigned*)h->sync.p + 1 + 2 * npan + (size_t)kCqrSyncWords * (c0 / nb), Gb,
                                    us));
       }

@@ -181,6 +181,42 @@ hipError_t launch_trsm_blocked(char trans, int n, int nrhs, const T* R, long ldr
   return launch_trsm_tri<T>('U', trans, 'N', n, nrhs, R, ldr, B, ldb, flags, epoch, abortf, st);
 }
 
+// The inverses of the 64 x 64 diagonal blocks of an upper-triangular R
+// (the last one may be smaller), written over the same blocks of X (the
+// rest of X untouched).  One wave per block, the block staged in LDS; lane j
+// back-substitutes column j of the inverse: x_j = 1 / r_jj, then for
+// i = j-1 .. 0, x_i = -(sum_{k=i+1..j} r_ik x_k) / r_ii.
+__global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const double* __restrict__ R, long ldr,
+                                                       double* __restrict__ X, long ldx) {
+  __shared__ double Rs[64][65];
+  __shared__ double Xs[64][65];
+  const int lane = threadIdx.x;
+  const int b0 = blockIdx.x * 64, nb = min(64, n - b0);
+  for (int j = 0; j < nb; ++j) {
+    Rs[lane][j] = lane < nb ? R[(size_t)(b0 + lane) + (size_t)(b0 + j) * ldr] : 0.0;
+    Xs[lane][j] = 0.0;
+  }
+  __syncthreads();
+  if (lane < nb) {
+    const int j = lane;
+    Xs[j][j] = 1.0 / Rs[j][j];
+    for (int i = j - 1; i >= 0; --i) {
+      double sacc = 0.0;
+      for (int k = i + 1; k <= j; ++k) sacc += Rs[i][k] * Xs[k][j];
+      Xs[i][j] = -sacc / Rs[i][i];
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j)
+    if (lane < nb) X[(size_t)(b0 + lane) + (size_t)(b0 + j) * ldx] = Xs[lane][j];
+}
+
+hipError_t launch_trtri_diag(int n, const double* R, long ldr, double* X, long ldx, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(trtri_diag_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, n, R, ldr, X, ldx);
+  return hipGetLastError();
+}
+
 template hipError_t launch_trsm_blocked<double>(char, int, int, const double*, long, double*, long, unsigned*, unsigned,
                                                 unsigned*, hipStream_t);
 template hipError_t launch_trsm_blocked<zd>(char, int, int, const zd*, long, zd*, long, unsigned*, unsigned, unsigned*,

@@ -7,7 +7,7 @@
 //     {"kind": "factor", "dtype": "d", "m": 8192, "n": 8192, "nb": 128, "ksplit": 0, "ms": 28.2},
 //     {"kind": "batched", "dtype": "d", "lds": 40960, "blocked_min": 33, "ms": 8.7}]}
 //
-// Factor entries give the outer block width (nb 32, 64 or 128: the width of
+// Factor entries give the outer block width (nb 32, 64, 128 or 256: the width of
 // the compact-WY block the trailing matrix sees) and the split of W = V^H C;
 // a factorization whose handle tuning is automatic (nb = 0) takes the entry
 // of its element type nearest in (log2 m, log2 n), within a factor of two in
@@ -171,7 +171,7 @@ struct Reader {
 bool valid(const Entry& e) {
   if (e.dtype != 'd' && e.dtype != 'z') return false;
   if (e.kind == "factor")
-    return e.m >= 1 && e.n >= 1 && (e.nb == 32 || e.nb == 64 || e.nb == 128) && e.ksplit >= 0 && e.ksplit <= 64;
+    return e.m >= 1 && e.n >= 1 && (e.nb == 32 || e.nb == 64 || e.nb == 128 || e.nb == 256) && e.ksplit >= 0 && e.ksplit <= 64;
   if (e.kind == "batched") return e.lds >= 4096 && e.lds <= 160 * 1024 && e.blocked_min >= 1;
   return false;
 }
