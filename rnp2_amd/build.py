@@ -29,6 +29,7 @@ SOURCES = [
     "rnpx_mixed.hip",
     "rnpx_noalloc.hip",
     "rnpx_exact.hip",
+    "rnpx_zgemm.hip",
     "rnpx_qr.cpp", "rnpx_qlrq.cpp", "rnpx_hess.cpp", "rnpx_lu_driver.cpp",
     "rnpx_lq.cpp",
     "rnpx_ls.cpp",

@@ -340,6 +340,20 @@ hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long
         done = true;
       }
     }
+  } else {
+    // Complex op(A) N/T/C times op(B) N: the double-buffered complex kernel
+    // (RNPX_ZGEMM=-1 keeps the general one).
+    static const bool zoff = [] {
+      const char* e = getenv("RNPX_ZGEMM");
+      return e && atoi(e) < 0;
+    }();
+    if (!zoff && tb == 'N') {
+      const hipError_t e = zgemm_mfma(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, pp, st);
+      if (e != hipErrorNotSupported) {
+        if (e != hipSuccess) return e;
+        done = true;
+      }
+    }
   }
   if (!done) {
 #define RNPX_GEMM_CASE(A_, B_)                                                                          \

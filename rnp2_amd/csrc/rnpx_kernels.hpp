@@ -301,6 +301,9 @@ hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long
 hipError_t dgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const double* A, long lda, const double* B,
                       long ldb, double beta, double* C, long ldc, int kslice, int ns, double* part, int cfg,
                       hipStream_t st);
+// The complex counterpart (rnpx_zgemm.hip): op(A) N/T/C, op(B) N.
+hipError_t zgemm_mfma(char ta, char tb, int M, int N, int K, zd alpha, const zd* A, long lda, const zd* B, long ldb,
+                      zd beta, zd* C, long ldc, int kslice, int ns, zd* part, hipStream_t st);
 int dgemm_tile_cfg(int M, int N, int K, int ksplit, int num_cus);
 
 // Small helpers.

@@ -175,6 +175,7 @@ def test_gels_dense(m, n, nrhs, cplx):
     B = R.colmajor_empty(m, nrhs, tdt)
     B.copy_(torch.from_numpy(b))
     tau = torch.zeros(n, dtype=tdt, device="cuda")
+    torch.cuda.synchronize()  # the copies run on torch's stream, gels on the handle's
     h = R.Handle()
     h.gels(A, tau, B)
     h.synchronize()

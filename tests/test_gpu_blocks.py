@@ -34,6 +34,7 @@ def test_block_width(nb, cplx, lookahead):
             dt = torch.complex128 if cplx else torch.float64
             A = R.colmajor_empty(m, n, dt)
             A.copy_(torch.from_numpy(np.ascontiguousarray(a)).to("cuda"))
+            torch.cuda.synchronize()  # the copy runs on torch's stream, geqrf on the handle's
             k = min(m, n)
             tau = torch.empty(k, dtype=dt, device="cuda")
             h.geqrf(A, tau)

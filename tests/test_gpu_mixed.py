@@ -36,6 +36,7 @@ def dev(x):
     import torch
     t = R.colmajor_empty(x.shape[0], x.shape[1], torch.float64)
     t.copy_(torch.from_numpy(np.ascontiguousarray(x)))
+    torch.cuda.synchronize()  # the handle's stream reads it next
     return t
 
 
