@@ -30,6 +30,7 @@ SOURCES = [
     "rnpx_noalloc.hip",
     "rnpx_exact.hip",
     "rnpx_zgemm.hip",
+    "rnpx_zcqr.hip",
     "rnpx_qr.cpp", "rnpx_qlrq.cpp", "rnpx_hess.cpp", "rnpx_lu_driver.cpp",
     "rnpx_lq.cpp",
     "rnpx_ls.cpp",

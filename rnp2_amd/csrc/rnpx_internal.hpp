@@ -62,12 +62,13 @@ struct Handle {
   Buffer exbuf;                            // non-finite flag, trim limits, T and work of the exact path
   Buffer exsave;                           // the first block's columns while the rest is checked
   Buffer rinv;                             // mixed LS: R^{-1}, its recursion's products and step vectors
+  Buffer zcqr;                             // complex CholeskyQR2 panel: Q1, the Grams, R1, R1^{-1}, M, partials
   unsigned trsm_epoch = 0;                 // value the flags take in the current solve
   // Every buffer above, for accounting and release.
   std::vector<Buffer*> buffers() {
     return {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc,
             &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
-            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf, &invbuf, &exbuf, &exsave, &rinv};
+            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf, &invbuf, &exbuf, &exsave, &rinv, &zcqr};
   }
   hipError_t sync_all_streams();       // the handle's stream, panel stream and partition streams
   void* batch = nullptr;               // batched-call descriptor buffers
@@ -114,6 +115,11 @@ void prof_reset(Handle* h);
 // applied to B right after its trailing update (the least-squares drivers).
 template <typename T>
 int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B = nullptr, int nrhs = 0, long ldb = 0);
+// Complex panel (nb = 32 columns, mp >= 2 nb) by CholeskyQR2 and
+// Householder reconstruction (rnpx_zcqr.hip); a rejected panel sets *fail
+// and is factored by the column-wise kernel (pa, rpt) gated on it.
+int zcqr_panel(Handle* h, int mp, int nb, zd* A, long lda, zd* tau, zd* V, long ldv, zd* Tm, long ldt, unsigned* fail,
+               const PanelArgs<zd>& pa, int rpt, hipStream_t st);
 // Inf/NaN anywhere in the m x n matrix: checked on stream st, which is then
 // synchronized.
 template <typename T> int any_nonfinite(Handle* h, int m, int n, const T* A, long lda, bool* out, hipStream_t st);

@@ -411,6 +411,13 @@ static int panel_step(Handle* h, int p, int npan, int i, int ib, int m, int nb, 
   pa.abortf = (unsigned*)h->flag.p;
   pa.run_if = nullptr;
   const int rpt = pick_rpt(h, mp, nb, S<T>::cplx);
+  if constexpr (S<T>::cplx) {
+    // Complex full-width panels of more than 256 rows: CholeskyQR2 and
+    // Householder reconstruction (two Gram reductions instead of one grid
+    // hand-off per column); smaller ones stay on one workgroup.
+    if (h->tune.panel != 1 && ib == nb && nb == 32 && mp > 256)
+      return zcqr_panel(h, mp, nb, pa.A, lda, pa.tau, V, ldv, Tm, ldt, sync + 1 + npan + p, pa, rpt, st);
+  }
   RNPX_TRY(launch_panel<T>(pa, nb, rpt, st));
   RNPX_TRY(launch_tfactor<T>((const T*)h->gram.p, nb, ib, nb, Tm, (int)ldt, st));
   return RNPX_OK;
