@@ -446,7 +446,10 @@ int rnpx_zlarfb_dev(rnpx_handle h, const char *side, const char *trans, size_t m
                     size_t ldt, rnpx_zcomplex *c, size_t ldc);
 
 /* C = alpha op(A) op(B) + beta C on FP64 MFMA, op in "N","T","C" (the
- * MultMM the reference forwards to xgemm, src/BLAS_mix.cpp:498-517). */
+ * MultMM the reference forwards to xgemm, src/BLAS_mix.cpp:498-517).
+ * With op(B) = "N" and 1 <= n <= 4 (matrix-vector shapes) the product runs
+ * on an HBM-bound kernel that reads A once; the reduction is split and
+ * added in a fixed order either way. */
 int rnpx_dgemm_dev(rnpx_handle h, const char *transa, const char *transb, size_t m, size_t n,
                    size_t k, double alpha, const double *a, size_t lda, const double *b,
                    size_t ldb, double beta, double *c, size_t ldc);

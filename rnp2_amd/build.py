@@ -31,6 +31,7 @@ SOURCES = [
     "rnpx_exact.hip",
     "rnpx_zgemm.hip",
     "rnpx_zcqr.hip",
+    "rnpx_gemv.hip",
     "rnpx_qr.cpp", "rnpx_qlrq.cpp", "rnpx_hess.cpp", "rnpx_lu_driver.cpp",
     "rnpx_lq.cpp",
     "rnpx_ls.cpp",

@@ -983,6 +983,9 @@ int rnpx_dgemm_dev(rnpx_handle h, const char* transa, const char* transb, size_t
   if (!transa) return -2;
   if (!transb) return -3;
   if (!fits_int(k)) return RNPX_ERR_UNSUPPORTED;
+  if (n >= 1 && n <= 4 && transb[0] == 'N' && (transa[0] == 'N' || transa[0] == 'T' || transa[0] == 'C'))
+    return gemv_dev<double>(H(h), transa[0], (int)m, (int)n, (int)k, alpha, a, (long)lda, b, (long)ldb, beta, c,
+                            (long)ldc, H(h)->stream);
   return hip_status(gemm<double>(transa[0], transb[0], (int)m, (int)n, (int)k, alpha, a, (long)lda, b, (long)ldb,
                                  beta, c, (long)ldc, 1, nullptr, H(h)->stream));
 }
@@ -993,6 +996,10 @@ int rnpx_zgemm_dev(rnpx_handle h, const char* transa, const char* transb, size_t
   if (!transa) return -2;
   if (!transb) return -3;
   if (!fits_int(k)) return RNPX_ERR_UNSUPPORTED;
+  if (n >= 1 && n <= 4 && transb[0] == 'N' && (transa[0] == 'N' || transa[0] == 'T' || transa[0] == 'C'))
+    return gemv_dev<zd>(H(h), transa[0], (int)m, (int)n, (int)k, mkz(alpha.re, alpha.im),
+                        reinterpret_cast<const zd*>(a), (long)lda, reinterpret_cast<const zd*>(b), (long)ldb,
+                        mkz(beta.re, beta.im), reinterpret_cast<zd*>(c), (long)ldc, H(h)->stream);
   return hip_status(gemm<zd>(transa[0], transb[0], (int)m, (int)n, (int)k, mkz(alpha.re, alpha.im),
                              reinterpret_cast<const zd*>(a), (long)lda, reinterpret_cast<const zd*>(b), (long)ldb,
                              mkz(beta.re, beta.im), reinterpret_cast<zd*>(c), (long)ldc, 1, nullptr, H(h)->stream));

@@ -138,6 +138,7 @@ struct Args {
   int kslice;  // K per split slice (multiple of BK)
   int tilesM, tilesN, ntiles;
   int avec, bvec;  // operand base and leading dimension allow 16-byte loads
+  long sa, sb, sc;  // strided batch (grid y): elements between the problems' A, B, C
 };
 
 // Workgroup -> (tile m, tile n, split z): contiguous chunks per XCD, then
@@ -205,12 +206,16 @@ __device__ inline void k_loop(const Args& a, f64x4 (&acc)[FM][FN], double* __res
 }
 
 template <int MODE, bool VEC, bool AKC, bool BKC, int BM, int BN, int BK>
-__global__ __launch_bounds__(kThreads) void dgemm_kernel(Args a) {
+__global__ __launch_bounds__(kThreads) void dgemm_kernel(Args a0) {
   using OA = Op<AKC, BM, BK>;
   using OB = Op<BKC, BN, BK>;
   constexpr int FM = BM / 32, FN = BN / 32;
   __shared__ __align__(16) double As[2][OA::ELEMS];
   __shared__ __align__(16) double Bs[2][OB::ELEMS];
+  Args a = a0;  // problem blockIdx.y of a strided batch (0 for one product)
+  a.A += (size_t)blockIdx.y * a0.sa;
+  a.B += (size_t)blockIdx.y * a0.sb;
+  a.C += (size_t)blockIdx.y * a0.sc;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -270,7 +275,7 @@ __global__ __launch_bounds__(kThreads) void dgemm_kernel(Args a) {
 }
 
 template <bool AKC, bool BKC, int BM, int BN, int BK>
-static hipError_t launch(Args a, hipStream_t st) {
+static hipError_t launch(Args a, hipStream_t st, int batch = 1) {
   // 16-byte loads: both operands aligned with even leading dimensions, and an
   // even extent along each operand's pairs (rows of op(A) or op(B) for an
   // operand contiguous along the output index, K for one contiguous along K)
@@ -280,7 +285,7 @@ static hipError_t launch(Args a, hipStream_t st) {
   a.tilesN = (a.N + BN - 1) / BN;
   const int ns = (a.mode == kSplit) ? std::max(1, (a.K + a.kslice - 1) / a.kslice) : 1;  // K = 0: one empty slice
   a.ntiles = a.tilesM * a.tilesN * ns;
-  const dim3 grid((unsigned)a.ntiles), block(kThreads);
+  const dim3 grid((unsigned)a.ntiles, (unsigned)std::max(1, batch)), block(kThreads);
 #define RNPX_DG_LAUNCH(MODE_)                                                                         \
   do {                                                                                                \
     if (vec)                                                                                          \
@@ -304,6 +309,158 @@ static bool aligned16(const double* p, long ld) {
   return ((reinterpret_cast<uintptr_t>(p) & 15u) == 0) && (ld % 2 == 0);
 }
 
+
+// ---------------------------------------------------------------------------
+// The FP32 product of the mixed form: FP64 operands in memory, rounded to
+// FP32 when a slice is staged into LDS, v_mfma_f32_16x16x4f32 with FP32
+// accumulation, the result added to C in FP64 (or written raw for the split-K
+// reduction, which sums in FP64).  Same tiling, staging, clamped loads and XCD
+// mapping as the FP64 kernel; lane l's four accumulators of a 16 x 16 block
+// lie along one column of C: C(i0 + l%16, j0 + 4 (l/16) + r).
+// ---------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <bool KC, int R, int BK>
+struct OpF {
+  using O = Op<KC, R, BK>;
+  static constexpr int LD = KC ? BK + 1 : R + 1;
+  static constexpr int ELEMS = (KC ? R : BK) * LD;
+  static constexpr int NV = O::NV;
+  __device__ static inline void store(const double2 (&v)[NV], float* __restrict__ sm, bool neg, int kvalid, int tid) {
+#pragma unroll
+    for (int s = 0; s < NV; ++s) {
+      int r, k;
+      O::coords(tid, s, r, k);
+      double2 x = v[s];
+      if (kvalid < BK) {
+        const int k1 = KC ? k + 1 : k;
+        x.x = (k < kvalid) ? x.x : 0.0;
+        x.y = (k1 < kvalid) ? x.y : 0.0;
+      }
+      const float a = (float)(neg ? -x.x : x.x), b = (float)(neg ? -x.y : x.y);
+      if constexpr (KC) {
+        sm[r * LD + k] = a;
+        sm[r * LD + k + 1] = b;
+      } else {
+        sm[k * LD + r] = a;
+        sm[k * LD + r + 1] = b;
+      }
+    }
+  }
+  __device__ static inline float read(const float* __restrict__ sm, int r, int k) {
+    return KC ? sm[r * LD + k] : sm[k * LD + r];
+  }
+};
+
+template <int MODE, bool VEC, bool AKC, bool BKC, int BM, int BN, int BK>
+__global__ __launch_bounds__(kThreads) void sgemm_kernel(Args a) {
+  using OA = OpF<AKC, BM, BK>;
+  using OB = OpF<BKC, BN, BK>;
+  constexpr int FM = BM / 32, FN = BN / 32;
+  __shared__ float As[2][OA::ELEMS];
+  __shared__ float Bs[2][OB::ELEMS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  int tm, tn, z;
+  tile_of(a, blockIdx.x, gridDim.x, tm, tn, z);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int kbeg = z * a.kslice;
+  const int kend = min(a.K, kbeg + a.kslice);
+  const bool inside = (m0 + BM <= a.M) && (n0 + BN <= a.N);
+  f32x4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  double2 ra[OA::NV], rb[OB::NV];
+  const int nsl = (kend - kbeg + BK - 1) / BK;
+  if (nsl > 0) {
+    OA::O::template fetch<VEC>(ra, a.A, a.lda, m0, kbeg, a.M, kend, tid);
+    OB::O::template fetch<VEC>(rb, a.B, a.ldb, n0, kbeg, a.N, kend, tid);
+    OA::store(ra, As[0], false, kend - kbeg, tid);
+    OB::store(rb, Bs[0], false, kend - kbeg, tid);
+  }
+  __syncthreads();
+  for (int s = 0; s < nsl; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < nsl) {
+      OA::O::template fetch<VEC>(ra, a.A, a.lda, m0, kbeg + (s + 1) * BK, a.M, kend, tid);
+      OB::O::template fetch<VEC>(rb, a.B, a.ldb, n0, kbeg + (s + 1) * BK, a.N, kend, tid);
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      float af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = OA::read(As[cur], wm * (BM / 2) + 16 * i + (lane & 15), kk + (lane >> 4));
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = OB::read(Bs[cur], wn * (BN / 2) + 16 * j + (lane & 15), kk + (lane >> 4));
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+    }
+    if (s + 1 < nsl) {
+      const int kvalid = kend - (kbeg + (s + 1) * BK);
+      OA::store(ra, As[cur ^ 1], false, kvalid, tid);
+      OB::store(rb, Bs[cur ^ 1], false, kvalid, tid);
+    }
+    __syncthreads();
+  }
+  double* out = a.C;
+  const long ldo = a.ldc;
+  if constexpr (MODE == kSplit) out = a.C + (size_t)z * a.zstride;
+  const int ib = m0 + wm * (BM / 2) + (lane & 15);
+  const int jb = n0 + wn * (BN / 2) + 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = ib + 16 * i, gj = jb + 16 * j + r;
+        if (inside || (gi < a.M && gj < a.N)) {
+          double* cp = out + (size_t)gi + (size_t)gj * ldo;
+          const double v = (double)acc[i][j][r];
+          if constexpr (MODE == kCInit)
+            *cp = *cp + a.alpha * v;
+          else if constexpr (MODE == kGeneral)
+            *cp = a.alpha * v + a.beta * (*cp);
+          else if constexpr (MODE == kBeta0)
+            *cp = a.alpha * v;
+          else
+            *cp = v;
+        }
+      }
+}
+
+template <bool AKC, bool BKC, int BM, int BN, int BK>
+static hipError_t launch32(Args a, hipStream_t st) {
+  const bool vec = a.avec && a.bvec && (AKC ? a.K % 2 == 0 : a.M % 2 == 0) && (BKC ? a.K % 2 == 0 : a.N % 2 == 0) &&
+                   (a.kslice % 2 == 0);
+  a.tilesM = (a.M + BM - 1) / BM;
+  a.tilesN = (a.N + BN - 1) / BN;
+  const int ns = (a.mode == kSplit) ? std::max(1, (a.K + a.kslice - 1) / a.kslice) : 1;
+  a.ntiles = a.tilesM * a.tilesN * ns;
+  const dim3 grid((unsigned)a.ntiles), block(kThreads);
+#define RNPX_SG_LAUNCH(MODE_)                                                                         \
+  do {                                                                                                \
+    if (vec)                                                                                          \
+      hipLaunchKernelGGL((sgemm_kernel<MODE_, true, AKC, BKC, BM, BN, BK>), grid, block, 0, st, a);   \
+    else                                                                                              \
+      hipLaunchKernelGGL((sgemm_kernel<MODE_, false, AKC, BKC, BM, BN, BK>), grid, block, 0, st, a);  \
+  } while (0)
+  if (a.mode == kCInit)
+    RNPX_SG_LAUNCH(kCInit);
+  else if (a.mode == kSplit)
+    RNPX_SG_LAUNCH(kSplit);
+  else if (a.mode == kBeta0)
+    RNPX_SG_LAUNCH(kBeta0);
+  else
+    RNPX_SG_LAUNCH(kGeneral);
+#undef RNPX_SG_LAUNCH
+  return hipGetLastError();
+}
+
 }  // namespace dg
 
 // The tile configurations built (see the header comment): 128 x 128 for
@@ -317,6 +474,61 @@ int dgemm_tile_cfg(int M, int N, int K, int ksplit, int num_cus) {
   if (t64 >= 2L * num_cus || (K > 256 && t64 >= num_cus)) return 2;
   if (t128 >= 2L * num_cus) return 1;
   return 3;
+}
+
+// The mixed form's FP32-MFMA product (FP64 operands and result, FP32
+// accumulation): op(A) N/T/C, op(B) N; ns > 1: raw slices of kslice (a
+// multiple of 32) to part.  Other forms return hipErrorNotSupported.
+hipError_t sgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const double* A, long lda, const double* B,
+                      long ldb, double beta, double* C, long ldc, int kslice, int ns, double* part, hipStream_t st) {
+  const bool akc = (ta == 'T' || ta == 'C');
+  if (tb != 'N' || !(akc || ta == 'N')) return hipErrorNotSupported;
+  constexpr int BK = 32;
+  if (ns > 1 && (kslice % BK != 0 || !part)) return hipErrorNotSupported;
+  static const int cus = [] {
+    int dev = 0, n = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+    return n > 0 ? n : 256;
+  }();
+  const int cfg = dgemm_tile_cfg(M, N, ns > 1 ? kslice : K, ns, cus);
+  dg::Args a{};
+  a.M = M;
+  a.N = N;
+  a.K = K;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.A = A;
+  a.lda = lda;
+  a.B = B;
+  a.ldb = ldb;
+  a.avec = dg::aligned16(A, lda);
+  a.bvec = dg::aligned16(B, ldb);
+  if (ns > 1) {
+    a.mode = dg::kSplit;
+    a.kslice = kslice;
+    a.C = part;
+    a.ldc = M;
+    a.zstride = (size_t)M * N;
+  } else {
+    if (beta == 1.0)
+      a.mode = dg::kCInit;  // C += alpha P in FP64
+    else if (beta == 0.0)
+      a.mode = (alpha == 1.0) ? dg::kSplit : dg::kBeta0;
+    else
+      a.mode = dg::kGeneral;
+    a.kslice = std::max(BK, ((K + BK - 1) / BK) * BK);
+    a.C = C;
+    a.ldc = ldc;
+    a.zstride = 0;
+  }
+  if (akc) {
+    if (cfg == 1) return dg::launch32<true, true, 128, 128, BK>(a, st);
+    if (cfg == 2) return dg::launch32<true, true, 128, 64, BK>(a, st);
+    return dg::launch32<true, true, 64, 64, BK>(a, st);
+  }
+  if (cfg == 1) return dg::launch32<false, true, 128, 128, BK>(a, st);
+  if (cfg == 2) return dg::launch32<false, true, 128, 64, BK>(a, st);
+  return dg::launch32<false, true, 64, 64, BK>(a, st);
 }
 
 // op(A) in {'N', 'T', 'C'} (real: 'C' = 'T'), op(B) 'N'; ns > 1: split-K
@@ -379,6 +591,55 @@ hipError_t dgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const
     case 1: return dg::launch<false, true, 128, 128, BK>(a, st);
     case 2: return dg::launch<false, true, 128, 64, BK>(a, st);
     default: return dg::launch<false, true, 64, 64, BK>(a, st);
+  }
+}
+
+// batch products C_b = alpha A_b B_b + beta C_b (op N, N), X_b = X + b sX:
+// one launch, problem b in grid row b.
+hipError_t dgemm_mfma_batched(int M, int N, int K, double alpha, const double* A, long lda, long sa, const double* B,
+                              long ldb, long sb, double beta, double* C, long ldc, long sc, int batch,
+                              hipStream_t st) {
+  if (M < 0 || N < 0 || K < 0 || batch < 0 || lda < std::max(1, M) || ldb < std::max(1, K) || ldc < std::max(1, M))
+    return hipErrorInvalidValue;
+  if (M == 0 || N == 0 || batch == 0) return hipSuccess;
+  if (batch > 65535) return hipErrorInvalidValue;
+  constexpr int BK = 16;
+  static const int cus = [] {
+    int dev = 0, n = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+    return n > 0 ? n : 256;
+  }();
+  // the tile choice for the whole launch: the batch multiplies the tile count
+  const int cfg = dgemm_tile_cfg(M, N, K, batch, cus);
+  dg::Args a{};
+  a.M = M;
+  a.N = N;
+  a.K = K;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.A = A;
+  a.lda = lda;
+  a.B = B;
+  a.ldb = ldb;
+  a.sa = sa;
+  a.sb = sb;
+  a.sc = sc;
+  a.avec = dg::aligned16(A, lda) && sa % 2 == 0;
+  a.bvec = dg::aligned16(B, ldb) && sb % 2 == 0;
+  if (beta == 1.0 && (alpha == 1.0 || alpha == -1.0))
+    a.mode = dg::kCInit;
+  else if (beta == 0.0)
+    a.mode = (alpha == 1.0) ? dg::kSplit : dg::kBeta0;
+  else
+    a.mode = dg::kGeneral;
+  a.kslice = std::max(BK, ((K + BK - 1) / BK) * BK);
+  a.C = C;
+  a.ldc = ldc;
+  a.zstride = 0;
+  switch (cfg) {
+    case 1: return dg::launch<false, true, 128, 128, BK>(a, st, batch);
+    case 2: return dg::launch<false, true, 128, 64, BK>(a, st, batch);
+    default: return dg::launch<false, true, 64, 64, BK>(a, st, batch);
   }
 }
 

@@ -308,12 +308,6 @@ hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long
   if (M < 0 || N < 0 || K < 0) return hipErrorInvalidValue;
   if (M == 0 || N == 0) return hipSuccess;
   if (ldc < M) return hipErrorInvalidValue;
-  if (K == 0) {
-    // C = beta * C
-    T* p = nullptr;
-    (void)p;
-    if (ksplit < 1) ksplit = 1;
-  }
   const bool ca = (ta == 'N' || ta == 'T' || ta == 'C'), cb = (tb == 'N' || tb == 'T' || tb == 'C');
   if (!ca || !cb) return hipErrorInvalidValue;
   if (ta == 'N' ? lda < M : lda < K) return hipErrorInvalidValue;

@@ -20,6 +20,15 @@ struct Buffer {
   void release();
 };
 
+// Growable pinned host memory for small read-backs inside loops (a copy into
+// pageable memory goes through the runtime's staging buffer).
+struct HostBuffer {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t b, Handle* h);
+  void release();
+};
+
 // Live per-kernel timing (HIP events recorded on the handle's stream around
 // the launches of one kernel family) with the launches' algorithmic work.
 enum KernelId { K_UPDATE = 0, K_WGEMM = 1, K_PANEL = 2, K_TFACTOR = 3, K_COUNT = 4 };
@@ -63,12 +72,14 @@ struct Handle {
   Buffer exsave;                           // the first block's columns while the rest is checked
   Buffer rinv;                             // mixed LS: R^{-1}, its recursion's products and step vectors
   Buffer zcqr;                             // complex CholeskyQR2 panel: Q1, the Grams, R1, R1^{-1}, M, partials
+  Buffer gemvbuf;                          // partial sums of the few-column products (gemv_dev)
+  HostBuffer hpin;                         // pinned read-back of the refinement's norms
   unsigned trsm_epoch = 0;                 // value the flags take in the current solve
   // Every buffer above, for accounting and release.
   std::vector<Buffer*> buffers() {
     return {&vbuf, &vfull, &gram, &tbuf, &wbuf, &zbuf, &part, &ppart, &rowbuf, &sync, &flag, &misc,
             &vbuf2, &gram2, &tbuf2, &stage1, &stage2, &stage3, &lsbuf, &lsnrm, &cqr_part, &cqr_gsum,
-            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf, &invbuf, &exbuf, &exsave, &rinv, &zcqr};
+            &cqr_scratch, &wbuf2, &zbuf2, &part2, &misc2, &trsm_flags, &lqbuf, &qlbuf, &lubuf, &rfbuf, &invbuf, &exbuf, &exsave, &rinv, &zcqr, &gemvbuf};
   }
   hipError_t sync_all_streams();       // the handle's stream, panel stream and partition streams
   void* batch = nullptr;               // batched-call descriptor buffers
@@ -130,6 +141,12 @@ template <typename T> int last_nonzero_limit(Handle* h, int which, int m, int n,
 // Inf or NaN); lwork as given to the reference (0: optimal).
 template <typename T> int geqrf_exact(Handle* h, int m, int n, T* A, long lda, T* tau, size_t lwork);
 template <typename T> int gels_dev(Handle* h, int m, int n, int nrhs, T* A, long lda, T* tau, T* B, long ldb);
+// Y (M x N, 1 <= N <= 4) = alpha op(A) X + beta Y, op(A) M x K, on stream st
+// (rnpx_gemv.hip): HBM-bound products with few right-hand sides, slices of the
+// reduction added in a fixed order.
+template <typename T>
+int gemv_dev(Handle* h, char ta, int M, int N, int K, T alpha, const T* A, long lda, const T* X, long ldx, T beta,
+             T* Y, long ldy, hipStream_t st);
 int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb, double* X,
                    long ldx, int max_iter, double tol, int* iters, double* berr);
 int ls_berr_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb,

@@ -360,8 +360,17 @@ hipError_t launch_trsm_un(int n, int nrhs, const T* R, long ldr, T* B, long ldb,
 // Inverses of the 64 x 64 diagonal blocks of upper-triangular R into X.
 hipError_t launch_trtri_diag(int n, const double* R, long ldr, double* X, long ldx, hipStream_t st);
 
+// Strided batch of FP64 MFMA products C_b = alpha A_b B_b + beta C_b (op N,
+// N; X_b = X + b sX), one launch.
+hipError_t dgemm_mfma_batched(int M, int N, int K, double alpha, const double* A, long lda, long sa, const double* B,
+                              long ldb, long sb, double beta, double* C, long ldc, long sc, int batch,
+                              hipStream_t st);
+
 // Mixed precision: FP64 operands, FP32 MFMA product (FP32 accumulation),
 // FP64 epilogue; split-K slices reduced in FP64.
+// FP32-MFMA product of FP64 operands (rnpx_dgemm.hip), used by gemm32.
+hipError_t sgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const double* A, long lda, const double* B,
+                      long ldb, double beta, double* C, long ldc, int kslice, int ns, double* part, hipStream_t st);
 hipError_t gemm32(char ta, char tb, int M, int N, int K, double alpha, const double* A, long lda, const double* B,
                   long ldb, double beta, double* C, long ldc, int ksplit, double* part, hipStream_t st);
 // R^H Y = S (R upper), S's first n rows in place.

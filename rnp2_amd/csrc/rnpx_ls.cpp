@@ -82,38 +82,75 @@ static int ls_scratch(Handle* h, int m, int n, int nrhs, LsScratch* sc) {
   return RNPX_OK;
 }
 
-// r = b - A x; s = A^T r; host gets per-column ||s||^2, ||x||^2, ||b||^2 and
-// the total ||A||_F^2 (when anorm2 < 0 it is computed).
-static int ls_residual(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb,
-                       const double* X, long ldx, const LsScratch& sc, double* anorm2, double* eta) {
+// Y = alpha op(A) X + beta Y for the refinement's products: the few-column
+// kernel (HBM-bound, every entry of A read once) up to four right-hand
+// sides, the MFMA GEMM (split over the reduction for A^T) beyond.
+static int ls_prod(Handle* h, char ta, int M, int N, int K, double alpha, const double* A, long lda, const double* X,
+                   long ldx, double beta, double* Y, long ldy) {
+  hipStream_t st = h->stream;
+  if (N <= 4) return gemv_dev<double>(h, ta, M, N, K, alpha, A, lda, X, ldx, beta, Y, ldy, st);
+  if (ta == 'N') {
+    RNPX_TRY(gemm<double>('N', 'N', M, N, K, alpha, A, lda, X, ldx, beta, Y, ldy, 1, nullptr, st));
+    return RNPX_OK;
+  }
+  const int ks = std::max(1, std::min(32, K / 512));
+  RNPX_TRY(h->part.ensure(sizeof(double) * (size_t)ks * M * N, h));
+  RNPX_TRY(gemm<double>('T', 'N', M, N, K, alpha, A, lda, X, ldx, beta, Y, ldy, ks, (double*)h->part.p, st));
+  return RNPX_OK;
+}
+
+// Enqueues r = b - A x, s = A^T r and the squared column norms
+//   nrm[q] = ||s_q||^2, nrm[nrhs + q] = ||x_q||^2,
+//   nrm[3 nrhs + q] = ||b_q||^2 (with_b), nrm[4 nrhs + j] = ||A(:, j)||^2 (with_a);
+// nrm[2 nrhs + q] is the refinement's ||d_q||^2, so that one read-back of
+// nrm[0:3 nrhs] serves a step.
+static int ls_residual_enqueue(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B,
+                               long ldb, const double* X, long ldx, const LsScratch& sc, bool with_b, bool with_a) {
   hipStream_t st = h->stream;
   RNPX_TRY(launch_copy<double>(m, nrhs, B, ldb, sc.R, m, st));
-  RNPX_TRY(gemm<double>('N', 'N', m, nrhs, n, -1.0, A, lda, X, ldx, 1.0, sc.R, m, 1, nullptr, st));
-  const int ks = std::max(1, std::min(32, m / 512));
-  RNPX_TRY(h->part.ensure(sizeof(double) * (size_t)ks * n * nrhs, h));
-  RNPX_TRY(gemm<double>('T', 'N', n, nrhs, m, 1.0, A, lda, sc.R, m, 0.0, sc.S, n, ks, (double*)h->part.p, st));
+  RNPX_CHECK(ls_prod(h, 'N', m, nrhs, n, -1.0, A, lda, X, ldx, 1.0, sc.R, m));
+  RNPX_CHECK(ls_prod(h, 'T', n, nrhs, m, 1.0, A, lda, sc.R, m, 0.0, sc.S, n));
   double* nr = sc.nrm;
   RNPX_TRY(launch_colnorm2(n, nrhs, sc.S, n, nr, st));
   RNPX_TRY(launch_colnorm2(n, nrhs, X, ldx, nr + nrhs, st));
-  RNPX_TRY(launch_colnorm2(m, nrhs, B, ldb, nr + 2 * nrhs, st));
-  const bool need_a = (*anorm2 < 0);
-  if (need_a) RNPX_TRY(launch_colnorm2(m, n, A, lda, nr + 3 * nrhs, st));
-  std::vector<double> hn(3 * nrhs + (need_a ? n : 0));
-  RNPX_TRY(hipMemcpyAsync(hn.data(), nr, sizeof(double) * hn.size(), hipMemcpyDeviceToHost, st));
-  RNPX_TRY(hipStreamSynchronize(st));
-  if (need_a) {
-    double a2 = 0.0;
-    for (int j = 0; j < n; ++j) a2 += hn[3 * nrhs + j];
-    *anorm2 = a2;
-  }
-  const double an = sqrt(*anorm2);
+  if (with_b) RNPX_TRY(launch_colnorm2(m, nrhs, B, ldb, nr + 3 * nrhs, st));
+  if (with_a) RNPX_TRY(launch_colnorm2(m, n, A, lda, nr + 4 * nrhs, st));
+  return RNPX_OK;
+}
+
+// eta (worst column) from ||s||^2, ||x||^2 (hn[0:2 nrhs] as above) and
+// ||b||^2 (b2[0:nrhs]).
+static double ls_eta(const double* hn, const double* b2, int nrhs, double anorm2) {
+  const double an = sqrt(anorm2);
   double worst = 0.0;
   for (int q = 0; q < nrhs; ++q) {
-    const double den = an * (an * sqrt(hn[nrhs + q]) + sqrt(hn[2 * nrhs + q]));
+    const double den = an * (an * sqrt(hn[nrhs + q]) + sqrt(b2[q]));
     const double e = den > 0 ? sqrt(hn[q]) / den : sqrt(hn[q]);
     worst = std::max(worst, e);
   }
-  *eta = worst;
+  return worst;
+}
+
+// r = b - A x; s = A^T r; eta of x, synchronously (||A||_F^2 computed when
+// *anorm2 < 0); ||b||^2 per column left in b2 when given.
+static int ls_residual(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb,
+                       const double* X, long ldx, const LsScratch& sc, double* anorm2, double* eta,
+                       std::vector<double>* b2 = nullptr) {
+  hipStream_t st = h->stream;
+  const bool need_a = (*anorm2 < 0);
+  RNPX_CHECK(ls_residual_enqueue(h, m, n, nrhs, A, lda, B, ldb, X, ldx, sc, true, need_a));
+  const size_t cnt = 4 * (size_t)nrhs + (need_a ? n : 0);
+  RNPX_TRY(h->hpin.ensure(sizeof(double) * cnt, h));
+  double* hn = (double*)h->hpin.p;
+  RNPX_TRY(hipMemcpyAsync(hn, sc.nrm, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+  RNPX_TRY(hipStreamSynchronize(st));
+  if (need_a) {
+    double a2 = 0.0;
+    for (int j = 0; j < n; ++j) a2 += hn[4 * nrhs + j];
+    *anorm2 = a2;
+  }
+  *eta = ls_eta(hn, hn + 3 * nrhs, nrhs, *anorm2);
+  if (b2) b2->assign(hn + 3 * nrhs, hn + 4 * nrhs);
   return RNPX_OK;
 }
 
@@ -140,27 +177,36 @@ int ls_berr_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, co
 // of the final x is checked against tol.  A and B are not modified; X
 // (n x nrhs) receives the solution.
 // X = R^{-1} for the upper-triangular n x n R (X upper triangular, zero
-// below): the 64 x 64 diagonal blocks inverted in LDS, then bottom-up
-// [X11 X12; 0 X22] with X12 = -X11 (R12 X22) on the MFMA GEMM.  tmp holds
-// (n/2 + 64)^2 doubles.
+// below): the 64 x 64 diagonal blocks inverted in LDS, then bottom-up: at
+// width w, every pair of neighbouring inverted w-blocks becomes
+// [X11 X12; 0 X22] with X12 = -X11 (R12 X22), all the level's full pairs in
+// one strided batch of MFMA products (and a last, narrower pair alone).
+// tmp holds n^2 / 4 + 64 n doubles.
 static int trtri_upper(Handle* h, int n, const double* R, long ldr, double* X, long ldx, double* tmp) {
   hipStream_t st = h->stream;
   RNPX_TRY(launch_set<double>(n, n, 0.0, 0.0, X, ldx, st));
   RNPX_TRY(launch_trtri_diag(n, R, ldr, X, ldx, st));
-  std::function<int(int, int)> join = [&](int i0, int sz) -> int {
-    if (sz <= 64) return RNPX_OK;
-    const int s1 = ((sz / 2 + 63) / 64) * 64, s2 = sz - s1;
-    RNPX_CHECK(join(i0, s1));
-    RNPX_CHECK(join(i0 + s1, s2));
-    const double* R12 = R + (size_t)i0 + (size_t)(i0 + s1) * ldr;
-    const double* X11 = X + (size_t)i0 + (size_t)i0 * ldx;
-    const double* X22 = X + (size_t)(i0 + s1) + (size_t)(i0 + s1) * ldx;
-    double* X12 = X + (size_t)i0 + (size_t)(i0 + s1) * ldx;
-    RNPX_TRY(gemm<double>('N', 'N', s1, s2, s2, 1.0, R12, ldr, X22, ldx, 0.0, tmp, s1, 1, nullptr, st));
-    RNPX_TRY(gemm<double>('N', 'N', s1, s2, s1, -1.0, X11, ldx, tmp, s1, 0.0, X12, ldx, 1, nullptr, st));
-    return RNPX_OK;
-  };
-  return join(0, n);
+  for (int w = 64; w < n; w *= 2) {
+    const int full = n / (2 * w), rest = n - full * 2 * w;
+    if (full > 0) {
+      const long sr = 2L * w * (1 + ldr), sx = 2L * w * (1 + ldx), st2 = (long)w * w;
+      // tmp_b = R12_b X22_b; X12_b = -X11_b tmp_b
+      RNPX_TRY(dgemm_mfma_batched(w, w, w, 1.0, R + (size_t)w * ldr, ldr, sr, X + (size_t)w * (1 + ldx), ldx, sx, 0.0,
+                                  tmp, w, st2, full, st));
+      RNPX_TRY(dgemm_mfma_batched(w, w, w, -1.0, X, ldx, sx, tmp, w, st2, 0.0, X + (size_t)w * ldx, ldx, sx, full,
+                                  st));
+    }
+    if (rest > w) {
+      const int i0 = full * 2 * w, s1 = w, s2 = rest - w;
+      const double* R12 = R + (size_t)i0 + (size_t)(i0 + s1) * ldr;
+      const double* X11 = X + (size_t)i0 + (size_t)i0 * ldx;
+      const double* X22 = X + (size_t)(i0 + s1) + (size_t)(i0 + s1) * ldx;
+      double* X12 = X + (size_t)i0 + (size_t)(i0 + s1) * ldx;
+      RNPX_TRY(gemm<double>('N', 'N', s1, s2, s2, 1.0, R12, ldr, X22, ldx, 0.0, tmp, s1, 1, nullptr, st));
+      RNPX_TRY(gemm<double>('N', 'N', s1, s2, s1, -1.0, X11, ldx, tmp, s1, 0.0, X12, ldx, 1, nullptr, st));
+    }
+  }
+  return RNPX_OK;
 }
 
 int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda, const double* B, long ldb, double* X,
@@ -189,40 +235,46 @@ int gels_mixed_dev(Handle* h, int m, int n, int nrhs, const double* A, long lda,
   // R^{-1} once: the refinement's two triangular solves per step become two
   // products (its error, kappa(R) eps, is far below the FP32 updates' that
   // the refinement corrects)
-  const size_t half = (size_t)(n / 2 + 64);
-  RNPX_TRY(h->rinv.ensure(sizeof(double) * ((size_t)n * n + half * half + (size_t)n * nrhs), h));
+  const size_t tmpn = (size_t)n * n / 4 + 64 * (size_t)n;
+  RNPX_TRY(h->rinv.ensure(sizeof(double) * ((size_t)n * n + tmpn + (size_t)n * nrhs), h));
   double* Ri = (double*)h->rinv.p;
   double* tmp = Ri + (size_t)n * n;
-  double* Tn = tmp + half * half;
+  double* Tn = tmp + tmpn;
   RNPX_CHECK(trtri_upper(h, n, Af, m, Ri, n, tmp));
   // x0 = R^{-1} (Q^T b)(0:n)
-  RNPX_TRY(gemm<double>('N', 'N', n, nrhs, n, 1.0, Ri, n, Bq, m, 0.0, X, ldx, 1, nullptr, st));
+  RNPX_CHECK(ls_prod(h, 'N', n, nrhs, n, 1.0, Ri, n, Bq, m, 0.0, X, ldx));
   LsScratch sc;
   if ((s = ls_scratch(h, m, n, nrhs, &sc))) return s;
   double a2 = -1.0, eta = 0.0, prev = 0.0;
-  std::vector<double> dn(2 * nrhs);
+  std::vector<double> b2;
+  if ((s = ls_residual(h, m, n, nrhs, A, lda, B, ldb, X, ldx, sc, &a2, &eta, &b2))) return s;
+  // one read-back per step (pinned): the correction's norms and the eta of
+  // the corrected x; ||b|| is kept from the first residual
+  RNPX_TRY(h->hpin.ensure(sizeof(double) * 3 * nrhs, h));
+  const double* hn = (const double*)h->hpin.p;
+  double* dnrm = sc.nrm + 2 * nrhs;
   int it = 0;
-  for (;; ++it) {
-    if ((s = ls_residual(h, m, n, nrhs, A, lda, B, ldb, X, ldx, sc, &a2, &eta))) return s;
-    if (it >= max_iter) break;
+  while (it < max_iter) {
     // d = R^{-1} R^{-T} s  (s is in sc.S, n x nrhs), x += d
-    RNPX_TRY(gemm<double>('T', 'N', n, nrhs, n, 1.0, Ri, n, sc.S, n, 0.0, Tn, n, 1, nullptr, st));
-    RNPX_TRY(gemm<double>('N', 'N', n, nrhs, n, 1.0, Ri, n, Tn, n, 0.0, sc.S, n, 1, nullptr, st));
+    RNPX_CHECK(ls_prod(h, 'T', n, nrhs, n, 1.0, Ri, n, sc.S, n, 0.0, Tn, n));
+    RNPX_CHECK(ls_prod(h, 'N', n, nrhs, n, 1.0, Ri, n, Tn, n, 0.0, sc.S, n));
     RNPX_TRY(launch_axpy_mat(n, nrhs, sc.S, n, X, ldx, st));
-    RNPX_TRY(launch_colnorm2(n, nrhs, sc.S, n, sc.nrm, st));
-    RNPX_TRY(launch_colnorm2(n, nrhs, X, ldx, sc.nrm + nrhs, st));
-    RNPX_TRY(hipMemcpyAsync(dn.data(), sc.nrm, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, st));
+    RNPX_TRY(launch_colnorm2(n, nrhs, sc.S, n, dnrm, st));
+    RNPX_CHECK(ls_residual_enqueue(h, m, n, nrhs, A, lda, B, ldb, X, ldx, sc, false, false));
+    RNPX_TRY(hipMemcpyAsync(h->hpin.p, sc.nrm, sizeof(double) * 3 * nrhs, hipMemcpyDeviceToHost, st));
     RNPX_TRY(hipStreamSynchronize(st));
+    ++it;
+    eta = ls_eta(hn, b2.data(), nrhs, a2);
     double rel = 0.0;
-    for (int q = 0; q < nrhs; ++q) rel = std::max(rel, dn[nrhs + q] > 0 ? sqrt(dn[q] / dn[nrhs + q]) : sqrt(dn[q]));
-    const bool small = rel <= 0x1p-50;
-    const bool stalled = it > 0 && !(rel < 0.5 * prev);
-    prev = rel;
-    if (small || stalled) {
-      ++it;
-      if ((s = ls_residual(h, m, n, nrhs, A, lda, B, ldb, X, ldx, sc, &a2, &eta))) return s;
-      break;
+    for (int q = 0; q < nrhs; ++q) {
+      const double d2 = hn[2 * nrhs + q], x2 = hn[nrhs + q];
+      rel = std::max(rel, x2 > 0 ? sqrt(d2 / x2) : sqrt(d2));
     }
+    // stop when the correction is at working precision or stops halving
+    const bool small = rel <= 0x1p-50;
+    const bool stalled = it > 1 && !(rel < 0.5 * prev);
+    prev = rel;
+    if (small || stalled) break;
   }
   *iters = it;
   *berr = eta;

@@ -195,6 +195,19 @@ hipError_t gemm32(char ta, char tb, int M, int N, int K, double alpha, const dou
     ns = (K + kslice - 1) / kslice;
   }
   double* pp = (ns > 1) ? part : nullptr;
+  // N/T times N: the double-buffered FP32-MFMA kernel (rnpx_dgemm.hip)
+  {
+    const hipError_t e = sgemm_mfma(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, pp, st);
+    if (e != hipErrorNotSupported) {
+      if (e != hipSuccess) return e;
+      if (ns > 1) {
+        const long tot = (long)M * N;
+        hipLaunchKernelGGL(reduce64_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, M, N, ns, part,
+                           alpha, beta, C, ldc);
+      }
+      return hipGetLastError();
+    }
+  }
   if (ta == 'N' && tb == 'N') launch32<'N', 'N'>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, pp, st);
   else if (ta == 'N') launch32<'N', 'T'>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, pp, st);
   else if (tb == 'N') launch32<'T', 'N'>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, pp, st);

@@ -38,6 +38,29 @@ hipError_t Buffer::ensure(size_t b, Handle* h) {
   bytes = want;
   return hipSuccess;
 }
+hipError_t HostBuffer::ensure(size_t b, Handle* h) {
+  if (b <= bytes) return hipSuccess;
+  if (p) {
+    hipError_t e = h ? h->sync_all_streams() : hipDeviceSynchronize();
+    if (e != hipSuccess) return e;
+    hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  const size_t want = std::max<size_t>(b, 4096);
+  hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return e;
+  }
+  bytes = want;
+  return hipSuccess;
+}
+void HostBuffer::release() {
+  if (p) hipHostFree(p);
+  p = nullptr;
+  bytes = 0;
+}
 void Buffer::release() {
   if (p) hipFree(p);
   p = nullptr;
@@ -155,6 +178,7 @@ Handle::~Handle() {
     }
   }
   for (Buffer* b : buffers()) b->release();
+  hpin.release();
   if (own_stream && own) hipStreamDestroy(own);
 }
 
@@ -257,7 +281,9 @@ static int pick_ksplit(const Handle* h, int rows, int cols_tiles) {
   // panels, where latency matters more than the partial sums' traffic.
   int want = (cols_tiles <= 2 ? 1024 : 512) / std::max(1, cols_tiles);
   int maxs = std::max(1, rows / (cols_tiles <= 2 ? 128 : 512));
-  return std::max(1, std::min(want, std::min(maxs, 32)));
+  // narrow products: up to 64 slices (a tall panel's W is otherwise a long
+  // chain of K slices on few workgroups)
+  return std::max(1, std::min(want, std::min(maxs, cols_tiles <= 2 ? 64 : 32)));
 }
 
 // ---------------------------------------------------------------------------

@@ -1,5 +1,5 @@
 """The hand-written MFMA products (rnpx_dgemm.hip real, rnpx_zgemm.hip
-complex) against numpy: op(A) in N/T/C times B for the alpha/beta forms the
+complex) and the few-column kernel (rnpx_gemv.hip) against numpy: op(A) in N/T/C times B for the alpha/beta forms the
 library takes (alpha = +-1 with beta = 1 into the accumulators, beta = 0,
 general), ragged shapes (tails in every dimension, a block of reflectors as
 M), and the split-K path through the block-reflector apply (W = V^H C split
@@ -48,6 +48,57 @@ def test_gemm_forms(cplx, ta):
                 got = C.cpu().numpy()
                 scale = k * np.abs(a).max() * np.abs(b).max() + np.abs(c).max()
                 assert np.abs(got - want).max() <= 1e-13 * scale
+    finally:
+        h.close()
+
+
+GEMV_SHAPES = [(1, 1, 1), (300, 1, 5000), (16384, 1, 2048), (2048, 4, 16384), (517, 3, 67), (70, 2, 0),
+               (4099, 4, 31), (256, 1, 256)]
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("ta", ["N", "T", "C"])
+def test_gemv_forms(cplx, ta):
+    """One to four columns of B: the HBM-bound kernel with its reduction
+    split in slices (row blocks x K chunks for N, columns x row chunks for
+    T/C) added in a fixed order, K = 0 included (C = beta C)."""
+    rng = np.random.default_rng(11)
+    h = R.Handle()
+    try:
+        for (m, n, k) in GEMV_SHAPES:
+            for alpha, beta in FORMS:
+                if not cplx:
+                    alpha, beta = complex(alpha).real, complex(beta).real
+                def rnd(r, c):
+                    x = rng.standard_normal((r, c))
+                    return x + 1j * rng.standard_normal((r, c)) if cplx else x
+                a = rnd(m, k) if ta == "N" else rnd(k, m)
+                b, c = rnd(k, n), rnd(m, n)
+                print(f"[case] gemv cplx={cplx} op={ta} {m}x{n}x{k} alpha={alpha} beta={beta}", flush=True)
+                A = dev(a) if a.size else R.colmajor_empty(a.shape[0], a.shape[1], torch.complex128 if cplx else torch.float64)
+                B = dev(b) if b.size else R.colmajor_empty(b.shape[0], b.shape[1], torch.complex128 if cplx else torch.float64)
+                C = dev(c)
+                h.gemm(ta, "N", alpha, A, B, beta, C)
+                h.synchronize()
+                opa = a if ta == "N" else (a.T if ta == "T" else a.conj().T)
+                want = alpha * (opa @ b) + beta * c
+                got = C.cpu().numpy()
+                amax = np.abs(a).max() if a.size else 0.0
+                bmax = np.abs(b).max() if b.size else 0.0
+                scale = k * amax * bmax + np.abs(c).max()
+                assert np.abs(got - want).max() <= 1e-13 * scale
+            # beta = 0 does not read C: NaN in C is cleared
+            c = np.full((m, n), np.nan) * (1 + 1j if cplx else 1)
+            a = rnd(m, k) if ta == "N" else rnd(k, m)
+            b = rnd(k, n)
+            if k == 0:
+                continue
+            A, B, C = dev(a), dev(b), dev(c)
+            h.gemm(ta, "N", 1.0, A, B, 0.0, C)
+            h.synchronize()
+            opa = a if ta == "N" else (a.T if ta == "T" else a.conj().T)
+            assert np.isfinite(C.cpu().numpy()).all()
+            assert np.abs(C.cpu().numpy() - opa @ b).max() <= 1e-13 * k * np.abs(a).max() * np.abs(b).max()
     finally:
         h.close()
 
