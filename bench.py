@@ -532,6 +532,13 @@ def main():
         "panel_hbm_gbs": round(pan["bytes"] / (pan["ms"] * 1e-3) / 1e9, 1) if pan["ms"] > 0 else None,
         "check": {"ok": ok, "residual": round(res, 4), "orthogonality": round(orth, 4)},
     }
+    # Each extra leg runs on a handle of its own, as in a process of its own:
+    # the headline's handle (two streams) is closed first, so that the legs'
+    # streams do not share the runtime's few hardware queues with idle ones.
+    h.synchronize()
+    h.close()
+    del a, a0, tau
+    torch.cuda.synchronize()
     legs = [] if args.legs == "none" else (["complex", "tall", "batched", "mixed", "serving", "lq", "qlrq", "lu"]
                                            if args.legs == "all" else args.legs.split(","))
     extra = {}

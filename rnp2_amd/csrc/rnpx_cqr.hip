@@ -987,20 +987,28 @@ hipError_t launch_cqr_block(const CqrBlockArgs& a, int G, hipStream_t st) {
 // started, so that a GEMM queued behind the gate on another stream cannot
 // take the CUs those workgroups need first.  Only the order of dispatch
 // depends on it (the GEMM reads nothing the block kernel writes), so the
-// wait gives up after 2 ms.
-__global__ void cqr_gate_kernel(const unsigned* block_sync, unsigned G) {
+// wait gives up after 2 ms.  A wait that gives up sets *off, which turns every
+// later gate of the handle into a no-op: the runtime can map the two
+// streams onto one hardware queue (more streams in the process than
+// queues), and then the block kernel sits behind the gate and every wait
+// would run to its limit.
+__global__ void cqr_gate_kernel(const unsigned* block_sync, unsigned G, unsigned* off) {
   if (threadIdx.x != 0) return;
+  if (__hip_atomic_load(off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
   const unsigned* c = block_sync + kBlkResident;
   const uint64_t t0 = wall_clock64();
   while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < G) {
-    if (wall_clock64() - t0 > 200000ull) break;
+    if (wall_clock64() - t0 > 200000ull) {
+      __hip_atomic_store(off, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
     __builtin_amdgcn_s_sleep(1);
   }
 }
 
-hipError_t launch_cqr_gate(const unsigned* block_sync, int G, hipStream_t st) {
-  if (!block_sync || G < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(cqr_gate_kernel, dim3(1), dim3(64), 0, st, block_sync, (unsigned)G);
+hipError_t launch_cqr_gate(const unsigned* block_sync, int G, unsigned* off, hipStream_t st) {
+  if (!block_sync || !off || G < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(cqr_gate_kernel, dim3(1), dim3(64), 0, st, block_sync, (unsigned)G, off);
   return hipGetLastError();
 }
 

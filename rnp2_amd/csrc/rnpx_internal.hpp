@@ -38,6 +38,11 @@ struct ProfRec {
   double flops, bytes;
 };
 
+// Words of the handle's 64-byte flag buffer (zero at creation): [0] the
+// sticky abort flag of the bounded waits, [kGateOffWord] set once a
+// look-ahead gate has given up (later gates return at once).
+constexpr int kGateOffWord = 1;
+
 struct Handle {
   int device = 0;
   int num_cus = 1;
@@ -103,6 +108,9 @@ struct Handle {
 };
 
 int hip_status(hipError_t e);
+// A non-blocking stream on a hardware queue of its own where the runtime
+// allows (a CU mask holding every CU), else a plain one.
+hipError_t create_own_queue_stream(hipStream_t* s, int num_cus);
 
 // Return from the enclosing int-status function on a HIP error (RNPX_TRY) or
 // on a nonzero status (RNPX_CHECK).

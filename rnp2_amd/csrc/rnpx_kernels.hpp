@@ -264,7 +264,7 @@ int cqr_block_groups(int mp, int max_groups);
 hipError_t launch_cqr_block(const CqrBlockArgs& a, int G, hipStream_t st);
 // Waits (bounded) until the G workgroups of the block kernel with these sync
 // words have all started.
-hipError_t launch_cqr_gate(const unsigned* block_sync, int G, hipStream_t st);
+hipError_t launch_cqr_gate(const unsigned* block_sync, int G, unsigned* off, hipStream_t st);
 
 // T factor of a forward, columnwise block reflector from its Gram matrix
 // (the larft analogue, RNP/LA/Reflector.hpp:573-626): k <= NB reflectors,

@@ -80,19 +80,25 @@ template <> __device__ inline zd ldexp_t<zd>(zd v, int e) { return mkz(ldexp(v.x
 template <typename T> struct RedChunk { static constexpr int value = 64; };
 template <> struct RedChunk<zd> { static constexpr int value = 32; };
 
-template <typename T, int NB, int RPT>
+// ONE: a single workgroup (at most 256 RPT rows); the per-column partial
+// vector and the diagonal row stay in LDS instead of crossing global memory.
+template <typename T, int NB, int RPT, bool ONE>
 __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
   // Gated launch: the whole grid reads the same word and returns together,
   // before any barrier.
   if (p.run_if && *p.run_if == 0u) return;
   constexpr int CH = (RedChunk<T>::value < NB) ? RedChunk<T>::value : NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int g = blockIdx.x, G = gridDim.x;
+  const int g = blockIdx.x, G = ONE ? 1 : (int)gridDim.x;
   const int mp = p.mp, nbp = p.nbp;
   const int kp = (mp < nbp) ? mp : nbp;
 
   __shared__ T red[4][NB];
   __shared__ double dsh[4];
+  __shared__ T lpart[ONE ? 2 * NB : 1];
+  __shared__ T lrow[ONE ? 2 * NB : 1];
+  T* const part = ONE ? lpart : p.part;
+  T* const rowbuf = ONE ? lrow : p.rowbuf;
 
   T a[RPT][NB];
   int row[RPT];
@@ -118,7 +124,7 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
       // MAXQ loads are issued back to back before any is consumed.
       constexpr int MAXQ = 8;
       const int j = tid % NB, sub = tid / NB;
-      const T* base = p.part + (size_t)(epoch & 1u) * (size_t)G * NB + j;
+      const T* base = part + (size_t)(epoch & 1u) * (size_t)G * NB + j;
       T s = S<T>::zero();
       for (int g0 = sub; g0 < G; g0 += SUB * MAXQ) {
         T buf[MAXQ];
@@ -147,7 +153,7 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
   };
   auto barrier = [&]() {
     ++epoch;
-    if (G == 1)
+    if (ONE)
       __syncthreads();  // one workgroup: its own partials, no grid hand-off
     else
       grid_barrier(p.counter, p.abortf, epoch * (unsigned)G);
@@ -168,11 +174,11 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
       for (int q = 0; q < RPT; ++q) {
         if (row[q] == c) {
 #pragma unroll
-          for (int j = 0; j < NB; ++j) p.rowbuf[(c & 1) * NB + j] = a[q][j];
+          for (int j = 0; j < NB; ++j) rowbuf[(c & 1) * NB + j] = a[q][j];
         }
       }
       reduce_products<T, NB, RPT, CH>(a, x, below, red,
-                                      p.part + (size_t)(epoch & 1u) * (size_t)G * NB + (size_t)g * NB, lane, wave,
+                                      part + (size_t)(epoch & 1u) * (size_t)G * NB + (size_t)g * NB, lane, wave,
                                       tid);
     }
     barrier();
@@ -182,7 +188,7 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
     for (int u = 0; u < (NB + 63) / 64; ++u) {
       const int j = lane + 64 * u;
       rcj[u] = S<T>::zero();
-      if (j < NB) rcj[u] = p.rowbuf[(c & 1) * NB + j];
+      if (j < NB) rcj[u] = rowbuf[(c & 1) * NB + j];
     }
     gather_all(Sj);
     ++epoch;
@@ -222,13 +228,13 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
       if (lane == 0) dsh[wave] = mx;
       __syncthreads();
       {
-        double* slot = reinterpret_cast<double*>(p.part + (size_t)(epoch & 1u) * (size_t)G * NB) + g;
+        double* slot = reinterpret_cast<double*>(part + (size_t)(epoch & 1u) * (size_t)G * NB) + g;
         if (tid == 0) *slot = fmax(fmax(dsh[0], dsh[1]), fmax(dsh[2], dsh[3]));
       }
       barrier();
       double M = 0.0;
       {
-        const double* slot = reinterpret_cast<const double*>(p.part + (size_t)(epoch & 1u) * (size_t)G * NB);
+        const double* slot = reinterpret_cast<const double*>(part + (size_t)(epoch & 1u) * (size_t)G * NB);
         for (int gg = 0; gg < G; ++gg) M = fmax(M, slot[gg]);
       }
       ++epoch;
@@ -244,13 +250,13 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
       if (lane == 0) dsh[wave] = ps;
       __syncthreads();
       {
-        double* slot = reinterpret_cast<double*>(p.part + (size_t)(epoch & 1u) * (size_t)G * NB) + g;
+        double* slot = reinterpret_cast<double*>(part + (size_t)(epoch & 1u) * (size_t)G * NB) + g;
         if (tid == 0) *slot = (dsh[0] + dsh[1]) + (dsh[2] + dsh[3]);
       }
       barrier();
       double ss = 0.0;
       {
-        const double* slot = reinterpret_cast<const double*>(p.part + (size_t)(epoch & 1u) * (size_t)G * NB);
+        const double* slot = reinterpret_cast<const double*>(part + (size_t)(epoch & 1u) * (size_t)G * NB);
         for (int gg = 0; gg < G; ++gg) ss += slot[gg];
       }
       ++epoch;
@@ -280,7 +286,7 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
         for (int q = 0; q < RPT; ++q) vq[q] = ldexp_t(x[q], escale) * inv;
         __syncthreads();
         reduce_products<T, NB, RPT, CH>(a, vq, below, red,
-                                        p.part + (size_t)(epoch & 1u) * (size_t)G * NB + (size_t)g * NB, lane, wave,
+                                        part + (size_t)(epoch & 1u) * (size_t)G * NB + (size_t)g * NB, lane, wave,
                                         tid);
         barrier();
         __syncthreads();
@@ -350,7 +356,10 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<T> p) {
 
 template <typename T, int NB, int RPT>
 static hipError_t launch_panel_t(const PanelArgs<T>& a, int G, hipStream_t st) {
-  hipLaunchKernelGGL((panel_kernel<T, NB, RPT>), dim3(G), dim3(256), 0, st, a);
+  if (G == 1)
+    hipLaunchKernelGGL((panel_kernel<T, NB, RPT, true>), dim3(1), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((panel_kernel<T, NB, RPT, false>), dim3(G), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -106,13 +106,29 @@ void Handle::prof_end(int kid, hipEvent_t a, double flops, double bytes, hipStre
   hipEventRecord(b, s ? s : stream);
   recs.push_back(ProfRec{kid, a, b, flops, bytes});
 }
+// A stream with a CU mask (here every CU) gets a hardware queue of its own,
+// where a plain one may share a queue with other streams of the process once
+// they outnumber the runtime's queues -- with the update stream among them,
+// the look-ahead would run in series.  A plain non-blocking stream where the
+// mask is refused.
+hipError_t create_own_queue_stream(hipStream_t* s, int num_cus) {
+  const int words = (num_cus + 31) / 32;
+  std::vector<uint32_t> all(words, 0u);
+  for (int c = 0; c < num_cus; ++c) all[c / 32] |= 1u << (c % 32);
+  hipError_t e = hipExtStreamCreateWithCUMask(s, (uint32_t)words, all.data());
+  if (e == hipSuccess) return e;
+  (void)hipGetLastError();
+  *s = nullptr;
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
 int Handle::ensure_pipeline() {
   if (pstream) return RNPX_OK;
   hipError_t e = hipSuccess;
   if (ext_pstream) {
     pstream = ext_pstream;
   } else {
-    e = hipStreamCreateWithFlags(&pstream, hipStreamNonBlocking);
+    e = create_own_queue_stream(&pstream, num_cus);
     if (e != hipSuccess) return hip_status(e);
   }
   for (hipEvent_t* ev : {&ev_pan[0], &ev_pan[1], &ev_next, &ev_start, &ev_rest}) {
@@ -203,7 +219,9 @@ int create_handle(Handle** out, size_t budget) {
   hipGetDevice(&h->device);
   hipDeviceGetAttribute(&h->num_cus, hipDeviceAttributeMultiprocessorCount, h->device);
   if (h->num_cus < 1) h->num_cus = 1;
-  hipError_t e = hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking);
+  // the handle's stream, on a hardware queue of its own (work of other
+  // handles in the process does not queue in front of it)
+  hipError_t e = create_own_queue_stream(&h->own, h->num_cus);
   if (e != hipSuccess) {
     delete h;
     return hip_status(e);
@@ -767,8 +785,9 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
       // is the only work of the other stream it waits for.
       if (b > 0) RNPX_TRY(hipStreamWaitEvent(ps, h->ev_rest, 0));
       if ((s = update(b, c0, c0 + wn, &panel_work))) return s;
-      if ((s = panel_phase(b + 1, ps, &panel_work))) return s;
-      RNPX_TRY(hipEventRecord(h->ev_pan[(b + 1) & 1], ps));
+      // The rest of block b's update is queued before panel b+1's launches:
+      // it depends only on panel b, and queued after them it would start
+      // only once the host had issued all of them.
       if constexpr (!S<T>::cplx) {
         // Once the panels are the longer chain, the rest of the update waits
         // until every workgroup of block b+1's kernel has a CU.
@@ -778,14 +797,18 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
         const int Gb = use_block ? cqr_block_groups(m - c0, std::min(256, h->num_cus)) : 0;
         if (wn >= 2 * nb && Gb > 0 && m - c0 <= gate_rows && c0 + wn < n && !h->ext_pstream)
           RNPX_TRY(launch_cqr_gate((const unsigned*)h->sync.p + 1 + 2 * npan + (size_t)kCqrSyncWords * (c0 / nb), Gb,
-                                   us));
+                                   (unsigned*)h->flag.p + kGateOffWord, us));
       }
       if ((s = update(b, c0 + wn, n, &upd_work))) return s;
-    } else {
-      if ((s = update(b, c0, n, &upd_work))) return s;
+      if ((s = update_rhs(b, &upd_work))) return s;
+      // block b's V and T are free again once this stream is past here
+      RNPX_TRY(hipEventRecord(h->ev_rest, us));
+      if ((s = panel_phase(b + 1, ps, &panel_work))) return s;
+      RNPX_TRY(hipEventRecord(h->ev_pan[(b + 1) & 1], ps));
+      continue;
     }
+    if ((s = update(b, c0, n, &upd_work))) return s;
     if ((s = update_rhs(b, &upd_work))) return s;
-    // block b's V and T are free again once this stream is past here
     RNPX_TRY(hipEventRecord(h->ev_rest, us));
   }
   if (us != st) {

@@ -272,7 +272,9 @@ int rnpx_server_create(const rnpx_server_config* cfg, rnpx_server* out) {
   s->cfg = *cfg;
   if (s->cfg.max_iter <= 0) s->cfg.max_iter = 30;
   if (s->cfg.tol <= 0) s->cfg.tol = 1e-13;
-  if (hipStreamCreateWithFlags(&s->panel_stream, hipStreamNonBlocking) != hipSuccess) {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (create_own_queue_stream(&s->panel_stream, cus > 0 ? cus : 256) != hipSuccess) {
     delete s;
     return RNPX_ERR_HIP;
   }

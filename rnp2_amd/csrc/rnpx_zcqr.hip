@@ -35,233 +35,322 @@ namespace rnpx {
 namespace zq {
 
 constexpr int kThreads = 256;  // the commit kernel
-// The small factorizations: one 256-thread workgroup, the matrices in LDS
-// (measured: one wave is slower -- the work, not the barriers, is the cost).
+// The small factorizations: one 256-thread workgroup; thread t holds rows
+// (t & 7) + 8 r (r < 4) of column t >> 3 of the 32 x 32 matrix being worked
+// on in registers, so the eight lanes of a column sit in one wave and share
+// values by shuffles; a matrix other columns read is staged in LDS.  A
+// Cholesky or LU step costs one workgroup barrier, a triangular inverse or
+// product none.
 constexpr int kSmall = 256;
+constexpr int NB = 32;
+constexpr int R4 = NB / 8;  // rows per thread
+using Mat = zd[NB][NB + 1];
 
 __device__ inline bool zfinite(zd v) { return isfinite(v.x) && isfinite(v.y); }
 __device__ inline double zabs2(zd v) { return v.x * v.x + v.y * v.y; }
 __device__ inline zd zconj(zd v) { return mkz(v.x, -v.y); }
-__device__ inline zd zdiv(zd a, zd b) {  // a / b, Smith's scaling
-  return a * zrecip(b);
+
+// v from lane src (0..7) of this lane's group of eight.
+__device__ inline zd gbcast(zd v, int src) { return mkz(__shfl(v.x, src, 8), __shfl(v.y, src, 8)); }
+
+// Column c of the column-major NB x NB matrix at P (leading dimension ld).
+__device__ inline void load_col(const zd* __restrict__ P, long ld, int c, int sub, zd (&v)[R4]) {
+#pragma unroll
+  for (int r = 0; r < R4; ++r) v[r] = P[(size_t)(sub + 8 * r) + (size_t)c * ld];
+}
+__device__ inline void store_col(zd* __restrict__ P, long ld, int c, int sub, const zd (&v)[R4]) {
+#pragma unroll
+  for (int r = 0; r < R4; ++r) P[(size_t)(sub + 8 * r) + (size_t)c * ld] = v[r];
+}
+__device__ inline void to_lds(Mat& M, int c, int sub, const zd (&v)[R4]) {
+#pragma unroll
+  for (int r = 0; r < R4; ++r) M[sub + 8 * r][c] = v[r];
 }
 
-// In place: G (Hermitian, NB x NB in LDS) -> its upper Cholesky factor R
-// (R^H R = G) in the upper triangle, the lower triangle cleared.  Returns
+// In registers: column c of the Hermitian G (its upper part, rows <= c) ->
+// column c of its upper Cholesky factor R (R^H R = G), zeros below.
+// Right-looking; row j of the updated G goes through rowb[j & 1].  Returns
 // false (uniformly) on a pivot below rel * its original diagonal or one not
-// finite.  Whole workgroup.
-template <int NB>
-__device__ bool chol_upper(zd (*G)[NB + 1], double rel, double* dsh) {
-  const int t = threadIdx.x;
-  if (t < NB) dsh[t] = G[t][t].x;
+// finite.
+__device__ __forceinline__ bool chol_cols(zd (&g)[R4], int c, int sub, double rel, zd (*rowb)[NB], double* dinit) {
+#pragma unroll
+  for (int r = 0; r < R4; ++r)
+    if (sub + 8 * r == c) dinit[c] = g[r].x;
+  if (sub == 0) rowb[0][c] = g[0];
   __syncthreads();
+#pragma unroll 1
   for (int j = 0; j < NB; ++j) {
-    const double d = G[j][j].x;
-    if (!(d > rel * dsh[j]) || !isfinite(d)) return false;  // uniform: every thread reads the same value
-    const double r = sqrt(d);
-    __syncthreads();
-    if (t == 0) G[j][j] = mkz(r, 0.0);
-    for (int l = j + 1 + t; l < NB; l += kSmall) G[j][l] = G[j][l] * (1.0 / r);
-    __syncthreads();
-    // G(i, l) -= conj(R(j, i)) R(j, l) for j < i <= l
-    const int rest = NB - j - 1;
-    for (int e = t; e < rest * rest; e += kSmall) {
-      const int i = j + 1 + e / rest, l = j + 1 + e % rest;  // consecutive threads along a row
-      if (i <= l) G[i][l] = G[i][l] - zconj(G[j][i]) * G[j][l];
+    const zd* rb = rowb[j & 1];
+    const double d = rb[j].x;
+    if (!(d > rel * dinit[j]) || !isfinite(d)) return false;  // every thread reads the same value
+    const double rr = sqrt(d), inv = 1.0 / rr;
+    const zd rjc = rb[c] * inv;  // R(j, c), c > j
+#pragma unroll
+    for (int r = 0; r < R4; ++r) {
+      const int i = sub + 8 * r;
+      if (i == j)
+        g[r] = (c > j) ? rjc : (c == j ? mkz(rr, 0.0) : S<zd>::zero());
+      else if (i > j && i <= c)
+        g[r] = g[r] - zconj(rb[i] * inv) * rjc;
+    }
+    if (j + 1 < NB) {
+#pragma unroll
+      for (int r = 0; r < R4; ++r)
+        if (sub + 8 * r == j + 1 && c >= j + 1) rowb[(j + 1) & 1][c] = g[r];
     }
     __syncthreads();
   }
-  for (int e = t; e < NB * NB; e += kSmall) {
-    const int i = e / NB, l = e % NB;
-    if (i > l) G[i][l] = S<zd>::zero();
-  }
-  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R4; ++r)
+    if (sub + 8 * r > c) g[r] = S<zd>::zero();
   return true;
 }
 
-// X = U^{-1} for upper-triangular U (NB x NB in LDS); thread c back-
-// substitutes column c.
-template <int NB>
-__device__ void inv_upper(const zd (*U)[NB + 1], zd (*X)[NB + 1]) {
-  const int c = threadIdx.x;
-  if (c < NB) {
-    for (int i = c + 1; i < NB; ++i) X[i][c] = S<zd>::zero();
-    X[c][c] = zrecip(U[c][c]);
-    for (int i = c - 1; i >= 0; --i) {
-      zd s = S<zd>::zero();
-      for (int k = i + 1; k <= c; ++k) s = s + U[i][k] * X[k][c];
-      X[i][c] = S<zd>::zero() - zdiv(s, U[i][i]);
+// Column c of X = U^{-1} for the upper triangular U in LDS (entries below
+// its diagonal never read), by the column sweep y = e_c, x_k = y_k / U_kk,
+// y_i -= U_ik x_k (i < k), k = NB-1 .. 0.  unit: U_kk = 1.  conjT: U_ik is
+// conj(L(k, i)) of the lower triangular L stored there instead.
+// The reciprocals of U's diagonal are formed first, in parallel, into rdiag
+// (LDS), off the sweep's serial chain.  Whole workgroup.
+template <bool UNIT, bool CONJT>
+__device__ __forceinline__ void inv_upper_col(const Mat& U, int c, int sub, zd (&x)[R4], zd* rdiag) {
+  if constexpr (!UNIT) {
+    __syncthreads();  // earlier readers of rdiag are done
+    if (threadIdx.x < NB) rdiag[threadIdx.x] = zrecip(U[threadIdx.x][threadIdx.x]);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < R4; ++r) x[r] = (sub + 8 * r == c) ? S<zd>::one() : S<zd>::zero();
+  // k = 8 rk + s: rk unrolled (a fixed register), s not
+#pragma unroll
+  for (int rk = R4 - 1; rk >= 0; --rk) {
+#pragma unroll 1
+    for (int s8 = 7; s8 >= 0; --s8) {
+      const int k = 8 * rk + s8;
+      zd xk = x[rk];
+      if constexpr (!UNIT) xk = xk * rdiag[k];
+      xk = gbcast(xk, s8);
+#pragma unroll
+      for (int r = 0; r < R4; ++r) {
+        const int i = sub + 8 * r;
+        if (i == k)
+          x[r] = xk;
+        else if (i < k)
+          x[r] = x[r] - (CONJT ? zconj(U[k][i]) : U[i][k]) * xk;
+      }
     }
   }
-  __syncthreads();
 }
 
-// C = A B for NB x NB matrices in LDS (any structure; C distinct from A, B).
-template <int NB>
-__device__ void mul(const zd (*A)[NB + 1], const zd (*B)[NB + 1], zd (*C)[NB + 1]) {
-  for (int e = threadIdx.x; e < NB * NB; e += kSmall) {
-    const int i = e / NB, l = e % NB;  // consecutive threads along a row of C and B
-    zd s = S<zd>::zero();
-    for (int k = 0; k < NB; ++k) s = s + A[i][k] * B[k][l];
-    C[i][l] = s;
+// Column c of A B for A in LDS (row scale ds[k] on column k of A when
+// given) and column c of B in registers.
+__device__ __forceinline__ void mul_col(const Mat& A, const zd (&b)[R4], int sub, zd (&out)[R4], const double* ds = nullptr) {
+#pragma unroll
+  for (int r = 0; r < R4; ++r) out[r] = S<zd>::zero();
+#pragma unroll
+  for (int rk = 0; rk < R4; ++rk) {
+#pragma unroll 2
+    for (int s8 = 0; s8 < 8; ++s8) {
+      const int k = 8 * rk + s8;
+      const zd bk = gbcast(b[rk], s8);
+#pragma unroll
+      for (int r = 0; r < R4; ++r) {
+        const zd aik = ds ? A[sub + 8 * r][k] * ds[k] : A[sub + 8 * r][k];
+        out[r] = out[r] + aik * bk;
+      }
+    }
   }
-  __syncthreads();
 }
 
 // Pass 1: R1 = chol(G1) and R1^{-1}, or the fail flag.
-template <int NB>
 __global__ __launch_bounds__(kSmall) void chol_kernel(const zd* __restrict__ G, zd* __restrict__ R1,
-                                                        zd* __restrict__ R1i, unsigned* fail) {
-  __shared__ zd Gs[NB][NB + 1];
-  __shared__ zd Xs[NB][NB + 1];
-  __shared__ double dsh[NB];
-  __shared__ int bad;
-  const int t = threadIdx.x;
-  if (t == 0) bad = 0;
-  __syncthreads();
-  for (int e = t; e < NB * NB; e += kSmall) {
-    const int i = e % NB, l = e / NB;
-    const zd g = G[(size_t)i + (size_t)l * NB];
-    Gs[i][l] = g;
-    if (!zfinite(g)) bad = 1;
-    if (i == l && !(g.x >= 0x1p-600 && g.x <= 0x1p600)) bad = 1;
+                                                      zd* __restrict__ R1i, unsigned* fail) {
+  __shared__ Mat Rs;
+  __shared__ zd rowb[2][NB];
+  __shared__ double dinit[NB];
+  const int t = threadIdx.x, c = t >> 3, sub = t & 7;
+  zd g[R4];
+  load_col(G, NB, c, sub, g);
+  int bad = 0;
+#pragma unroll
+  for (int r = 0; r < R4; ++r) {
+    const int i = sub + 8 * r;
+    if (!zfinite(g[r])) bad = 1;
+    if (i == c && !(g[r].x >= 0x1p-600 && g[r].x <= 0x1p600)) bad = 1;
   }
-  __syncthreads();
-  bool ok = bad == 0;
-  if (ok) ok = chol_upper<NB>(Gs, 0x1p-40, dsh);
+  bool ok = __syncthreads_or(bad) == 0;
+  if (ok) ok = chol_cols(g, c, sub, 0x1p-40, rowb, dinit);
   if (!ok) {
     if (t == 0) *fail = 1u;
     return;
   }
-  inv_upper<NB>(Gs, Xs);
-  for (int e = t; e < NB * NB; e += kSmall) {
-    const int i = e % NB, l = e / NB;
-    R1[(size_t)i + (size_t)l * NB] = Gs[i][l];
-    R1i[(size_t)i + (size_t)l * NB] = Xs[i][l];
-  }
+  store_col(R1, NB, c, sub, g);
+  to_lds(Rs, c, sub, g);
+  __syncthreads();
+  zd x[R4];
+  inv_upper_col<false, false>(Rs, c, sub, x, rowb[0]);
+  store_col(R1i, NB, c, sub, x);
 }
 
 // Pass 2 and the reconstruction.  In: G2 = Q1^H Q1, R1, the top NB rows of
 // Q1.  Out: tau, T (ldt), the clean V's top NB rows (ldv), Atop = R (upper)
 // and L1 (strictly lower) for the commit, M = R2^{-1} U^{-1} for the rest of
 // V; or the fail flag.
-template <int NB>
 __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2, const zd* __restrict__ R1,
-                                                         const zd* __restrict__ Q1, long ldq, zd* __restrict__ tau,
-                                                         zd* __restrict__ Tm, long ldt, zd* __restrict__ V, long ldv,
-                                                         zd* __restrict__ Atop, zd* __restrict__ M, unsigned* fail) {
-  __shared__ zd S1[NB][NB + 1];  // G2 -> R2
-  __shared__ zd S2[NB][NB + 1];  // R2^{-1}
-  __shared__ zd S3[NB][NB + 1];  // scratch: R1, then Q_top -> L1\U, ...
-  __shared__ zd S4[NB][NB + 1];
-  __shared__ double dsh[NB];
+                                                       const zd* __restrict__ Q1, long ldq, zd* __restrict__ tau,
+                                                       zd* __restrict__ Tm, long ldt, zd* __restrict__ V, long ldv,
+                                                       zd* __restrict__ Atop, zd* __restrict__ M, unsigned* fail) {
+  __shared__ Mat S1;  // R2
+  __shared__ Mat S2;  // R2^{-1}
+  __shared__ Mat S3;  // Q1's top rows
+  __shared__ Mat S4;  // L1 \ U
+  __shared__ zd rowb[2][NB];
+  __shared__ double dinit[NB];
   __shared__ double Ds[NB];
-  __shared__ int bad;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, c = t >> 3, sub = t & 7;
   if (*fail) return;  // pass 1 rejected the panel (uniform)
-  if (t == 0) bad = 0;
-  __syncthreads();
-  for (int e = t; e < NB * NB; e += kSmall) {
-    const int i = e % NB, l = e / NB;
-    const zd g = G2[(size_t)i + (size_t)l * NB];
-    S1[i][l] = g;
-    const zd eij = (i == l) ? mkz(g.x - 1.0, g.y) : g;
-    if (!zfinite(g) || zabs2(eij) > 0x1p-54) bad = 1;  // |E_ij| > 2^-27
+  zd g[R4];
+  load_col(G2, NB, c, sub, g);
+  int bad = 0;
+#pragma unroll
+  for (int r = 0; r < R4; ++r) {
+    const int i = sub + 8 * r;
+    const zd eij = (i == c) ? mkz(g[r].x - 1.0, g[r].y) : g[r];
+    if (!zfinite(g[r]) || zabs2(eij) > 0x1p-54) bad = 1;  // |E_ij| > 2^-27
   }
-  __syncthreads();
-  bool ok = bad == 0;
-  if (ok) ok = chol_upper<NB>(S1, 0x1p-2, dsh);
+  // Q1's top rows to LDS (row-major) while the Gram is checked
+#pragma unroll
+  for (int r = 0; r < R4; ++r) S3[sub + 8 * r][c] = Q1[(size_t)(sub + 8 * r) + (size_t)c * ldq];
+  int big = 0;  // |E_ij| > 2^-40: R2 by Cholesky
+#pragma unroll
+  for (int r = 0; r < R4; ++r) {
+    const int i = sub + 8 * r;
+    const zd eij = (i == c) ? mkz(g[r].x - 1.0, g[r].y) : g[r];
+    if (zabs2(eij) > 0x1p-80) big = 1;
+  }
+  bool ok = __syncthreads_or(bad) == 0;
+  const bool first_order = __syncthreads_or(big) == 0;
+  zd x2[R4];
+  if (first_order) {
+    // R2 = I + E_U, E_U = triu(E) - diag(E)/2, and R2^{-1} = I - E_U, both
+    // exact to O(|E|^2) <= 2^-80
+#pragma unroll
+    for (int r = 0; r < R4; ++r) {
+      const int i = sub + 8 * r;
+      if (i < c) {
+        x2[r] = S<zd>::zero() - g[r];
+      } else if (i == c) {
+        const double e2 = 0.5 * (g[r].x - 1.0);
+        g[r] = mkz(1.0 + e2, 0.0);
+        x2[r] = mkz(1.0 - e2, 0.0);
+      } else {
+        g[r] = S<zd>::zero();
+        x2[r] = S<zd>::zero();
+      }
+    }
+    to_lds(S1, c, sub, g);
+  } else {
+    if (ok) ok = chol_cols(g, c, sub, 0x1p-2, rowb, dinit);
+    if (!ok) {
+      if (t == 0) *fail = 1u;
+      return;
+    }
+    to_lds(S1, c, sub, g);  // R2
+    inv_upper_col<false, false>(S1, c, sub, x2, rowb[0]);  // column c of R2^{-1}
+  }
   if (!ok) {
     if (t == 0) *fail = 1u;
     return;
   }
-  inv_upper<NB>(S1, S2);  // S2 = R2^{-1}
-  // S4 = Q_top = Q1_top R2^{-1}
-  for (int e = t; e < NB * NB; e += kSmall) {
-    const int i = e % NB, l = e / NB;
-    S3[i][l] = Q1[(size_t)i + (size_t)l * ldq];
-  }
+  to_lds(S2, c, sub, x2);
   __syncthreads();
-  mul<NB>(S3, S2, S4);
-  // LU without pivoting of Q_top - D in S4 (L1 strictly below, U on and above)
-  for (int j = 0; j < NB; ++j) {
-    const zd piv = S4[j][j];
-    const double dj = piv.x > 0.0 ? -1.0 : 1.0;  // uniform
-    __syncthreads();
-    if (t == 0) {
-      Ds[j] = dj;
-      S4[j][j] = mkz(piv.x - dj, piv.y);
-    }
-    __syncthreads();
-    const zd rp = zrecip(S4[j][j]);
-    for (int i = j + 1 + t; i < NB; i += kSmall) S4[i][j] = S4[i][j] * rp;
-    __syncthreads();
-    const int rest = NB - j - 1;
-    for (int e = t; e < rest * rest; e += kSmall) {
-      const int i = j + 1 + e / rest, l = j + 1 + e % rest;
-      S4[i][l] = S4[i][l] - S4[i][j] * S4[j][l];
-    }
-    __syncthreads();
-  }
-  // R = D R2 R1 -> Atop (upper), L1 (strictly lower) -> Atop
-  for (int e = t; e < NB * NB; e += kSmall) {
-    const int i = e % NB, l = e / NB;
-    S3[i][l] = R1[(size_t)i + (size_t)l * NB];
-  }
-  __syncthreads();
-  {
-    // Atop(i, l) = D_i (R2 R1)(i, l) for i <= l, L1(i, l) for i > l
-    for (int e = t; e < NB * NB; e += kSmall) {
-      const int i = e / NB, l = e % NB;
-      zd v;
-      if (i <= l) {
-        zd s = S<zd>::zero();
-        for (int k = i; k <= l; ++k) s = s + S1[i][k] * S3[k][l];
-        v = s * Ds[i];
-      } else {
-        v = S4[i][l];
+  // q = column c of Q_top = Q1_top R2^{-1}
+  zd q[R4];
+  mul_col(S3, x2, sub, q);
+  // LU without pivoting of Q_top - D: L1 strictly below, U on and above;
+  // D_j = -sign(Re pivot) (+1 for a zero real part)
+  zd (*Lc)[NB] = rowb;
+#pragma unroll
+  for (int rj = 0; rj < R4; ++rj)
+#pragma unroll 1
+  for (int s8 = 0; s8 < 8; ++s8) {
+    const int j = 8 * rj + s8;
+    if (c == j) {
+      const zd piv = gbcast(q[rj], s8);
+      const double dj = piv.x > 0.0 ? -1.0 : 1.0;
+      const zd pivp = mkz(piv.x - dj, piv.y);
+      const zd rp = zrecip(pivp);
+#pragma unroll
+      for (int r = 0; r < R4; ++r) {
+        const int i = sub + 8 * r;
+        if (i == j) {
+          q[r] = pivp;
+        } else if (i > j) {
+          q[r] = q[r] * rp;
+          Lc[j & 1][i] = q[r];
+        }
       }
-      Atop[(size_t)i + (size_t)l * NB] = v;
-      // the clean V's top rows: unit lower L1
-      V[(size_t)i + (size_t)l * ldv] = (i > l) ? S4[i][l] : (i == l ? S<zd>::one() : S<zd>::zero());
+      if (sub == 0) Ds[j] = dj;
+    }
+    __syncthreads();
+    if (c > j) {
+      const zd u = gbcast(q[rj], s8);  // U(j, c)
+#pragma unroll
+      for (int r = 0; r < R4; ++r) {
+        const int i = sub + 8 * r;
+        if (i > j) q[r] = q[r] - Lc[j & 1][i] * u;
+      }
     }
   }
+  to_lds(S4, c, sub, q);
+  // R = D R2 R1 (upper) and L1 (strictly lower) -> Atop; V's top rows = unit L1
+  {
+    zd r1[R4], rr[R4];
+    load_col(R1, NB, c, sub, r1);
+    mul_col(S1, r1, sub, rr);
+#pragma unroll
+    for (int r = 0; r < R4; ++r) {
+      const int i = sub + 8 * r;
+      Atop[(size_t)i + (size_t)c * NB] = (i <= c) ? rr[r] * Ds[i] : q[r];
+      V[(size_t)i + (size_t)c * ldv] = (i > c) ? q[r] : (i == c ? S<zd>::one() : S<zd>::zero());
+    }
+  }
+  __syncthreads();  // S4, Ds complete
   if (t < NB) tau[t] = S<zd>::zero() - S4[t][t] * Ds[t];
-  __syncthreads();
-  // S3 = U (upper part of S4), then S1 = U^{-1}
-  for (int e = t; e < NB * NB; e += kSmall) {
-    const int i = e / NB, l = e % NB;
-    S3[i][l] = (i <= l) ? S4[i][l] : S<zd>::zero();
-  }
-  __syncthreads();
-  inv_upper<NB>(S3, S1);  // S1 = U^{-1} (R2 no longer needed: S2 holds R2^{-1})
   // M = R2^{-1} U^{-1}
-  for (int e = t; e < NB * NB; e += kSmall) {
-    const int i = e / NB, l = e % NB;
-    zd s = S<zd>::zero();
-    for (int k = i; k <= l; ++k) s = s + S2[i][k] * S1[k][l];
-    M[(size_t)i + (size_t)l * NB] = (i <= l) ? s : S<zd>::zero();
+  {
+    zd ui[R4], mm[R4];
+    inv_upper_col<false, false>(S4, c, sub, ui, rowb[0]);
+    mul_col(S2, ui, sub, mm);
+#pragma unroll
+    for (int r = 0; r < R4; ++r)
+      if (sub + 8 * r > c) mm[r] = S<zd>::zero();
+    store_col(M, NB, c, sub, mm);
   }
-  __syncthreads();
-  // X = L1^{-1} (unit lower) into S2, thread c forward-substitutes column c
-  if (t < NB) {
-    const int c = t;
-    for (int i = 0; i < c; ++i) S2[i][c] = S<zd>::zero();
-    S2[c][c] = S<zd>::one();
-    for (int i = c + 1; i < NB; ++i) {
-      zd s = S<zd>::zero();
-      for (int k = c; k < i; ++k) s = s + S4[i][k] * S2[k][c];
-      S2[i][c] = S<zd>::zero() - s;
+  // T = -(U D) L1^{-H}: Y = L1^{-H}, the inverse of the unit upper L1^H
+  {
+    zd y[R4], tt[R4];
+    inv_upper_col<true, true>(S4, c, sub, y, rowb[0]);
+    // (U D) with U the upper part of S4: entries below the diagonal masked
+    zd u[R4];
+#pragma unroll
+    for (int r = 0; r < R4; ++r) tt[r] = S<zd>::zero();
+#pragma unroll
+    for (int rl = 0; rl < R4; ++rl)
+#pragma unroll 2
+    for (int s8 = 0; s8 < 8; ++s8) {
+      const int l = 8 * rl + s8;
+      const zd yl = gbcast(y[rl], s8);
+#pragma unroll
+      for (int r = 0; r < R4; ++r) {
+        const int i = sub + 8 * r;
+        u[r] = (l >= i) ? S4[i][l] * Ds[l] : S<zd>::zero();
+        tt[r] = tt[r] + u[r] * yl;
+      }
     }
-  }
-  __syncthreads();
-  // T = -(U D) L1^{-H}: T(i, k) = -sum_{l = i..k} U(i, l) D_l conj(X(k, l))
-  for (int e = t; e < NB * NB; e += kSmall) {
-    const int i = e / NB, k = e % NB;
-    zd s = S<zd>::zero();
-    if (i <= k)
-      for (int l = i; l <= k; ++l) s = s + (S3[i][l] * Ds[l]) * zconj(S2[k][l]);
-    Tm[(size_t)i + (size_t)k * ldt] = S<zd>::zero() - s;
+#pragma unroll
+    for (int r = 0; r < R4; ++r) tt[r] = S<zd>::zero() - tt[r];
+    store_col(Tm, ldt, c, sub, tt);
   }
 }
 
@@ -301,12 +390,12 @@ int zcqr_panel(Handle* h, int mp, int nb, zd* A, long lda, zd* tau, zd* V, long 
   const zd one = S<zd>::one(), zero = S<zd>::zero();
   // pass 1
   RNPX_TRY(gemm<zd>('C', 'N', nb, nb, mp, one, A, lda, A, lda, zero, G, nb, ks, part, st));
-  hipLaunchKernelGGL((zq::chol_kernel<32>), dim3(1), dim3(zq::kSmall), 0, st, G, R1, R1i, fail);
+  hipLaunchKernelGGL(zq::chol_kernel, dim3(1), dim3(zq::kSmall), 0, st, G, R1, R1i, fail);
   RNPX_TRY(hipGetLastError());
   RNPX_TRY(gemm<zd>('N', 'N', mp, nb, nb, one, A, lda, R1i, nb, zero, Q1, mp, 1, nullptr, st));
   // pass 2 and the reconstruction
   RNPX_TRY(gemm<zd>('C', 'N', nb, nb, mp, one, Q1, mp, Q1, mp, zero, G, nb, ks, part, st));
-  hipLaunchKernelGGL((zq::recon_kernel<32>), dim3(1), dim3(zq::kSmall), 0, st, G, R1, Q1, (long)mp, tau, Tm, ldt,
+  hipLaunchKernelGGL(zq::recon_kernel, dim3(1), dim3(zq::kSmall), 0, st, G, R1, Q1, (long)mp, tau, Tm, ldt,
                      V, ldv, Atop, M, fail);
   RNPX_TRY(hipGetLastError());
   // the rest of V: Q1_bot R2^{-1} U^{-1}
