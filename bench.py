@@ -386,10 +386,16 @@ def leg_serving(R, torch, count=120, rate=40.0, seed=5, mode=1):
         arrivals.append(t_next)
         ids.append(srv.submit(a, b)[1])
     lat, worst, paths = [], 0.0, {}
+    wait, service, by_path, flops = [], [], {}, []
     t_end = 0.0
     for (a, b, _), arr, rid in zip(reqs, arrivals, ids):
         x, info = srv.poll(rid)
         lat.append(info.t_done - arr)
+        wait.append(info.t_admit - info.t_submit)
+        service.append(info.t_done - info.t_admit)
+        by_path.setdefault(R.PATHS[info.path], []).append(info.t_done - arr)
+        mm, nn = a.shape
+        flops.append(2.0 * mm * nn * nn - 2.0 * nn ** 3 / 3.0)
         t_end = max(t_end, info.t_done)
         paths[R.PATHS[info.path]] = paths.get(R.PATHS[info.path], 0) + 1
         r = b - a @ x
@@ -399,10 +405,24 @@ def leg_serving(R, torch, count=120, rate=40.0, seed=5, mode=1):
     stats = srv.stats()
     srv.close()
     lat = np.array(lat) * 1e3
+    pct = lambda v, q: round(float(np.percentile(np.asarray(v) * 1e3, q)), 3)
+    # where the tail comes from: admission wait vs time on the device, per
+    # path, and the largest tenth of the requests (by factorization flops)
+    big = np.asarray(flops) >= np.percentile(flops, 90)
+    breakdown = {
+        "admission_wait_ms": {"p50": pct(wait, 50), "p99": pct(wait, 99)},
+        "service_ms": {"p50": pct(service, 50), "p99": pct(service, 99)},
+        "by_path_ms": {k: {"count": len(v), "p50": pct(v, 50), "p99": pct(v, 99)} for k, v in by_path.items()},
+        "largest_tenth_ms": {"count": int(big.sum()), "p50": round(float(np.percentile(lat[big], 50)), 3),
+                             "max": round(float(lat[big].max()), 3)},
+        "rest_ms": {"p50": round(float(np.percentile(lat[~big], 50)), 3),
+                    "p99": round(float(np.percentile(lat[~big], 99)), 3)},
+    }
     return {"requests": count, "rate_per_s": rate, "mode": "mixed+fp64-fallback" if mode == 1 else "fp64",
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_ms": round(float(lat.mean()), 3), "throughput_rps": round(count / (t_end - arrivals[0]), 2),
-            "max_berr": worst, "paths": paths, "rejected": stats["rejected"], "batches": stats["batches"]}
+            "max_berr": worst, "paths": paths, "rejected": stats["rejected"], "batches": stats["batches"],
+            "breakdown": breakdown}
 
 
 def main():

@@ -278,6 +278,11 @@ int rnpx_tuning_load(const char *path);
 /* 1 and the entry's nb and W split when the table has an entry for an
  * m x n factorization of element type dtype ('d' or 'z'), else 0. */
 int rnpx_tuning_lookup_factor(char dtype, size_t m, size_t n, int *nb, int *ksplit);
+/* 1 and the entry's panel_cus (CUs reserved for the look-ahead panel stream;
+ * 0 when the entry gives none) for an m x n factorization of element type
+ * dtype, else 0.  No counterpart in the reference (its Tuning hooks,
+ * RNP/LA/QR.hpp:41-51, have no device to partition). */
+int rnpx_tuning_lookup_panel_cus(char dtype, size_t m, size_t n, int *panel_cus);
 /* 1 and the batched parameters for element type dtype, else 0. */
 int rnpx_tuning_lookup_batched(char dtype, int *lds_bytes, int *blocked_min);
 /* Wait for the handle's stream; reports RNPX_ERR_SYNC if a panel barrier

@@ -142,6 +142,7 @@ SIGNATURES = {
     "rnpx_tuning_load": (_i, [ctypes.c_char_p]),
     "rnpx_tuning_lookup_factor": (_i, [ctypes.c_char, _sz, _sz, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "rnpx_tuning_lookup_batched": (_i, [ctypes.c_char, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "rnpx_tuning_lookup_panel_cus": (_i, [ctypes.c_char, _sz, _sz, ctypes.POINTER(_i)]),
     "rnpx_zgeqrf_batched": (_i, [_p, _i, _p, _p, _p, _p]),
     "rnpx_dgeqrf_batched_plan": (_i, [_p, _i, _p, _p, _p, _p, ctypes.POINTER(_p)]),
     "rnpx_zgeqrf_batched_plan": (_i, [_p, _i, _p, _p, _p, _p, ctypes.POINTER(_p)]),
@@ -582,6 +583,13 @@ def tuning_lookup_factor(dtype, m, n):
     nb, ks = ctypes.c_int(), ctypes.c_int()
     found = lib().rnpx_tuning_lookup_factor(dtype.encode(), m, n, ctypes.byref(nb), ctypes.byref(ks))
     return (nb.value, ks.value) if found == 1 else None
+
+
+def tuning_lookup_panel_cus(dtype, m, n):
+    """panel_cus of the table's entry for an m x n factorization, or None."""
+    v = ctypes.c_int()
+    found = lib().rnpx_tuning_lookup_panel_cus(dtype.encode(), m, n, ctypes.byref(v))
+    return v.value if found == 1 else None
 
 
 def tuning_lookup_batched(dtype):

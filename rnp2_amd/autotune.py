@@ -4,8 +4,9 @@ the path given), the device counterpart of RNP's QR::Tuning hooks
 (RNP/LA/QR.hpp:41-51), which fix nb = 32 for every shape.
 
 Factor: for each shape and element type, every outer block width in
-{32, 64, 128, 256} (the width of the compact-WY block the trailing matrix sees)
-and W = V^H C split factor in the candidates, on RNP Normal01 input resident
+{32, 64, 128, 256} (the width of the compact-WY block the trailing matrix sees),
+W = V^H C split factor and CU count reserved for the look-ahead panel stream
+in the candidates, on RNP Normal01 input resident
 in HBM, best of `reps` timed runs after a warm-up; the fastest is recorded.
 Batched: for the batched configuration's distribution (m log-uniform in
 [16, 256], n uniform in [8, m]), every small-tier LDS budget per one-wave
@@ -27,6 +28,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PATH = os.path.join(HERE, "rnpx_tuning.json")
 WIDTHS = (32, 64, 128, 256)
 KSPLITS = (0, 4, 8, 16)
+PANEL_CUS = (0, 32, 64)  # CUs reserved for the look-ahead panel stream (0: shared)
 BATCH_LDS = (20480, 40960, 81920)
 BATCH_BLOCKED_MIN = (17, 33, 65)
 
@@ -43,9 +45,9 @@ def _time(fn, sync, reps):
     return best
 
 
-def tune_factor(m, n, cplx=False, widths=WIDTHS, ksplits=KSPLITS, reps=3):
-    """Times geqrf of an m x n matrix for every (width, split); returns the
-    fastest entry and all timings."""
+def tune_factor(m, n, cplx=False, widths=WIDTHS, ksplits=KSPLITS, panel_cus=PANEL_CUS, reps=3):
+    """Times geqrf of an m x n matrix for every (width, split, panel CUs);
+    returns the fastest entry and all timings."""
     import torch
     dt = torch.complex128 if cplx else torch.float64
     h = Handle()
@@ -58,20 +60,33 @@ def tune_factor(m, n, cplx=False, widths=WIDTHS, ksplits=KSPLITS, reps=3):
     results = []
     for w in widths:
         for ks in ksplits:
-            h.set_tuning(nb=w, ksplit=ks)
+            for pc in panel_cus:
+                h.set_tuning(nb=w, ksplit=ks, panel_cus=pc if pc else -1)
 
-            def run():
-                with torch.cuda.stream(stream):
-                    a.copy_(a0)
-                h.geqrf(a, tau)
+                def run():
+                    with torch.cuda.stream(stream):
+                        a.copy_(a0)
+                    h.geqrf(a, tau)
 
-            t = _time(run, h.synchronize, reps)
-            results.append({"nb": w, "ksplit": ks, "ms": round(t * 1e3, 3)})
+                t = _time(run, h.synchronize, reps)
+                results.append({"nb": w, "ksplit": ks, "panel_cus": pc, "ms": round(t * 1e3, 3)})
     h.set_tuning()
-    best = min(results, key=lambda r: r["ms"])
+    best = pick_factor(results)
     entry = {"kind": "factor", "dtype": "z" if cplx else "d", "m": m, "n": n, "nb": best["nb"],
-             "ksplit": best["ksplit"], "ms": best["ms"]}
+             "ksplit": best["ksplit"], "panel_cus": best["panel_cus"], "ms": best["ms"]}
     return entry, results
+
+
+def pick_factor(results, margin=0.01):
+    """The fastest candidate, except that a shared-CU one (panel_cus 0)
+    within `margin` of it is preferred: a CU partition has to pay for itself."""
+    best = min(results, key=lambda r: r["ms"])
+    shared = [r for r in results if r.get("panel_cus", 0) == 0]
+    if shared:
+        bs = min(shared, key=lambda r: r["ms"])
+        if bs["ms"] <= best["ms"] * (1.0 + margin):
+            return bs
+    return best
 
 
 def tune_batched(count=10000, cplx=False, lds=BATCH_LDS, blocked_min=BATCH_BLOCKED_MIN, reps=3, seed=0):

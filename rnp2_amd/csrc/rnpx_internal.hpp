@@ -178,7 +178,7 @@ template <typename T> void batch_params(BatchSetup* out);
 // The persisted autotuning table (rnpx_tune.cpp): the entry nearest to an
 // m x n factorization of element type dtype ('d' or 'z'), and the batched
 // tiers' parameters.  False: no entry (defaults apply).
-bool tuning_factor(char dtype, int m, int n, int* nb, int* ksplit);
+bool tuning_factor(char dtype, int m, int n, int* nb, int* ksplit, int* panel_cus = nullptr);
 bool tuning_batched(char dtype, int* lds, int* blocked_min);
 int tuning_load(const char* path);
 

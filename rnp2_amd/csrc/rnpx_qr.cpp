@@ -502,10 +502,11 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
     ~TuneRestore() { h->tune = t; }
   } tune_restore{h, h->tune};
   if (h->tune.nb == 0) {
-    int tnb = 0, tks = 0;
-    if (tuning_factor(S<T>::cplx ? 'z' : 'd', m, n, &tnb, &tks)) {
+    int tnb = 0, tks = 0, tcus = 0;
+    if (tuning_factor(S<T>::cplx ? 'z' : 'd', m, n, &tnb, &tks, &tcus)) {
       h->tune.nb = tnb;
       if (h->tune.ksplit == 0) h->tune.ksplit = tks;
+      if (h->tune.panel_cus == 0) h->tune.panel_cus = tcus;
     }
   }
   hipStream_t st = h->stream;
@@ -625,7 +626,9 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
   std::function<int(int, int, T*, T*, hipStream_t, const GemmWork*)> phase;
   phase = [&](int i, int w, T* Vp, T* Tp, hipStream_t ps, const GemmWork* gw) -> int {
     if constexpr (!S<T>::cplx) {
-      const int Gb = (use_block && w == 2 * nb) ? cqr_block_groups(m - i, std::min(256, h->num_cus)) : 0;
+      // every workgroup of the block kernel must have a CU of the panel set
+      const int bcap = std::min(256, h->panel_cap > 0 ? h->panel_cap : h->num_cus);
+      const int Gb = (use_block && w == 2 * nb) ? cqr_block_groups(m - i, bcap) : 0;
       if (Gb > 0) {
         // Both panels, the apply between them and the merge in one launch.
         const int p1 = i / nb;
@@ -794,7 +797,8 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
         // Not under a panel stream shared with other handles (the server's):
         // there block b+1's kernel can queue behind other requests' work and
         // the gate would hold this stream for nothing.
-        const int Gb = use_block ? cqr_block_groups(m - c0, std::min(256, h->num_cus)) : 0;
+        const int Gb =
+            use_block ? cqr_block_groups(m - c0, std::min(256, h->panel_cap > 0 ? h->panel_cap : h->num_cus)) : 0;
         if (wn >= 2 * nb && Gb > 0 && m - c0 <= gate_rows && c0 + wn < n && !h->ext_pstream)
           RNPX_TRY(launch_cqr_gate((const unsigned*)h->sync.p + 1 + 2 * npan + (size_t)kCqrSyncWords * (c0 / nb), Gb,
                                    (unsigned*)h->flag.p + kGateOffWord, us));

@@ -8,7 +8,8 @@
 //     {"kind": "batched", "dtype": "d", "lds": 40960, "blocked_min": 33, "ms": 8.7}]}
 //
 // Factor entries give the outer block width (nb 32, 64, 128 or 256: the width of
-// the compact-WY block the trailing matrix sees) and the split of W = V^H C;
+// the compact-WY block the trailing matrix sees), the split of W = V^H C and
+// optionally the CUs reserved for the look-ahead panel stream (panel_cus);
 // a factorization whose handle tuning is automatic (nb = 0) takes the entry
 // of its element type nearest in (log2 m, log2 n), within a factor of two in
 // each.  Batched entries give the LDS bytes per one-wave team of the small
@@ -43,6 +44,7 @@ struct Entry {
   char dtype = 'd';
   double m = 0, n = 0;
   int nb = 0, ksplit = 0, lds = 0, blocked_min = 0;
+  int panel_cus = 0;  // factor: CUs of the look-ahead panel stream (0 automatic, -1 none)
 };
 
 std::mutex g_mu;
@@ -141,6 +143,7 @@ struct Reader {
         else if (k == "ksplit") e->ksplit = (int)v;
         else if (k == "lds") e->lds = (int)v;
         else if (k == "blocked_min") e->blocked_min = (int)v;
+        else if (k == "panel_cus") e->panel_cus = (int)v;
       }
     } while (eat(','));
     return eat('}');
@@ -171,7 +174,8 @@ struct Reader {
 bool valid(const Entry& e) {
   if (e.dtype != 'd' && e.dtype != 'z') return false;
   if (e.kind == "factor")
-    return e.m >= 1 && e.n >= 1 && (e.nb == 32 || e.nb == 64 || e.nb == 128 || e.nb == 256) && e.ksplit >= 0 && e.ksplit <= 64;
+    return e.m >= 1 && e.n >= 1 && (e.nb == 32 || e.nb == 64 || e.nb == 128 || e.nb == 256) && e.ksplit >= 0 &&
+           e.ksplit <= 64 && e.panel_cus >= -1 && e.panel_cus <= 1024;
   if (e.kind == "batched") return e.lds >= 4096 && e.lds <= 160 * 1024 && e.blocked_min >= 1;
   return false;
 }
@@ -228,7 +232,7 @@ int tuning_load(const char* path) {
   return r;
 }
 
-bool tuning_factor(char dtype, int m, int n, int* nb, int* ksplit) {
+bool tuning_factor(char dtype, int m, int n, int* nb, int* ksplit, int* panel_cus) {
   std::lock_guard<std::mutex> lk(g_mu);
   ensure_loaded();
   const Entry* best = nullptr;
@@ -245,6 +249,7 @@ bool tuning_factor(char dtype, int m, int n, int* nb, int* ksplit) {
   if (!best) return false;
   *nb = best->nb;
   *ksplit = best->ksplit;
+  if (panel_cus) *panel_cus = best->panel_cus;
   return true;
 }
 
@@ -271,6 +276,13 @@ int rnpx_tuning_lookup_factor(char dtype, size_t m, size_t n, int* nb, int* kspl
   if (!nb || !ksplit) return RNPX_ERR_ARG;
   if (m > (size_t)0x7fffffff || n > (size_t)0x7fffffff) return 0;
   return rnpx::tuning_factor(dtype, (int)m, (int)n, nb, ksplit) ? 1 : 0;
+}
+
+int rnpx_tuning_lookup_panel_cus(char dtype, size_t m, size_t n, int* panel_cus) {
+  if (!panel_cus) return RNPX_ERR_ARG;
+  if (m > (size_t)0x7fffffff || n > (size_t)0x7fffffff) return 0;
+  int nb = 0, ks = 0;
+  return rnpx::tuning_factor(dtype, (int)m, (int)n, &nb, &ks, panel_cus) ? 1 : 0;
 }
 
 int rnpx_tuning_lookup_batched(char dtype, int* lds_bytes, int* blocked_min) {

@@ -28,11 +28,13 @@ EPS = np.finfo(np.float64).eps
 LEAD = 256
 
 
-def _device_factor_checks(m, n, cplx, iseed):
+def _device_factor_checks(m, n, cplx, iseed, panel_cus=0):
     import torch
     dt = torch.complex128 if cplx else torch.float64
-    print(f"[case] headline Factor {m}x{n} {'z' if cplx else 'd'} iseed={iseed}", flush=True)
+    print(f"[case] headline Factor {m}x{n} {'z' if cplx else 'd'} iseed={iseed} panel_cus={panel_cus}", flush=True)
     h = R.Handle()
+    if panel_cus:
+        h.set_tuning(panel_cus=panel_cus)
     a0 = R.colmajor_empty(m, n, dt)
     h.larnv(O.NORMAL01, iseed, a0)
     h.synchronize()
@@ -76,6 +78,16 @@ def test_factor_32768x4096_tall():
 
 def test_factor_4096_complex():
     _device_factor_checks(4096, 4096, True, [1, 2, 3, 5])
+
+
+@pytest.mark.parametrize("cus", [32, 64])
+@pytest.mark.parametrize("m,n,cplx", [(4096, 4096, False), (2048, 2048, True)])
+def test_factor_cu_partition(m, n, cplx, cus):
+    """Look-ahead with the panel stream on a reserved set of CUs: every
+    panel and block kernel of the call (their workgroups meet at grid-wide
+    reductions) is sized to that set, so 4096 rows (64 block-kernel
+    workgroups) take the two-panel path while the set holds fewer."""
+    _device_factor_checks(m, n, cplx, [2, 3, 5, 7], panel_cus=cus)
 
 
 def _q_of(torch, h, rows, cols, sd):

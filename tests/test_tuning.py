@@ -11,7 +11,7 @@ import rnp2_amd as R
 TABLE = {"version": 1, "device": "test", "entries": [
     {"kind": "factor", "dtype": "d", "m": 8192, "n": 8192, "nb": 128, "ksplit": 8, "ms": 28.0},
     {"kind": "factor", "dtype": "d", "m": 32768, "n": 4096, "nb": 64, "ksplit": 0, "ms": 37.0},
-    {"kind": "factor", "dtype": "z", "m": 4096, "n": 4096, "nb": 32, "ksplit": 4, "ms": 70.0},
+    {"kind": "factor", "dtype": "z", "m": 4096, "n": 4096, "nb": 32, "ksplit": 4, "panel_cus": 32, "ms": 70.0},
     {"kind": "batched", "dtype": "d", "lds": 81920, "blocked_min": 17, "ms": 8.0}],
     "candidates": {"factor d 8192x8192": [{"nb": 32, "ksplit": 0, "ms": 90.0}]}}
 
@@ -38,12 +38,17 @@ def test_load_and_nearest_lookup(tmp_path):
     assert R.tuning_lookup_factor("z", 8192, 8192) == (32, 4)           # 2x in each: still in range
     assert R.tuning_lookup_batched("d") == (81920, 17)
     assert R.tuning_lookup_batched("z") is None
+    # the optional panel stream CU count: given for the complex entry only
+    assert R.tuning_lookup_panel_cus("z", 4096, 4096) == 32
+    assert R.tuning_lookup_panel_cus("d", 8192, 8192) == 0
+    assert R.tuning_lookup_panel_cus("d", 2000, 2000) is None
 
 
 @pytest.mark.parametrize("bad", [
     "{", '{"entries": [{"kind": "factor", "dtype": "d", "m": 8, "n": 8, "nb": 48, "ksplit": 0}]}',   # width not built
     '{"entries": [{"kind": "factor", "dtype": "q", "m": 8, "n": 8, "nb": 32, "ksplit": 0}]}',        # element type
     '{"entries": [{"kind": "batched", "dtype": "d", "lds": 10, "blocked_min": 3}]}',                 # LDS out of range
+    '{"entries": [{"kind": "factor", "dtype": "d", "m": 8, "n": 8, "nb": 32, "ksplit": 0, "panel_cus": -2}]}',  # CUs
     '{"entries": [{"kind": "factor" "dtype": "d"}]}',                                                # syntax
     '{"entries": []} trailing'])
 def test_malformed_tables_are_refused_and_keep_the_old_one(tmp_path, bad):
