@@ -386,11 +386,12 @@ def leg_serving(R, torch, count=120, rate=40.0, seed=5, mode=1):
         arrivals.append(t_next)
         ids.append(srv.submit(a, b)[1])
     lat, worst, paths = [], 0.0, {}
-    wait, service, by_path, flops = [], [], {}, []
+    lag, wait, service, by_path, flops = [], [], [], {}, []
     t_end = 0.0
     for (a, b, _), arr, rid in zip(reqs, arrivals, ids):
         x, info = srv.poll(rid)
         lat.append(info.t_done - arr)
+        lag.append(info.t_submit - arr)
         wait.append(info.t_admit - info.t_submit)
         service.append(info.t_done - info.t_admit)
         by_path.setdefault(R.PATHS[info.path], []).append(info.t_done - arr)
@@ -410,6 +411,10 @@ def leg_serving(R, torch, count=120, rate=40.0, seed=5, mode=1):
     # path, and the largest tenth of the requests (by factorization flops)
     big = np.asarray(flops) >= np.percentile(flops, 90)
     breakdown = {
+        # scheduled arrival -> submit called (submit copies A and b into
+        # the request, so a large request delays the client loop's next
+        # submissions); the copy itself counts in the admission wait
+        "submit_lag_ms": {"p50": pct(lag, 50), "p99": pct(lag, 99)},
         "admission_wait_ms": {"p50": pct(wait, 50), "p99": pct(wait, 99)},
         "service_ms": {"p50": pct(service, 50), "p99": pct(service, 99)},
         "by_path_ms": {k: {"count": len(v), "p50": pct(v, 50), "p99": pct(v, 99)} for k, v in by_path.items()},

@@ -416,7 +416,9 @@ typedef struct rnpx_request_info {
 	int iters;    /* refinement corrections (mixed) */
 	int worker;   /* worker that ran it */
 	double berr;  /* eta of the solution (dense paths) */
-	double t_submit, t_admit, t_done;  /* steady-clock seconds */
+	/* steady-clock seconds: submit called (before its copy of A and b),
+	 * admitted to a worker, result ready */
+	double t_submit, t_admit, t_done;
 } rnpx_request_info;
 typedef struct rnpx_server_stats {
 	long long submitted, admitted, completed, rejected, failed, batches, queued;

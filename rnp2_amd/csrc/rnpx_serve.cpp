@@ -308,6 +308,7 @@ int rnpx_server_submit(rnpx_server srv, size_t m, size_t n, size_t nrhs, const d
   if (!b) return -7;
   if (ldb < m) return -8;
   if (!id) return -9;
+  const double t_in = now_s();  // the request's own copy of A and b below counts as its wait
   Request* r = new Request();
   r->m = (int)m;
   r->n = (int)n;
@@ -317,7 +318,7 @@ int rnpx_server_submit(rnpx_server srv, size_t m, size_t n, size_t nrhs, const d
   for (size_t j = 0; j < n; ++j) std::memcpy(&r->A[j * m], a + j * lda, sizeof(double) * m);
   for (size_t j = 0; j < nrhs; ++j) std::memcpy(&r->B[j * m], b + j * ldb, sizeof(double) * m);
   r->bytes = footprint(r->m, r->n, r->nrhs, s->cfg.mode);
-  r->info.t_submit = now_s();
+  r->info.t_submit = t_in;
   std::lock_guard<std::mutex> lk(s->mu);
   r->id = s->next_id++;
   *id = r->id;
