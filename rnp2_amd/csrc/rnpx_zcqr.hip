@@ -24,7 +24,7 @@
 // Rejection: a first-pass pivot below 2^-40 of its diagonal, a squared
 // column norm outside [2^-600, 2^600], anything not finite, or a second-pass
 // Gram further than 2^-27 from I sets the fail flag; then nothing is written
-// to A by this path (the commit kernel is gated on the flag) and the
+// to A by this path (the write-back kernel is gated on the flag) and the
 // column-by-column panel kernel, gated on the same flag, factors the panel.
 #include "rnpx_internal.hpp"
 
@@ -34,7 +34,6 @@ namespace rnpx {
 
 namespace zq {
 
-constexpr int kThreads = 256;  // the commit kernel
 // The small factorizations: one 256-thread workgroup; thread t holds rows
 // (t & 7) + 8 r (r < 4) of column t >> 3 of the 32 x 32 matrix being worked
 // on in registers, so the eight lanes of a column sit in one wave and share
@@ -194,7 +193,7 @@ __global__ __launch_bounds__(kSmall) void chol_kernel(const zd* __restrict__ G, 
 
 // Pass 2 and the reconstruction.  In: G2 = Q1^H Q1, R1, the top NB rows of
 // Q1.  Out: tau, T (ldt), the clean V's top NB rows (ldv), Atop = R (upper)
-// and L1 (strictly lower) for the commit, M = R2^{-1} U^{-1} for the rest of
+// and L1 (strictly lower) for the write-back, M = R2^{-1} U^{-1} for the rest of
 // V; or the fail flag.
 __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2, const zd* __restrict__ R1,
                                                        const zd* __restrict__ Q1, long ldq, zd* __restrict__ tau,
@@ -354,16 +353,38 @@ __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2
   }
 }
 
-// A's panel from Atop (top NB rows) and the clean V (rows NB..mp), unless
-// the panel was rejected.
-__global__ __launch_bounds__(kThreads) void commit_kernel(int mp, int nb, const zd* __restrict__ Atop,
-                                                          const zd* __restrict__ V, long ldv, zd* __restrict__ A,
-                                                          long lda, const unsigned* fail) {
-  if (*fail) return;
-  const long total = (long)mp * nb;
-  for (long e = (long)blockIdx.x * kThreads + threadIdx.x; e < total; e += (long)gridDim.x * kThreads) {
-    const int r = (int)(e % mp), c = (int)(e / mp);
-    A[(size_t)r + (size_t)c * lda] = r < nb ? Atop[(size_t)r + (size_t)c * nb] : V[(size_t)r + (size_t)c * ldv];
+// The rest of V and the write-back in one pass: rows [nb, mp) of V are
+// Q1 M (M = R2^{-1} U^{-1}); unless the panel was rejected, A's rows [0, nb)
+// become Atop (R and L1) and its rows below become V.  A workgroup takes 64
+// rows: their Q1 rows and M staged in LDS, thread t forms columns
+// 8 (t / 64) .. + 7 of row t mod 64.
+__global__ __launch_bounds__(256) void vcommit_kernel(int mp, const zd* __restrict__ Q1, const zd* __restrict__ Mm,
+                                                      const zd* __restrict__ Atop, zd* __restrict__ V, long ldv,
+                                                      zd* __restrict__ A, long lda, const unsigned* fail) {
+  __shared__ zd Ms[NB][NB + 1];
+  __shared__ zd Qs[64][NB + 1];
+  const int t = threadIdx.x, r0 = blockIdx.x * 64;
+  const bool ok = (*fail == 0u);  // uniform
+  for (int e = t; e < NB * NB; e += 256) Ms[e % NB][e / NB] = Mm[e];
+  for (int e = t; e < 64 * NB; e += 256) {
+    const int rr = e % 64, k = e / 64, r = min(r0 + rr, mp - 1);
+    Qs[rr][k] = Q1[(size_t)r + (size_t)k * mp];
+  }
+  __syncthreads();
+  const int rl = t & 63, cg = t >> 6, r = r0 + rl;
+  if (r >= mp) return;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int c = 8 * cg + q;
+    if (r < NB) {
+      if (ok) A[(size_t)r + (size_t)c * lda] = Atop[(size_t)r + (size_t)c * NB];
+      continue;
+    }
+    zd v = S<zd>::zero();
+#pragma unroll 8
+    for (int k = 0; k < NB; ++k) v = v + Qs[rl][k] * Ms[k][c];
+    V[(size_t)r + (size_t)c * ldv] = v;
+    if (ok) A[(size_t)r + (size_t)c * lda] = v;
   }
 }
 
@@ -398,10 +419,10 @@ int zcqr_panel(Handle* h, int mp, int nb, zd* A, long lda, zd* tau, zd* V, long 
   hipLaunchKernelGGL(zq::recon_kernel, dim3(1), dim3(zq::kSmall), 0, st, G, R1, Q1, (long)mp, tau, Tm, ldt,
                      V, ldv, Atop, M, fail);
   RNPX_TRY(hipGetLastError());
-  // the rest of V: Q1_bot R2^{-1} U^{-1}
-  RNPX_TRY(gemm<zd>('N', 'N', mp - nb, nb, nb, one, Q1 + nb, mp, M, nb, zero, V + nb, ldv, 1, nullptr, st));
-  const int cgrid = std::min(1024, (int)(((long)mp * nb + zq::kThreads - 1) / zq::kThreads));
-  hipLaunchKernelGGL(zq::commit_kernel, dim3(cgrid), dim3(zq::kThreads), 0, st, mp, nb, Atop, V, ldv, A, lda, fail);
+  // the rest of V (Q1_bot R2^{-1} U^{-1}) and, unless rejected, the panel
+  // written back
+  hipLaunchKernelGGL(zq::vcommit_kernel, dim3((unsigned)((mp + 63) / 64)), dim3(256), 0, st, mp, Q1, M, Atop, V, ldv,
+                     A, lda, fail);
   RNPX_TRY(hipGetLastError());
   // a rejected panel: column by column (both launches return at once otherwise)
   PanelArgs<zd> fb = pa;
