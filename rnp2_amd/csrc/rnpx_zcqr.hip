@@ -353,6 +353,68 @@ __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2
   }
 }
 
+// Q1 = A R1^{-1} and the second pass's Gram partials in one pass: a
+// workgroup takes 64 rows of the panel (A's rows and R1^{-1} staged in LDS),
+// thread t forms columns 8 (t / 64) .. + 7 of row t mod 64 of Q1, then the
+// workgroup's partial Q1_rows^H Q1_rows (four entries a thread) goes to
+// part + blockIdx 1024.
+__global__ __launch_bounds__(256) void q1gram_kernel(int mp, const zd* __restrict__ A, long lda,
+                                                     const zd* __restrict__ R1i, zd* __restrict__ Q1,
+                                                     zd* __restrict__ part, const unsigned* fail) {
+  __shared__ zd Rs[NB][NB + 1];
+  __shared__ zd Xs[64][NB + 1];
+  __shared__ zd Ys[64][NB + 1];
+  if (*fail) return;  // pass 1 rejected the panel (uniform)
+  const int t = threadIdx.x, r0 = blockIdx.x * 64;
+  for (int e = t; e < NB * NB; e += 256) Rs[e % NB][e / NB] = R1i[e];
+  for (int e = t; e < 64 * NB; e += 256) {
+    const int rr = e % 64, k = e / 64, r = min(r0 + rr, mp - 1);
+    Xs[rr][k] = A[(size_t)r + (size_t)k * lda];
+  }
+  __syncthreads();
+  const int rl = t & 63, cg = t >> 6, r = r0 + rl;
+  // the row in registers; R1^{-1}(k, c) is the same address across the wave
+  zd x[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) x[k] = Xs[rl][k];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int c = 8 * cg + q;
+    zd y = S<zd>::zero();
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+      if (k <= c) y = y + x[k] * Rs[k][c];  // R1^{-1} upper triangular
+    if (r < mp) Q1[(size_t)r + (size_t)c * mp] = y;
+    Ys[rl][c] = (r < mp) ? y : S<zd>::zero();
+  }
+  __syncthreads();
+  // partial Gram: thread t a 2 x 2 block, rows i0, i0+1 and columns j0, j0+1
+  const int i0 = 2 * (t & 15), j0 = 2 * (t >> 4);
+  zd g00 = S<zd>::zero(), g10 = S<zd>::zero(), g01 = S<zd>::zero(), g11 = S<zd>::zero();
+#pragma unroll 4
+  for (int rr = 0; rr < 64; ++rr) {
+    const zd a0 = zconj(Ys[rr][i0]), a1 = zconj(Ys[rr][i0 + 1]), b0 = Ys[rr][j0], b1 = Ys[rr][j0 + 1];
+    g00 = g00 + a0 * b0;
+    g10 = g10 + a1 * b0;
+    g01 = g01 + a0 * b1;
+    g11 = g11 + a1 * b1;
+  }
+  zd* pp = part + (size_t)blockIdx.x * NB * NB;
+  pp[i0 + j0 * NB] = g00;
+  pp[i0 + 1 + j0 * NB] = g10;
+  pp[i0 + (j0 + 1) * NB] = g01;
+  pp[i0 + 1 + (j0 + 1) * NB] = g11;
+}
+
+// G = the sum of ns partial 32 x 32 Grams, in order: 16 workgroups of 64
+// threads, one entry a thread.
+__global__ __launch_bounds__(64) void gram_sum_kernel(int ns, const zd* __restrict__ part, zd* __restrict__ G,
+                                                      const unsigned* fail) {
+  if (*fail) return;
+  const int e = blockIdx.x * 64 + threadIdx.x;  // < NB * NB
+  G[e] = sum_strided(part + e, ns, (size_t)NB * NB);
+}
+
 // The rest of V and the write-back in one pass: rows [nb, mp) of V are
 // Q1 M (M = R2^{-1} U^{-1}); unless the panel was rejected, A's rows [0, nb)
 // become Atop (R and L1) and its rows below become V.  A workgroup takes 64
@@ -373,16 +435,24 @@ __global__ __launch_bounds__(256) void vcommit_kernel(int mp, const zd* __restri
   __syncthreads();
   const int rl = t & 63, cg = t >> 6, r = r0 + rl;
   if (r >= mp) return;
+  if (r < NB) {
+    if (ok) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        A[(size_t)r + (size_t)(8 * cg + q) * lda] = Atop[(size_t)r + (size_t)(8 * cg + q) * NB];
+    }
+    return;
+  }
+  // the row in registers; M(k, c) is the same address across the wave
+  zd x[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) x[k] = Qs[rl][k];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int c = 8 * cg + q;
-    if (r < NB) {
-      if (ok) A[(size_t)r + (size_t)c * lda] = Atop[(size_t)r + (size_t)c * NB];
-      continue;
-    }
     zd v = S<zd>::zero();
-#pragma unroll 8
-    for (int k = 0; k < NB; ++k) v = v + Qs[rl][k] * Ms[k][c];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) v = v + x[k] * Ms[k][c];
     V[(size_t)r + (size_t)c * ldv] = v;
     if (ok) A[(size_t)r + (size_t)c * lda] = v;
   }
@@ -400,22 +470,28 @@ int zcqr_panel(Handle* h, int mp, int nb, zd* A, long lda, zd* tau, zd* V, long 
   if (nb != 32 || mp < 2 * nb) return RNPX_ERR_UNSUPPORTED;
   const size_t nn = (size_t)nb * nb;
   const int ks = std::max(1, std::min(64, mp / 64));  // 64-row slices: short chains per workgroup
-  RNPX_TRY(h->zcqr.ensure(sizeof(zd) * ((size_t)mp * nb + (6 + (size_t)ks) * nn), h));
+  const size_t nrb_max = (size_t)(mp + 63) / 64;
+  RNPX_TRY(h->zcqr.ensure(sizeof(zd) * ((size_t)mp * nb + (6 + (size_t)ks + nrb_max) * nn), h));
   zd* Q1 = (zd*)h->zcqr.p;
   zd* G = Q1 + (size_t)mp * nb;
   zd* R1 = G + nn;
   zd* R1i = R1 + nn;
   zd* Atop = R1i + nn;
   zd* M = Atop + nn;
-  zd* part = M + nn;  // split-K partials of the Gram products
+  zd* part = M + nn;              // split-K partials of the first Gram
+  zd* part2 = part + (size_t)ks * nn;  // per-64-row partials of the second
   const zd one = S<zd>::one(), zero = S<zd>::zero();
   // pass 1
   RNPX_TRY(gemm<zd>('C', 'N', nb, nb, mp, one, A, lda, A, lda, zero, G, nb, ks, part, st));
   hipLaunchKernelGGL(zq::chol_kernel, dim3(1), dim3(zq::kSmall), 0, st, G, R1, R1i, fail);
   RNPX_TRY(hipGetLastError());
-  RNPX_TRY(gemm<zd>('N', 'N', mp, nb, nb, one, A, lda, R1i, nb, zero, Q1, mp, 1, nullptr, st));
+  // Q1 = A R1^{-1} and the pass-2 Gram G2 = Q1^H Q1 (partials per 64 rows)
+  const int nrb = (mp + 63) / 64;
+  hipLaunchKernelGGL(zq::q1gram_kernel, dim3((unsigned)nrb), dim3(256), 0, st, mp, A, lda, R1i, Q1, part2, fail);
+  RNPX_TRY(hipGetLastError());
+  hipLaunchKernelGGL(zq::gram_sum_kernel, dim3(zq::NB * zq::NB / 64), dim3(64), 0, st, nrb, part2, G, fail);
+  RNPX_TRY(hipGetLastError());
   // pass 2 and the reconstruction
-  RNPX_TRY(gemm<zd>('C', 'N', nb, nb, mp, one, Q1, mp, Q1, mp, zero, G, nb, ks, part, st));
   hipLaunchKernelGGL(zq::recon_kernel, dim3(1), dim3(zq::kSmall), 0, st, G, R1, Q1, (long)mp, tau, Tm, ldt,
                      V, ldv, Atop, M, fail);
   RNPX_TRY(hipGetLastError());
