@@ -141,13 +141,33 @@ __device__ void team_factor(int m, int n, T* W, int ld, T* tau, double* sh) {
         }
       }
     } else {
+      // Four independent partial sums (rows r mod 4) and unrolled loops, so
+      // that the LDS loads of four rows are in flight at once: with one wave
+      // per SIMD nothing else hides their latency.
+      const T* v = W + c * ld;
       for (int j = c + 1 + t; j < n; j += NT) {
         T* col = W + j * ld;
-        T w = col[c];
-        for (int r = c + 1; r < m; ++r) w = w + S<T>::cmul(W[r + c * ld], col[r]);
-        const T cw = ctau * w;
+        T w0 = col[c], w1 = S<T>::zero(), w2 = S<T>::zero(), w3 = S<T>::zero();
+        int r = c + 1;
+        for (; r + 4 <= m; r += 4) {
+          w0 = w0 + S<T>::cmul(v[r], col[r]);
+          w1 = w1 + S<T>::cmul(v[r + 1], col[r + 1]);
+          w2 = w2 + S<T>::cmul(v[r + 2], col[r + 2]);
+          w3 = w3 + S<T>::cmul(v[r + 3], col[r + 3]);
+        }
+        for (; r < m; ++r) w0 = w0 + S<T>::cmul(v[r], col[r]);
+        const T cw = ctau * ((w0 + w1) + (w2 + w3));
         col[c] = col[c] - cw;
-        for (int r = c + 1; r < m; ++r) col[r] = col[r] - W[r + c * ld] * cw;
+        r = c + 1;
+        for (; r + 4 <= m; r += 4) {
+          const T c0 = col[r] - v[r] * cw, c1 = col[r + 1] - v[r + 1] * cw;
+          const T c2 = col[r + 2] - v[r + 2] * cw, c3 = col[r + 3] - v[r + 3] * cw;
+          col[r] = c0;
+          col[r + 1] = c1;
+          col[r + 2] = c2;
+          col[r + 3] = c3;
+        }
+        for (; r < m; ++r) col[r] = col[r] - v[r] * cw;
       }
     }
     if (t == 0) {
