@@ -96,22 +96,25 @@ __global__ __launch_bounds__(64) void batched_small_kernel(int count, const int*
   }
 }
 
-// Tier L: one 512-thread workgroup per problem.  Problems of more than one
+// Tier L: one 256-thread workgroup per problem, two workgroups per CU (80 KB
+// of LDS each): the per-column reductions of a panel leave most of a CU idle,
+// so two problems share one.  Problems of more than one
 // panel's columns take the blocked path below (panels in LDS, block
 // reflector updates on MFMA); narrower ones that fit are staged in LDS whole
 // (column-major, ld = m: the waves read down columns); the rest (taller than
 // the blocked path's panel buffer) are factored in place in global memory.
 // The Householder application gives each wave one column at a time with the
 // lanes splitting the rows (team_factor's WAVECOL mode).
-constexpr int kLargeLds = 20352;  // doubles of LDS per workgroup (159 KiB)
-constexpr int kLargeThreads = 512;
-constexpr int kBlockedMinCols = 33;  // more than one panel: blocked (measured faster than whole-LDS)
+constexpr int kLargeLds = 10176;  // doubles of LDS per workgroup (79.5 KiB: two workgroups per CU)
+constexpr int kLargeThreads = 256;
+constexpr int kLargePerCU = 2;
+constexpr int kBlockedMinCols = 17;  // more than one panel: blocked (measured faster than whole-LDS)
 
 // Panel width and the most panel rows the LDS carve-up holds, per type.
 template <typename T> struct Blk;
-template <> struct Blk<double> { static constexpr int PW = 32, ROWS = 512; };
+template <> struct Blk<double> { static constexpr int PW = 16, ROWS = 512; };
 template <> struct Blk<zd> { static constexpr int PW = 16, ROWS = 256; };
-static_assert(Blk<double>::ROWS * Blk<double>::PW + 2 * 32 * 33 <= kLargeLds, "LDS carve-up (real)");
+static_assert(Blk<double>::ROWS * Blk<double>::PW + 2 * 16 * 17 <= kLargeLds, "LDS carve-up (real)");
 static_assert(2 * (Blk<zd>::ROWS * Blk<zd>::PW + 2 * 16 * 17) <= kLargeLds, "LDS carve-up (complex)");
 
 // acc += a * b on FP64 MFMA for real or complex (4 real products) operands.
@@ -299,6 +302,8 @@ __global__ __launch_bounds__(kLargeThreads) void batched_large_kernel(int count,
     __syncthreads();
   }
 }
+
+int batched_large_teams(int count, int num_cus) { return std::max(1, std::min(count, kLargePerCU * num_cus)); }
 
 template <typename T>
 hipError_t launch_batched(int tier, int count, const int* order, const rnpx_qr_desc* desc, T* A, T* tau, T* B,
