@@ -96,26 +96,27 @@ __global__ __launch_bounds__(64) void batched_small_kernel(int count, const int*
   }
 }
 
-// Tier L: one 256-thread workgroup per problem, two workgroups per CU (80 KB
-// of LDS each): the per-column reductions of a panel leave most of a CU idle,
-// so two problems share one.  Problems of more than one
+// Tier L: one workgroup per problem, several per CU (real: four of 128
+// threads, complex: two of 256): the per-column reductions of a panel leave
+// most of a CU idle, so problems share one.  Problems of more than one
 // panel's columns take the blocked path below (panels in LDS, block
 // reflector updates on MFMA); narrower ones that fit are staged in LDS whole
 // (column-major, ld = m: the waves read down columns); the rest (taller than
 // the blocked path's panel buffer) are factored in place in global memory.
 // The Householder application gives each wave one column at a time with the
 // lanes splitting the rows (team_factor's WAVECOL mode).
-constexpr int kLargeLds = 10176;  // doubles of LDS per workgroup (79.5 KiB: two workgroups per CU)
-constexpr int kLargeThreads = 256;
-constexpr int kLargePerCU = 2;
+// Per element type: threads per team, teams per CU and LDS doubles per team
+// (real: four 128-thread teams of 39.5 KiB; complex: two 256-thread teams of
+// 79.5 KiB), the panel width and the most panel rows the LDS carve-up holds.
+template <typename T> struct Large;
+template <> struct Large<double> { static constexpr int NT = 128, PER_CU = 4, LDS = 5056, PW = 16, ROWS = 256; };
+template <> struct Large<zd> { static constexpr int NT = 256, PER_CU = 2, LDS = 10176, PW = 16, ROWS = 256; };
 constexpr int kBlockedMinCols = 17;  // more than one panel: blocked (measured faster than whole-LDS)
-
-// Panel width and the most panel rows the LDS carve-up holds, per type.
-template <typename T> struct Blk;
-template <> struct Blk<double> { static constexpr int PW = 16, ROWS = 512; };
-template <> struct Blk<zd> { static constexpr int PW = 16, ROWS = 256; };
-static_assert(Blk<double>::ROWS * Blk<double>::PW + 2 * 16 * 17 <= kLargeLds, "LDS carve-up (real)");
-static_assert(2 * (Blk<zd>::ROWS * Blk<zd>::PW + 2 * 16 * 17) <= kLargeLds, "LDS carve-up (complex)");
+template <typename T> using Blk = Large<T>;
+static_assert(Large<double>::ROWS * Large<double>::PW + 2 * 16 * 17 <= Large<double>::LDS, "LDS carve-up (real)");
+static_assert(2 * (Large<zd>::ROWS * Large<zd>::PW + 2 * 16 * 17) <= Large<zd>::LDS, "LDS carve-up (complex)");
+static_assert(Large<double>::PER_CU * (Large<double>::LDS * 8 + 256) <= 160 * 1024, "teams per CU (real)");
+static_assert(Large<zd>::PER_CU * (Large<zd>::LDS * 8 + 256) <= 160 * 1024, "teams per CU (complex)");
 
 // acc += a * b on FP64 MFMA for real or complex (4 real products) operands.
 struct CAcc {
@@ -157,7 +158,7 @@ template <> struct AccOf<zd> {
 // C^T -= Z^T V^T so that the C tile's loads and stores run down columns.
 template <typename T>
 __device__ void blocked_factor(int m, int n, T* a, long lda, T* tau, double* lds, double* sh) {
-  constexpr int NT = kLargeThreads, NW = NT / 64, PW = Blk<T>::PW, NB2 = PW / 16, TS = PW + 1;
+  constexpr int NT = Large<T>::NT, NW = NT / 64, PW = Blk<T>::PW, NB2 = PW / 16, TS = PW + 1;
   using AO = AccOf<T>;
   using Acc = typename AO::type;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -263,14 +264,15 @@ __device__ void blocked_factor(int m, int n, T* a, long lda, T* tau, double* lds
 }
 
 template <typename T>
-__global__ __launch_bounds__(kLargeThreads) void batched_large_kernel(int count, const int* __restrict__ order,
+__global__ __launch_bounds__(Large<T>::NT) void batched_large_kernel(int count, const int* __restrict__ order,
                                                             const rnpx_qr_desc* __restrict__ desc, T* A, T* tau,
                                                             T* B, unsigned* counter, int bmin) {
-  __shared__ double lds[kLargeLds];
+  constexpr int NT = Large<T>::NT;
+  __shared__ double lds[Large<T>::LDS];
   __shared__ double sh[8];
   __shared__ T xs;
   __shared__ int next;
-  constexpr int cap = kLargeLds / (int)(sizeof(T) / sizeof(double));
+  constexpr int cap = Large<T>::LDS / (int)(sizeof(T) / sizeof(double));
   T* Ws = reinterpret_cast<T*>(lds);
   const int t = threadIdx.x;
   while (true) {
@@ -284,26 +286,31 @@ __global__ __launch_bounds__(kLargeThreads) void batched_large_kernel(int count,
     T* a = A + d.a_off;
     const bool solve = d.nrhs > 0 && B && m >= n;
     if (m * n <= cap && (n < bmin || m > Blk<T>::ROWS)) {
-      for (int e = t; e < m * n; e += kLargeThreads) Ws[e] = a[(size_t)(e % m) + (size_t)(e / m) * d.lda];
+      for (int e = t; e < m * n; e += NT) Ws[e] = a[(size_t)(e % m) + (size_t)(e / m) * d.lda];
       __syncthreads();
-      team_factor<T, kLargeThreads, true>(m, n, Ws, m, tau + d.tau_off, sh);
+      team_factor<T, NT, true>(m, n, Ws, m, tau + d.tau_off, sh);
       __syncthreads();
-      for (int e = t; e < m * n; e += kLargeThreads) a[(size_t)(e % m) + (size_t)(e / m) * d.lda] = Ws[e];
-      if (solve) team_solve<T, kLargeThreads>(m, n, Ws, m, tau + d.tau_off, d.nrhs, B + d.b_off, d.ldb, sh, &xs);
+      for (int e = t; e < m * n; e += NT) a[(size_t)(e % m) + (size_t)(e / m) * d.lda] = Ws[e];
+      if (solve) team_solve<T, NT>(m, n, Ws, m, tau + d.tau_off, d.nrhs, B + d.b_off, d.ldb, sh, &xs);
     } else {
       if (m <= Blk<T>::ROWS)
         blocked_factor<T>(m, n, a, d.lda, tau + d.tau_off, lds, sh);
       else
-        team_factor<T, kLargeThreads, true>(m, n, a, d.lda, tau + d.tau_off, sh);
+        team_factor<T, NT, true>(m, n, a, d.lda, tau + d.tau_off, sh);
       __threadfence_block();
       __syncthreads();
-      if (solve) team_solve<T, kLargeThreads>(m, n, a, d.lda, tau + d.tau_off, d.nrhs, B + d.b_off, d.ldb, sh, &xs);
+      if (solve) team_solve<T, NT>(m, n, a, d.lda, tau + d.tau_off, d.nrhs, B + d.b_off, d.ldb, sh, &xs);
     }
     __syncthreads();
   }
 }
 
-int batched_large_teams(int count, int num_cus) { return std::max(1, std::min(count, kLargePerCU * num_cus)); }
+template <typename T>
+int batched_large_teams(int count, int num_cus) {
+  return std::max(1, std::min(count, Large<T>::PER_CU * num_cus));
+}
+template int batched_large_teams<double>(int, int);
+template int batched_large_teams<zd>(int, int);
 
 template <typename T>
 hipError_t launch_batched(int tier, int count, const int* order, const rnpx_qr_desc* desc, T* A, T* tau, T* B,
@@ -325,7 +332,7 @@ hipError_t launch_batched(int tier, int count, const int* order, const rnpx_qr_d
       return e ? atoi(e) : 0;
     }();
     const int bmin = env_bmin > 0 ? env_bmin : (param > 0 ? param : kBlockedMinCols);
-    hipLaunchKernelGGL((batched_large_kernel<T>), dim3(teams), dim3(kLargeThreads), 0, st, count, order, desc, A, tau, B,
+    hipLaunchKernelGGL((batched_large_kernel<T>), dim3(teams), dim3(Large<T>::NT), 0, st, count, order, desc, A, tau, B,
                        counter, bmin);
   }
   return hipGetLastError();

@@ -386,7 +386,7 @@ hipError_t launch_axpy_mat(int m, int n, const double* X, long ldx, double* Y, l
 // the column count from which real problems take the blocked path, 0 = the
 // default).
 // Workgroups of the large tier for count problems (a few per CU).
-int batched_large_teams(int count, int num_cus);
+template <typename T> int batched_large_teams(int count, int num_cus);
 template <typename T>
 hipError_t launch_batched(int tier, int count, const int* order, const rnpx_qr_desc* desc, T* A, T* tau, T* B,
                           unsigned* counter, int teams, int param, hipStream_t st);

@@ -335,7 +335,7 @@ static int batch_launch(Handle* h, const BatchSetup& bs, BatchBuffers* bb, T* A,
   if (nl > 0) {
     RNPX_TRY(hipEventRecord(h->ev_start, h->stream));
     RNPX_TRY(hipStreamWaitEvent(h->pstream, h->ev_start, 0));
-    RNPX_TRY(launch_batched<T>(1, nl, order, d, A, tau, B, ctr, batched_large_teams(nl, h->num_cus), bs.blocked_min, h->pstream));
+    RNPX_TRY(launch_batched<T>(1, nl, order, d, A, tau, B, ctr, batched_large_teams<T>(nl, h->num_cus), bs.blocked_min, h->pstream));
   }
   if (ns > 0) {
     const int per_cu = std::max(1, (160 * 1024) / bs.lds_bytes);

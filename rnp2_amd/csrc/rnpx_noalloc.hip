@@ -97,7 +97,7 @@ static int geqrf_batched_noalloc(Handle* h, int count, const rnpx_qr_desc* desc,
   RNPX_TRY_HIP(hipStreamSynchronize(st));
   const int nl = (int)bs.large.size(), ns = (int)bs.small.size();
   if (nl > 0)
-    RNPX_TRY_HIP(launch_batched<T>(1, nl, d_order, d_desc, A, tau, B, d_ctr, batched_large_teams(nl, h->num_cus), bs.blocked_min,
+    RNPX_TRY_HIP(launch_batched<T>(1, nl, d_order, d_desc, A, tau, B, d_ctr, batched_large_teams<T>(nl, h->num_cus), bs.blocked_min,
                                    st));
   if (ns > 0) {
     const int per_cu = std::max(1, (160 * 1024) / bs.lds_bytes);
