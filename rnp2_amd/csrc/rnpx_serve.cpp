@@ -37,6 +37,27 @@
 
 namespace rnpx {
 
+// dst (m x n, leading dimension m) = src (leading dimension lds): large
+// matrices split over host threads by columns (one thread copies a few GB/s;
+// a 8192 x 4096 request is 268 MB).
+static void copy_columns(double* dst, const double* src, size_t m, size_t n, size_t lds) {
+  const size_t bytes = sizeof(double) * m * n;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>({(size_t)8, (size_t)hw, n, std::max<size_t>(1, bytes >> 24)});  // 16 MB each
+  auto part = [&](size_t j0, size_t j1) {
+    for (size_t j = j0; j < j1; ++j) std::memcpy(dst + j * m, src + j * lds, sizeof(double) * m);
+  };
+  if (nt <= 1) {
+    part(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (n + nt - 1) / nt;
+  for (size_t q = 1; q < nt; ++q) th.emplace_back(part, std::min(n, q * per), std::min(n, (q + 1) * per));
+  part(0, std::min(n, per));
+  for (auto& x : th) x.join();
+}
+
 static double now_s() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -315,8 +336,8 @@ int rnpx_server_submit(rnpx_server srv, size_t m, size_t n, size_t nrhs, const d
   r->nrhs = (int)nrhs;
   r->A.resize(m * n);
   r->B.resize(m * nrhs);
-  for (size_t j = 0; j < n; ++j) std::memcpy(&r->A[j * m], a + j * lda, sizeof(double) * m);
-  for (size_t j = 0; j < nrhs; ++j) std::memcpy(&r->B[j * m], b + j * ldb, sizeof(double) * m);
+  copy_columns(r->A.data(), a, m, n, lda);
+  copy_columns(r->B.data(), b, m, nrhs, ldb);
   r->bytes = footprint(r->m, r->n, r->nrhs, s->cfg.mode);
   r->info.t_submit = t_in;
   std::lock_guard<std::mutex> lk(s->mu);
