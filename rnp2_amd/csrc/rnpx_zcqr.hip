@@ -34,36 +34,37 @@ namespace rnpx {
 
 namespace zq {
 
-// The small factorizations: one 256-thread workgroup; thread t holds rows
-// (t & 7) + 8 r (r < 4) of column t >> 3 of the 32 x 32 matrix being worked
-// on in registers, so the eight lanes of a column sit in one wave and share
-// values by shuffles; a matrix other columns read is staged in LDS.  A
-// Cholesky or LU step costs one workgroup barrier, a triangular inverse or
-// product none.
-constexpr int kSmall = 256;
+// The small factorizations: one workgroup of LPC lanes per column; thread t
+// holds rows (t mod LPC) + LPC r (r < NB / LPC) of column t / LPC of the
+// 32 x 32 matrix being worked on in registers, so the lanes of a column sit
+// in one wave and share values by shuffles; a matrix other columns read is
+// staged in LDS.  A Cholesky or LU step costs one workgroup barrier, a
+// triangular inverse or product none.
 constexpr int NB = 32;
-constexpr int R4 = NB / 8;  // rows per thread
+constexpr int LPC = 8;  // lanes per column (measured: 4 and 16 are slower)
+constexpr int kSmall = NB * LPC;
+constexpr int R4 = NB / LPC;        // rows per thread
 using Mat = zd[NB][NB + 1];
 
 __device__ inline bool zfinite(zd v) { return isfinite(v.x) && isfinite(v.y); }
 __device__ inline double zabs2(zd v) { return v.x * v.x + v.y * v.y; }
 __device__ inline zd zconj(zd v) { return mkz(v.x, -v.y); }
 
-// v from lane src (0..7) of this lane's group of eight.
-__device__ inline zd gbcast(zd v, int src) { return mkz(__shfl(v.x, src, 8), __shfl(v.y, src, 8)); }
+// v from lane src (0..LPC-1) of this lane's column group.
+__device__ inline zd gbcast(zd v, int src) { return mkz(__shfl(v.x, src, LPC), __shfl(v.y, src, LPC)); }
 
 // Column c of the column-major NB x NB matrix at P (leading dimension ld).
 __device__ inline void load_col(const zd* __restrict__ P, long ld, int c, int sub, zd (&v)[R4]) {
 #pragma unroll
-  for (int r = 0; r < R4; ++r) v[r] = P[(size_t)(sub + 8 * r) + (size_t)c * ld];
+  for (int r = 0; r < R4; ++r) v[r] = P[(size_t)(sub + LPC * r) + (size_t)c * ld];
 }
 __device__ inline void store_col(zd* __restrict__ P, long ld, int c, int sub, const zd (&v)[R4]) {
 #pragma unroll
-  for (int r = 0; r < R4; ++r) P[(size_t)(sub + 8 * r) + (size_t)c * ld] = v[r];
+  for (int r = 0; r < R4; ++r) P[(size_t)(sub + LPC * r) + (size_t)c * ld] = v[r];
 }
 __device__ inline void to_lds(Mat& M, int c, int sub, const zd (&v)[R4]) {
 #pragma unroll
-  for (int r = 0; r < R4; ++r) M[sub + 8 * r][c] = v[r];
+  for (int r = 0; r < R4; ++r) M[sub + LPC * r][c] = v[r];
 }
 
 // In registers: column c of the Hermitian G (its upper part, rows <= c) ->
@@ -74,7 +75,7 @@ __device__ inline void to_lds(Mat& M, int c, int sub, const zd (&v)[R4]) {
 __device__ __forceinline__ bool chol_cols(zd (&g)[R4], int c, int sub, double rel, zd (*rowb)[NB], double* dinit) {
 #pragma unroll
   for (int r = 0; r < R4; ++r)
-    if (sub + 8 * r == c) dinit[c] = g[r].x;
+    if (sub + LPC * r == c) dinit[c] = g[r].x;
   if (sub == 0) rowb[0][c] = g[0];
   __syncthreads();
 #pragma unroll 1
@@ -86,7 +87,7 @@ __device__ __forceinline__ bool chol_cols(zd (&g)[R4], int c, int sub, double re
     const zd rjc = rb[c] * inv;  // R(j, c), c > j
 #pragma unroll
     for (int r = 0; r < R4; ++r) {
-      const int i = sub + 8 * r;
+      const int i = sub + LPC * r;
       if (i == j)
         g[r] = (c > j) ? rjc : (c == j ? mkz(rr, 0.0) : S<zd>::zero());
       else if (i > j && i <= c)
@@ -95,13 +96,13 @@ __device__ __forceinline__ bool chol_cols(zd (&g)[R4], int c, int sub, double re
     if (j + 1 < NB) {
 #pragma unroll
       for (int r = 0; r < R4; ++r)
-        if (sub + 8 * r == j + 1 && c >= j + 1) rowb[(j + 1) & 1][c] = g[r];
+        if (sub + LPC * r == j + 1 && c >= j + 1) rowb[(j + 1) & 1][c] = g[r];
     }
     __syncthreads();
   }
 #pragma unroll
   for (int r = 0; r < R4; ++r)
-    if (sub + 8 * r > c) g[r] = S<zd>::zero();
+    if (sub + LPC * r > c) g[r] = S<zd>::zero();
   return true;
 }
 
@@ -119,19 +120,19 @@ __device__ __forceinline__ void inv_upper_col(const Mat& U, int c, int sub, zd (
     __syncthreads();
   }
 #pragma unroll
-  for (int r = 0; r < R4; ++r) x[r] = (sub + 8 * r == c) ? S<zd>::one() : S<zd>::zero();
-  // k = 8 rk + s: rk unrolled (a fixed register), s not
+  for (int r = 0; r < R4; ++r) x[r] = (sub + LPC * r == c) ? S<zd>::one() : S<zd>::zero();
+  // k = LPC rk + s: rk unrolled (a fixed register), s not
 #pragma unroll
   for (int rk = R4 - 1; rk >= 0; --rk) {
 #pragma unroll 1
-    for (int s8 = 7; s8 >= 0; --s8) {
-      const int k = 8 * rk + s8;
+    for (int s8 = LPC - 1; s8 >= 0; --s8) {
+      const int k = LPC * rk + s8;
       zd xk = x[rk];
       if constexpr (!UNIT) xk = xk * rdiag[k];
       xk = gbcast(xk, s8);
 #pragma unroll
       for (int r = 0; r < R4; ++r) {
-        const int i = sub + 8 * r;
+        const int i = sub + LPC * r;
         if (i == k)
           x[r] = xk;
         else if (i < k)
@@ -149,12 +150,12 @@ __device__ __forceinline__ void mul_col(const Mat& A, const zd (&b)[R4], int sub
 #pragma unroll
   for (int rk = 0; rk < R4; ++rk) {
 #pragma unroll 2
-    for (int s8 = 0; s8 < 8; ++s8) {
-      const int k = 8 * rk + s8;
+    for (int s8 = 0; s8 < LPC; ++s8) {
+      const int k = LPC * rk + s8;
       const zd bk = gbcast(b[rk], s8);
 #pragma unroll
       for (int r = 0; r < R4; ++r) {
-        const zd aik = ds ? A[sub + 8 * r][k] * ds[k] : A[sub + 8 * r][k];
+        const zd aik = ds ? A[sub + LPC * r][k] * ds[k] : A[sub + LPC * r][k];
         out[r] = out[r] + aik * bk;
       }
     }
@@ -167,13 +168,13 @@ __global__ __launch_bounds__(kSmall) void chol_kernel(const zd* __restrict__ G, 
   __shared__ Mat Rs;
   __shared__ zd rowb[2][NB];
   __shared__ double dinit[NB];
-  const int t = threadIdx.x, c = t >> 3, sub = t & 7;
+  const int t = threadIdx.x, c = t / LPC, sub = t % LPC;
   zd g[R4];
   load_col(G, NB, c, sub, g);
   int bad = 0;
 #pragma unroll
   for (int r = 0; r < R4; ++r) {
-    const int i = sub + 8 * r;
+    const int i = sub + LPC * r;
     if (!zfinite(g[r])) bad = 1;
     if (i == c && !(g[r].x >= 0x1p-600 && g[r].x <= 0x1p600)) bad = 1;
   }
@@ -206,24 +207,24 @@ __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2
   __shared__ zd rowb[2][NB];
   __shared__ double dinit[NB];
   __shared__ double Ds[NB];
-  const int t = threadIdx.x, c = t >> 3, sub = t & 7;
+  const int t = threadIdx.x, c = t / LPC, sub = t % LPC;
   if (*fail) return;  // pass 1 rejected the panel (uniform)
   zd g[R4];
   load_col(G2, NB, c, sub, g);
   int bad = 0;
 #pragma unroll
   for (int r = 0; r < R4; ++r) {
-    const int i = sub + 8 * r;
+    const int i = sub + LPC * r;
     const zd eij = (i == c) ? mkz(g[r].x - 1.0, g[r].y) : g[r];
     if (!zfinite(g[r]) || zabs2(eij) > 0x1p-54) bad = 1;  // |E_ij| > 2^-27
   }
   // Q1's top rows to LDS (row-major) while the Gram is checked
 #pragma unroll
-  for (int r = 0; r < R4; ++r) S3[sub + 8 * r][c] = Q1[(size_t)(sub + 8 * r) + (size_t)c * ldq];
+  for (int r = 0; r < R4; ++r) S3[sub + LPC * r][c] = Q1[(size_t)(sub + LPC * r) + (size_t)c * ldq];
   int big = 0;  // |E_ij| > 2^-40: R2 by Cholesky
 #pragma unroll
   for (int r = 0; r < R4; ++r) {
-    const int i = sub + 8 * r;
+    const int i = sub + LPC * r;
     const zd eij = (i == c) ? mkz(g[r].x - 1.0, g[r].y) : g[r];
     if (zabs2(eij) > 0x1p-80) big = 1;
   }
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2
     // exact to O(|E|^2) <= 2^-80
 #pragma unroll
     for (int r = 0; r < R4; ++r) {
-      const int i = sub + 8 * r;
+      const int i = sub + LPC * r;
       if (i < c) {
         x2[r] = S<zd>::zero() - g[r];
       } else if (i == c) {
@@ -272,8 +273,8 @@ __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2
 #pragma unroll
   for (int rj = 0; rj < R4; ++rj)
 #pragma unroll 1
-  for (int s8 = 0; s8 < 8; ++s8) {
-    const int j = 8 * rj + s8;
+  for (int s8 = 0; s8 < LPC; ++s8) {
+    const int j = LPC * rj + s8;
     if (c == j) {
       const zd piv = gbcast(q[rj], s8);
       const double dj = piv.x > 0.0 ? -1.0 : 1.0;
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2
       const zd rp = zrecip(pivp);
 #pragma unroll
       for (int r = 0; r < R4; ++r) {
-        const int i = sub + 8 * r;
+        const int i = sub + LPC * r;
         if (i == j) {
           q[r] = pivp;
         } else if (i > j) {
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2
       const zd u = gbcast(q[rj], s8);  // U(j, c)
 #pragma unroll
       for (int r = 0; r < R4; ++r) {
-        const int i = sub + 8 * r;
+        const int i = sub + LPC * r;
         if (i > j) q[r] = q[r] - Lc[j & 1][i] * u;
       }
     }
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2
     mul_col(S1, r1, sub, rr);
 #pragma unroll
     for (int r = 0; r < R4; ++r) {
-      const int i = sub + 8 * r;
+      const int i = sub + LPC * r;
       Atop[(size_t)i + (size_t)c * NB] = (i <= c) ? rr[r] * Ds[i] : q[r];
       V[(size_t)i + (size_t)c * ldv] = (i > c) ? q[r] : (i == c ? S<zd>::one() : S<zd>::zero());
     }
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2
     mul_col(S2, ui, sub, mm);
 #pragma unroll
     for (int r = 0; r < R4; ++r)
-      if (sub + 8 * r > c) mm[r] = S<zd>::zero();
+      if (sub + LPC * r > c) mm[r] = S<zd>::zero();
     store_col(M, NB, c, sub, mm);
   }
   // T = -(U D) L1^{-H}: Y = L1^{-H}, the inverse of the unit upper L1^H
@@ -337,12 +338,12 @@ __global__ __launch_bounds__(kSmall) void recon_kernel(const zd* __restrict__ G2
 #pragma unroll
     for (int rl = 0; rl < R4; ++rl)
 #pragma unroll 2
-    for (int s8 = 0; s8 < 8; ++s8) {
-      const int l = 8 * rl + s8;
+    for (int s8 = 0; s8 < LPC; ++s8) {
+      const int l = LPC * rl + s8;
       const zd yl = gbcast(y[rl], s8);
 #pragma unroll
       for (int r = 0; r < R4; ++r) {
-        const int i = sub + 8 * r;
+        const int i = sub + LPC * r;
         u[r] = (l >= i) ? S4[i][l] * Ds[l] : S<zd>::zero();
         tt[r] = tt[r] + u[r] * yl;
       }
