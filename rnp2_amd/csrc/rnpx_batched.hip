@@ -184,25 +184,41 @@ __device__ void blocked_factor(int m, int n, T* a, long lda, T* tau, double* lds
       if (c >= pw || r <= c) Pn[e] = (r == c && c < pw) ? S<T>::one() : zero;
     }
     __syncthreads();
-    for (int e = t; e < PW * PW; e += NT) {
-      const int l = e / PW, c = e % PW;
-      T g = zero;
-      if (l < c && c < pw)
-        for (int r = c; r < mp; ++r) g = g + S<T>::cmul(Pn[r + l * mp], Pn[r + c * mp]);
-      Gs[l * TS + c] = g;
-      Tt[l * TS + c] = zero;
+    static_assert(PW == 16, "the Gram below is one 16 x 16 MFMA block");
+    if (wave == 0) {
+      // V's Gram V^H V on MFMA over the panel's rows (the clean V is zero
+      // above its unit diagonal, so the full sum is the reference's
+      // sum over rows >= c); lane item q holds G(lk + 4q, li)
+      const int li = lane & 15, lk = lane >> 4;
+      Acc g = AO::zero();
+      for (int ks = 0; ks < mp; ks += 4) {
+        const int r = ks + lk;
+        const T v = (r < mp) ? Pn[r + li * mp] : zero;
+        cmma(g, S<T>::conj(v), v);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int l = lk + 4 * q;
+        Gs[l * TS + li] = (l < li && li < pw) ? AO::get(g, q) : zero;
+        Tt[l * TS + li] = zero;
+      }
     }
     __syncthreads();
-    for (int c = 0; c < pw; ++c) {
-      const T tc = tau[p0 + c];
-      if (t < c) {
-        T sacc = zero;
-        for (int l = t; l < c; ++l) sacc = sacc + Tt[t * TS + l] * Gs[l * TS + c];
-        Tt[t * TS + c] = zero - tc * sacc;
+    // T column by column: row t of T reads only row t's earlier entries
+    // (written by this same thread) and the Gram, so no barrier per column
+    if (t < pw) {
+      for (int c = t; c < pw; ++c) {
+        const T tc = tau[p0 + c];
+        if (t == c) {
+          Tt[c * TS + c] = tc;
+        } else {
+          T sacc = zero;
+          for (int l = t; l < c; ++l) sacc = sacc + Tt[t * TS + l] * Gs[l * TS + c];
+          Tt[t * TS + c] = zero - tc * sacc;
+        }
       }
-      if (t == c) Tt[c * TS + c] = tc;
-      __syncthreads();
     }
+    __syncthreads();
     // Trailing columns [p0 + pw, n), rows [p0, m): 16-column tiles per wave.
     T* cp = a + (size_t)p0;  // row p0 of column 0
     const int jbeg = p0 + pw;
