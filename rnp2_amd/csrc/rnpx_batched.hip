@@ -230,14 +230,28 @@ __device__ void blocked_factor(int m, int n, T* a, long lda, T* tau, double* lds
 #pragma unroll
       for (int ib = 0; ib < NB2; ++ib) w[ib] = AO::zero();
       {
-        const int j = j0 + li;
-        for (int ks = 0; ks < mp; ks += 4) {
-          const int r = ks + lk;
-          const T cv = (r < mp && j < n) ? cp[(size_t)r + (size_t)j * lda] : zero;
+        // Branch-free loads (indices clamped into the matrix, out-of-range
+        // values selected to zero afterwards), eight row slices at a time so
+        // that their global loads are in flight together.
+        const int j = j0 + li, jc = min(j, n - 1);
+        const bool jok = j < n;
+        constexpr int U = 8;
+        for (int ks0 = 0; ks0 < mp; ks0 += 4 * U) {
+          T cv[U], vv[U][NB2];
 #pragma unroll
-          for (int ib = 0; ib < NB2; ++ib) {
-            const T vv = (r < mp) ? S<T>::conj(Pn[r + (16 * ib + li) * mp]) : zero;
-            cmma(w[ib], vv, cv);  // W = V^H C: a = V^H[i][k], b = C[k][j]
+          for (int u = 0; u < U; ++u) {
+            const int rc = min(ks0 + 4 * u + lk, mp - 1);
+            cv[u] = cp[(size_t)rc + (size_t)jc * lda];
+#pragma unroll
+            for (int ib = 0; ib < NB2; ++ib) vv[u][ib] = Pn[rc + (16 * ib + li) * mp];
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const bool rok = ks0 + 4 * u + lk < mp;
+            const T c = (rok && jok) ? cv[u] : zero;
+#pragma unroll
+            for (int ib = 0; ib < NB2; ++ib)
+              cmma(w[ib], rok ? S<T>::conj(vv[u][ib]) : zero, c);  // W = V^H C: a = V^H[i][k], b = C[k][j]
           }
         }
       }
@@ -253,25 +267,30 @@ __device__ void blocked_factor(int m, int n, T* a, long lda, T* tau, double* lds
             cmma(z[ib], tv, AO::get(w[kb], sl));
           }
       }
+#pragma unroll 2
       for (int r0 = 0; r0 < mp; r0 += 16) {
-        const int rr = r0 + li;
+        const int rr = r0 + li, rc = min(rr, mp - 1);
+        const bool rok = rr < mp;
         Acc acc = AO::zero();
+        T cl[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int j = j0 + lk + 4 * q;
-          AO::set(acc, q, (rr < mp && j < n) ? cp[(size_t)rr + (size_t)j * lda] : zero);
-        }
+        for (int q = 0; q < 4; ++q) cl[q] = cp[(size_t)rc + (size_t)min(j0 + lk + 4 * q, n - 1) * lda];
+        T vl[NB2][4];
 #pragma unroll
         for (int kb = 0; kb < NB2; ++kb)
 #pragma unroll
-          for (int sl = 0; sl < 4; ++sl) {
-            const T vv = (rr < mp) ? Pn[rr + (16 * kb + 4 * sl + lk) * mp] : zero;
-            cmma(acc, zero - AO::get(z[kb], sl), vv);  // C^T -= Z^T V^T
-          }
+          for (int sl = 0; sl < 4; ++sl) vl[kb][sl] = Pn[rc + (16 * kb + 4 * sl + lk) * mp];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) AO::set(acc, q, (rok && j0 + lk + 4 * q < n) ? cl[q] : zero);
+#pragma unroll
+        for (int kb = 0; kb < NB2; ++kb)
+#pragma unroll
+          for (int sl = 0; sl < 4; ++sl)
+            cmma(acc, zero - AO::get(z[kb], sl), rok ? vl[kb][sl] : zero);  // C^T -= Z^T V^T
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int j = j0 + lk + 4 * q;
-          if (rr < mp && j < n) cp[(size_t)rr + (size_t)j * lda] = AO::get(acc, q);
+          if (rok && j < n) cp[(size_t)rr + (size_t)j * lda] = AO::get(acc, q);
         }
       }
     }
