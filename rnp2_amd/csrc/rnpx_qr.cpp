@@ -266,7 +266,11 @@ static int pick_nb(const Handle* h, int m, int n) {
   if (nb == 32 || nb == 64 || (!S<T>::cplx && nb == 128)) {
     // explicit choice
   } else {
-    nb = 64;
+    // Untuned: 64-column panels for real matrices (the CholeskyQR2 block
+    // kernel); 32 for complex ones, the only width with a CholeskyQR2 panel
+    // (64-column complex panels go column by column, about three times
+    // slower at 4096^2).
+    nb = S<T>::cplx ? 32 : 64;
   }
   // The T-factor kernel holds NB x NB blocks in LDS: 64 is the largest built.
   if (nb > 64) nb = 64;
@@ -292,7 +296,10 @@ static int pick_rpt(const Handle* h, int mp, int nb, bool cplx) {
 }
 
 static int pick_ksplit(const Handle* h, int rows, int cols_tiles) {
-  if (h->tune.ksplit > 0) return h->tune.ksplit;
+  // A tuned split applies to the wide products (the trailing update's W);
+  // the narrow ones keep the latency-oriented split below whatever the
+  // tuning says.
+  if (h->tune.ksplit > 0 && cols_tiles > 2) return h->tune.ksplit;
   // About 512 workgroup tiles of 128 x 128 in all (two per CU; measured best
   // for W = V^H C at 4096-8192 rows).  K slices of at least 512, or 128 for
   // the narrow products (one or two column tiles) that sit on the chain of
@@ -525,7 +532,7 @@ int geqrf_dev(Handle* h, int m, int n, T* A, long lda, T* tau, T* B, int nrhs, l
     G0 = (m + 256 * rpt0 - 1) / (256 * rpt0);
   }
   if (G0 > gcap) return RNPX_ERR_UNSUPPORTED;
-  const int nbo = (h->tune.nb == 256 ? 4 : (h->tune.nb == 0 || h->tune.nb == 128) ? 2 : 1) * nb;
+  const int nbo = (h->tune.nb == 256 ? 4 : ((h->tune.nb == 0 && !S<T>::cplx) || h->tune.nb == 128) ? 2 : 1) * nb;
   // Outer blocks: nbo columns while the remaining height exceeds wide_rows
   // (where the trailing products are the longer chain and K = nbo runs them
   // faster), 2 nb columns below it (where the panels are the longer chain).

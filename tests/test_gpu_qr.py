@@ -208,6 +208,22 @@ def test_reference_test_identities():
         assert max(errs) < 30.0
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_untuned_defaults(tmp_path, cplx):
+    """With no factorization entry in the tuning table the built-in widths
+    apply (64-column real panels in 128-column blocks, 32-column complex
+    CholeskyQR2 panels); they match the oracle like the tuned ones."""
+    p = tmp_path / "batched_only.json"
+    p.write_text('{"version": 1, "entries": [{"kind": "batched", "dtype": "d", "lds": 20480, "blocked_min": 33}]}')
+    assert R.tuning_load(str(p)) == 1
+    try:
+        assert R.tuning_lookup_factor("z" if cplx else "d", 2048, 1024) is None
+        a = seeded(2048, 1024, cplx, seed=77)
+        check_factor(a, f"untuned 2048x1024 cplx={cplx}")
+    finally:
+        R.lib().rnpx_tuning_load(None)
+
+
 def test_profile_families():
     """Live kernel timing selects kernel families: with only the trailing
     update instrumented, no W = V^H C launch is recorded, and enabling resets
