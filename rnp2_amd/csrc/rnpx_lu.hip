@@ -154,6 +154,11 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuPanelArgs a) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
+    // Old row c does not depend on the pivot: its loads go out with the
+    // candidates' (one round trip fewer per column).
+    static_assert(NB <= 256, "a thread per entry of row c");
+    T ocv = zero;
+    if (tid < NB) ocv = ld_part<T>(rowc + TE * tid);
     {
       double v = -2.0;
       int i = 0x7fffffff, w = -1;
@@ -183,6 +188,7 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuPanelArgs a) {
         redi[wave] = i;
         redw[wave] = w;
       }
+      if (tid < NB) ocbuf[tid] = ocv;
       __syncthreads();
       if (tid == 0) {
         double bv2 = redv[0];
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuPanelArgs a) {
             bw2 = redw[q];
           }
         // The sequential scan starts from row c: a NaN there is kept.
-        const T xc = ld_part<T>(rowc + TE * c);
+        const T xc = ocbuf[c];
         const double nc = norm1_of(xc);
         if (nc != nc || bi2 == 0x7fffffff) {
           bi2 = c;
@@ -206,12 +212,8 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuPanelArgs a) {
       __syncthreads();
     }
     const int p = pw[0], w = pw[1];
-    // 3. pivot row u (the winner's row, or row c itself), old row c
-    const double* urow = (w >= 0) ? part + (size_t)w * kLuSlot + 2 : rowc;
-    for (int q = tid; q < NB; q += 256) {
-      ubuf[q] = ld_part<T>(urow + TE * q);
-      ocbuf[q] = ld_part<T>(rowc + TE * q);
-    }
+    // 3. pivot row u: the winner's row, or row c itself (already in ocbuf)
+    if (tid < NB) ubuf[tid] = (w >= 0) ? ld_part<T>(part + (size_t)w * kLuSlot + 2 + TE * tid) : ocbuf[tid];
     __syncthreads();
     if (c >= gr0 && c < gr0 + RW)
       for (int q = tid; q < NB; q += 256) Sl[c - gr0][q] = ubuf[q];
