@@ -33,6 +33,8 @@
 // and a store instruction writes 128 contiguous bytes per column.
 #include "rnpx_kernels.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cstdint>
 
@@ -274,8 +276,9 @@ __global__ __launch_bounds__(kThreads) void dgemm_kernel(Args a0) {
       }
 }
 
+// ev0/ev1 (optional): start and stop events recorded by the launch itself.
 template <bool AKC, bool BKC, int BM, int BN, int BK>
-static hipError_t launch(Args a, hipStream_t st, int batch = 1) {
+static hipError_t launch(Args a, hipStream_t st, int batch = 1, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
   // 16-byte loads: both operands aligned with even leading dimensions, and an
   // even extent along each operand's pairs (rows of op(A) or op(B) for an
   // operand contiguous along the output index, K for one contiguous along K)
@@ -288,7 +291,13 @@ static hipError_t launch(Args a, hipStream_t st, int batch = 1) {
   const dim3 grid((unsigned)a.ntiles, (unsigned)std::max(1, batch)), block(kThreads);
 #define RNPX_DG_LAUNCH(MODE_)                                                                         \
   do {                                                                                                \
-    if (vec)                                                                                          \
+    if (ev0 && vec)                                                                                   \
+      hipExtLaunchKernelGGL((dgemm_kernel<MODE_, true, AKC, BKC, BM, BN, BK>), grid, block, 0, st,    \
+                            ev0, ev1, 0, a);                                                          \
+    else if (ev0)                                                                                     \
+      hipExtLaunchKernelGGL((dgemm_kernel<MODE_, false, AKC, BKC, BM, BN, BK>), grid, block, 0, st,   \
+                            ev0, ev1, 0, a);                                                          \
+    else if (vec)                                                                                     \
       hipLaunchKernelGGL((dgemm_kernel<MODE_, true, AKC, BKC, BM, BN, BK>), grid, block, 0, st, a);   \
     else                                                                                              \
       hipLaunchKernelGGL((dgemm_kernel<MODE_, false, AKC, BKC, BM, BN, BK>), grid, block, 0, st, a);  \
@@ -537,7 +546,7 @@ hipError_t sgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const
 // the general kernel).
 hipError_t dgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const double* A, long lda, const double* B,
                       long ldb, double beta, double* C, long ldc, int kslice, int ns, double* part, int cfg,
-                      hipStream_t st) {
+                      hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   const bool akc = (ta == 'T' || ta == 'C');
   if (tb != 'N' || !(akc || ta == 'N')) return hipErrorNotSupported;
   constexpr int BK = 16;
@@ -582,15 +591,15 @@ hipError_t dgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const
   }
   if (akc) {
     switch (cfg) {
-      case 1: return dg::launch<true, true, 128, 128, BK>(a, st);
-      case 2: return dg::launch<true, true, 128, 64, BK>(a, st);
-      default: return dg::launch<true, true, 64, 64, BK>(a, st);
+      case 1: return dg::launch<true, true, 128, 128, BK>(a, st, 1, ev0, ev1);
+      case 2: return dg::launch<true, true, 128, 64, BK>(a, st, 1, ev0, ev1);
+      default: return dg::launch<true, true, 64, 64, BK>(a, st, 1, ev0, ev1);
     }
   }
   switch (cfg) {
-    case 1: return dg::launch<false, true, 128, 128, BK>(a, st);
-    case 2: return dg::launch<false, true, 128, 64, BK>(a, st);
-    default: return dg::launch<false, true, 64, 64, BK>(a, st);
+    case 1: return dg::launch<false, true, 128, 128, BK>(a, st, 1, ev0, ev1);
+    case 2: return dg::launch<false, true, 128, 64, BK>(a, st, 1, ev0, ev1);
+    default: return dg::launch<false, true, 64, 64, BK>(a, st, 1, ev0, ev1);
   }
 }
 

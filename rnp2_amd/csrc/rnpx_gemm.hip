@@ -304,7 +304,7 @@ static int dgemm_cfg_override() {
 
 template <typename T>
 hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long lda, const T* B, long ldb, T beta,
-                T* C, long ldc, int ksplit, T* part, hipStream_t st) {
+                T* C, long ldc, int ksplit, T* part, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   if (M < 0 || N < 0 || K < 0) return hipErrorInvalidValue;
   if (M == 0 || N == 0) return hipSuccess;
   if (ldc < M) return hipErrorInvalidValue;
@@ -324,15 +324,30 @@ hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long
   if (K == 0) kslice = 0;
   T* pp = (ns > 1) ? part : nullptr;
   bool done = false;
+  // The timing events ride on the launch when the product is one FP64 MFMA
+  // launch; otherwise they are recorded around the product's kernels.
+  bool ride = false;
+  if constexpr (!S<T>::cplx) ride = ev0 && ev1 && ns == 1 && tb == 'N' && dgemm_cfg_override() >= 0;
+  if (ev0 && ev1 && !ride) {
+    const hipError_t e = hipEventRecord(ev0, st);
+    if (e != hipSuccess) return e;
+  }
   if constexpr (!S<T>::cplx) {
     // Real op(A) N/T/C times op(B) N: the double-buffered 128 x 128 kernel.
     const int cfg = dgemm_cfg_override();
     if (cfg >= 0 && tb == 'N') {
-      const hipError_t e = dgemm_mfma(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, pp, cfg, st);
+      const hipError_t e = dgemm_mfma(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kslice, ns, pp, cfg, st,
+                                      ride ? ev0 : nullptr, ride ? ev1 : nullptr);
       if (e != hipErrorNotSupported) {
         if (e != hipSuccess) return e;
         done = true;
       }
+    }
+    if (!done && ride) {
+      // not launched there: the events go around the general kernel
+      const hipError_t e = hipEventRecord(ev0, st);
+      if (e != hipSuccess) return e;
+      ride = false;
     }
   } else {
     // Complex op(A) N/T/C times op(B) N: the double-buffered complex kernel
@@ -364,12 +379,16 @@ hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long
     hipLaunchKernelGGL((gemm_reduce_kernel<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, M, N, ns,
                        part, alpha, beta, C, ldc);
   }
+  if (ev0 && ev1 && !ride) {
+    const hipError_t e = hipEventRecord(ev1, st);
+    if (e != hipSuccess) return e;
+  }
   return hipGetLastError();
 }
 
 template hipError_t gemm<double>(char, char, int, int, int, double, const double*, long, const double*, long, double,
-                                 double*, long, int, double*, hipStream_t);
+                                 double*, long, int, double*, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t gemm<zd>(char, char, int, int, int, zd, const zd*, long, const zd*, long, zd, zd*, long, int, zd*,
-                             hipStream_t);
+                             hipStream_t, hipEvent_t, hipEvent_t);
 
 }  // namespace rnpx

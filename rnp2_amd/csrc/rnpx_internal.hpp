@@ -100,6 +100,11 @@ struct Handle {
   // Bracket one launch: call prof_begin before and prof_end after.
   int prof_begin(int kid, hipEvent_t* a, hipStream_t s = nullptr);
   void prof_end(int kid, hipEvent_t a, double flops, double bytes, hipStream_t s = nullptr);
+  // Or: two timing events that the launch itself records (a kernel launched
+  // with start and stop events, no marker packets around it), then
+  // prof_push; both null when kid is not timed.
+  void prof_take(int kid, hipEvent_t* a, hipEvent_t* b);
+  void prof_push(int kid, hipEvent_t a, hipEvent_t b, double flops, double bytes);
   int ensure_pipeline();
   int ensure_partition(int cus);
   size_t held_bytes() const;

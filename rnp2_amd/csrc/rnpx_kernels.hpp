@@ -288,19 +288,23 @@ hipError_t launch_gram(int n, int k, const T* V, long ldv, const T* tau, T* gram
 // With ksplit > 1 the K range is cut into ksplit slices whose raw products go
 // to part[s] (M x N, ld M) and a second kernel sums them in slice order and
 // applies alpha/beta (deterministic).  part must hold ksplit*M*N elements.
+// ev0/ev1 (optional timing events) are recorded at the start and the end of
+// the product: carried by the kernel launch itself where the product is one
+// FP64 MFMA launch, as marker packets around its kernels otherwise.
 template <typename T>
 hipError_t gemm(char ta, char tb, int M, int N, int K, T alpha, const T* A, long lda,
                 const T* B, long ldb, T beta, T* C, long ldc, int ksplit, T* part,
-                hipStream_t st);
+                hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // The real trailing-update product on FP64 MFMA with LDS double-buffered
 // 128 x 128 tiles (rnpx_dgemm.hip): op(A) 'N', 'T' or 'C', op(B) 'N'; ns > 1
 // writes the raw products of K slices of kslice (a multiple of 16) to part
 // (M x N each, leading dimension M); cfg 0 picks the tile (1: 128 x 128,
-// 2: 128 x 64, 3: 64 x 64).  hipErrorNotSupported for other operand forms.
+// 2: 128 x 64, 3: 64 x 64).  hipErrorNotSupported for other operand forms
+// (nothing launched).  ev0/ev1: start and stop events the launch records.
 hipError_t dgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const double* A, long lda, const double* B,
                       long ldb, double beta, double* C, long ldc, int kslice, int ns, double* part, int cfg,
-                      hipStream_t st);
+                      hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // The complex counterpart (rnpx_zgemm.hip): op(A) N/T/C, op(B) N.
 hipError_t zgemm_mfma(char ta, char tb, int M, int N, int K, zd alpha, const zd* A, long lda, const zd* B, long ldb,
                       zd beta, zd* C, long ldc, int kslice, int ns, zd* part, hipStream_t st);

@@ -106,6 +106,20 @@ void Handle::prof_end(int kid, hipEvent_t a, double flops, double bytes, hipStre
   hipEventRecord(b, s ? s : stream);
   recs.push_back(ProfRec{kid, a, b, flops, bytes});
 }
+void Handle::prof_take(int kid, hipEvent_t* a, hipEvent_t* b) {
+  *a = *b = nullptr;
+  if (!prof || !((prof_mask >> kid) & 1u)) return;
+  *a = take_event();
+  *b = take_event();
+  if (!*a || !*b) {
+    for (hipEvent_t e : {*a, *b})
+      if (e) event_pool.push_back(e);
+    *a = *b = nullptr;
+  }
+}
+void Handle::prof_push(int kid, hipEvent_t a, hipEvent_t b, double flops, double bytes) {
+  if (a && b) recs.push_back(ProfRec{kid, a, b, flops, bytes});
+}
 // A stream with a CU mask (here every CU) gets a hardware queue of its own,
 // where a plain one may share a queue with other streams of the process once
 // they outnumber the runtime's queues -- with the update stream among them,
@@ -361,12 +375,18 @@ static int apply_block_dev(Handle* h, char side, char trans, int mc, int nc, int
     if (!f32) RNPX_TRY(gemm<T>('C', 'N', k, nc, mc, one, V, ldv, C, ldc, zero, W, k, ks, (T*)pb.p, st));
     if (gw->prof) h->prof_end(K_WGEMM, ev, fl, esz * ((double)mc * nc + (double)mc * k + (double)k * nc), st);
     RNPX_TRY(gemm<T>(trans == 'N' ? 'N' : 'C', 'N', k, nc, k, one, Tm, ldt, W, k, zero, Z, k, 1, nullptr, st));
-    if (gw->prof) h->prof_begin(K_UPDATE, &ev, st);
+    // The update's timing events ride on its launch (none between kernels).
+    hipEvent_t u0 = nullptr, u1 = nullptr;
+    if (gw->prof) h->prof_take(K_UPDATE, &u0, &u1);
     if constexpr (!S<T>::cplx) {
-      if (f32) RNPX_TRY(gemm32('N', 'N', mc, nc, k, -1.0, V, ldv, Z, k, 1.0, C, ldc, 1, nullptr, st));
+      if (f32) {
+        if (u0) RNPX_TRY(hipEventRecord(u0, st));
+        RNPX_TRY(gemm32('N', 'N', mc, nc, k, -1.0, V, ldv, Z, k, 1.0, C, ldc, 1, nullptr, st));
+        if (u0) RNPX_TRY(hipEventRecord(u1, st));
+      }
     }
-    if (!f32) RNPX_TRY(gemm<T>('N', 'N', mc, nc, k, mone, V, ldv, Z, k, one, C, ldc, 1, nullptr, st));
-    if (gw->prof) h->prof_end(K_UPDATE, ev, fl, esz * (2.0 * mc * nc + (double)mc * k + (double)k * nc), st);
+    if (!f32) RNPX_TRY(gemm<T>('N', 'N', mc, nc, k, mone, V, ldv, Z, k, one, C, ldc, 1, nullptr, st, u0, u1));
+    h->prof_push(K_UPDATE, u0, u1, fl, esz * (2.0 * mc * nc + (double)mc * k + (double)k * nc));
   } else {
     // W = C V (mc x k), Z = W op(T), C -= Z V^H
     const int tiles = (mc + 127) / 128;
