@@ -291,8 +291,10 @@ int rnpx_synchronize(rnpx_handle h);
 /* Bytes of device workspace the handle currently holds. */
 size_t rnpx_workspace_bytes(rnpx_handle h);
 
-/* Live kernel timing on the handle's stream (HIP events around each launch
- * of a kernel family).  kernel: 0 trailing update C -= V Z, 1 W = V^H C,
+/* Live kernel timing on the handle's streams (HIP events around each launch
+ * of a kernel family; the FP64 trailing update's launch records its own
+ * start and stop events, so its time is the kernel's alone).
+ * kernel: 0 trailing update C -= V Z, 1 W = V^H C,
  * 2 panel, 3 T factor.  on: 0 off, 1 every family, otherwise bit (k + 1)
  * of on selects family k (fewer events perturb the timed work less).
  * Enabling resets the records.  flops/bytes are the algorithmic counts of
