@@ -71,3 +71,38 @@ def test_committed_table_is_valid():
     assert R.tuning_load(path) > 0
     assert R.tuning_lookup_factor("d", 8192, 8192) is not None
     assert R.tuning_lookup_batched("d") is not None
+
+
+def test_pick_factor_prefers_shared_cus_within_margin():
+    """The autotuner keeps the CUs shared unless a partition wins by more
+    than the margin."""
+    from rnp2_amd.autotune import pick_factor
+    res = [{"nb": 128, "ksplit": 8, "panel_cus": 0, "ms": 28.40},
+           {"nb": 128, "ksplit": 8, "panel_cus": 64, "ms": 28.25},
+           {"nb": 64, "ksplit": 0, "panel_cus": 0, "ms": 29.90}]
+    assert pick_factor(res)["panel_cus"] == 0                      # 0.5% slower: shared kept
+    res[1]["ms"] = 27.50
+    assert pick_factor(res) == res[1]                              # 3% faster: partition taken
+    assert pick_factor(res, margin=0.05)["panel_cus"] == 0         # within a 5% margin again
+    only = [{"nb": 32, "ksplit": 16, "panel_cus": 64, "ms": 26.2}, {"nb": 32, "ksplit": 4, "panel_cus": 32, "ms": 27.0}]
+    assert pick_factor(only) == only[0]                            # no shared candidate: the fastest
+
+
+def test_written_table_loads_back(tmp_path, monkeypatch):
+    """A table written by the autotuner is read back by the library's loader
+    with the same widths, splits, panel CUs and batched settings."""
+    import rnp2_amd.autotune as T
+    monkeypatch.setattr(T, "device_name", lambda: "test device")
+    entries = [{"kind": "factor", "dtype": "d", "m": 8192, "n": 8192, "nb": 128, "ksplit": 8, "panel_cus": 0, "ms": 28.5},
+               {"kind": "factor", "dtype": "z", "m": 4096, "n": 4096, "nb": 32, "ksplit": 16, "panel_cus": 64,
+                "ms": 26.2},
+               {"kind": "batched", "dtype": "d", "lds": 20480, "blocked_min": 33, "ms": 4.8}]
+    path = T.write_table(entries, path=str(tmp_path / "t.json"), extra={"candidates": {"x": [{"ms": 1.0}]}})
+    doc = json.loads(open(path).read())
+    assert doc["device"] == "test device" and doc["entries"] == entries
+    assert R.tuning_load(path) == 3
+    assert R.tuning_lookup_factor("d", 8192, 8192) == (128, 8)
+    assert R.tuning_lookup_factor("z", 4096, 4096) == (32, 16)
+    assert R.tuning_lookup_panel_cus("z", 4096, 4096) == 64
+    assert R.tuning_lookup_panel_cus("d", 8192, 8192) == 0
+    assert R.tuning_lookup_batched("d") == (20480, 33)
