@@ -208,6 +208,18 @@ def test_bench_refuses_multi_gpu():
     assert r.returncode == 2 and "one GPU" in r.stderr
 
 
+def test_bench_reads_committed_traffic():
+    """The roofline's traffic field comes from the committed counter summary
+    of the 8192^2 workload: a per-launch byte count at least the algorithmic
+    one (8 (2 mc nc + mc k + k nc) averaged over the launches, 3.69e8 B)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    t = bench.update_traffic_bytes()
+    assert t is not None and 3.69e8 <= t <= 3.0 * 3.69e8, t
+
+
 def test_cpp_lq_surface_compiles_and_queries():
     """include/rnpx_lq.hpp: LQ's template signatures and workspace queries
     (LQ.hpp:135-138 nb*n, 329-333 nb*nw + nb^2, 500-504 n*nb)."""
